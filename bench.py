@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Throughput of the river-reach solver on one MI355X.
+
+A step is one solver step of the whole reach (fused collide-stream kernel,
+inlet/outlet rows) inside the run driver's 20-step cadence: every chunk of 20
+steps also refreshes the wall-model reference speed (column reduction) and
+adds the time-mean sums, as a run does in its averaging window. The
+workload is the 1500 x 522 x 30 trapezoidal reach with one pusher tug
+(23.49 M cells, 20.64 M fluid), FP32 populations, synthetic inputs built by
+the case builder; all state is resident in GPU memory before timing starts.
+
+Prints one JSON line: MLUPS (all cells, the reference's convention), the
+step kernel's HBM roofline fraction from HIP events around every launch on
+the solver's stream, and a CPU baseline (the OpenMP oracle in its
+Numba-like precision, on a bounded sample of the same reach).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS & % HBM roofline, plain FP32 vs FP16-store/batched/learned SGS"
+PEAK_HBM_GBS = 8000.0                       # MI355X HBM3E
+# algorithmic bytes per fluid-cell update: 19 populations read + 19 written + 1 flag byte
+B_ALG = {"f32": 19 * 4 * 2 + 1, "f16dev": 19 * 2 * 2 + 1, "bf16dev": 19 * 2 * 2 + 1, "f64": 19 * 8 * 2 + 1}
+PUBLISHED_MLUPS = 900.0                     # reference CUDA path, RTX 5070 Laptop (top of 550-900)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_f32_trapezoid_tug_traffic.txt")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_case(name):
+    from water_flow_simulation_amd import workloads
+    return getattr(workloads, name)()
+
+
+def solver_args(reach):
+    nu_lb = max(1e-6 * reach.dt / reach.dx ** 2, 2e-4)
+    return (reach.solid, reach.Fx, reach.Fy, reach.Fz, (reach.uin_phys / reach.vel).astype(np.float32),
+            0.5 + 3 * nu_lb, reach.cfg.cs, reach.cf)
+
+
+def chunks(lbm, nsteps, cadence=20):
+    k = 0
+    while k < nsteps:
+        m = min(cadence, nsteps - k)
+        lbm.update_uref()
+        lbm.step(m)
+        lbm.accumulate()
+        k += m
+
+
+def time_gpu(reach, storage, steps, warmup, block_x):
+    from water_flow_simulation_amd.lbm3d import LBM3D
+    lbm = LBM3D(None, *solver_args(reach), storage=storage, block_x=block_x)
+    lbm.set_bedwall(reach.bedwall)
+    chunks(lbm, warmup)
+    lbm.synchronize()
+    lbm.check_finite()
+    lbm.reset_sums()
+    lbm.timing(True)
+    lbm.synchronize()
+    t0 = time.perf_counter()
+    chunks(lbm, steps)
+    lbm.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms, launches = lbm.timing_read()
+    lbm.timing(False)
+    lbm.check_finite()
+    mass = lbm.total("rho")
+    lbm.close()
+    return dict(wall=wall, kernel_ms=kernel_ms, launches=launches, mass=mass)
+
+
+def cpu_baseline(reach, target_s=15.0):
+    """The oracle's Numba-like instantiation (float storage, double arithmetic,
+    OpenMP over cells) on a few steps of the same reach."""
+    from oracle import oracle as O
+    args = solver_args(reach)
+    orc = O.OracleLBM(*args, precision="mixed")
+    t0 = time.perf_counter()
+    orc.step(1)
+    one = time.perf_counter() - t0
+    n = int(min(max(target_s / max(one, 1e-3), 2), 40))
+    t0 = time.perf_counter()
+    orc.step(n)
+    dt = time.perf_counter() - t0
+    return dict(value=n * reach.cells / dt / 1e6, unit="MLUPS", cores=O.num_threads(), kind="port",
+                sample=f"{n} steps of the full {reach.nx}x{reach.ny}x{reach.nz} reach (all cells counted)")
+
+
+def traffic_per_launch():
+    """HBM bytes per step-kernel launch from the committed rocprofv3 counter summary, if any."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    m = re.search(r"hbm_bytes_per_launch\s*[=:]\s*([0-9.eE+]+)", open(TRAFFIC_FILE).read())
+    return float(m.group(1)) if m else None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="trapezoid_tug")
+    ap.add_argument("--storage", default="f32", choices=sorted(B_ALG))
+    ap.add_argument("--block-x", type=int, default=0)
+    ap.add_argument("--variants", default="f16dev,bf16dev", help="extra storage formats timed (comma list)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the solver has no multi-GPU path in this build "
+              "(x-slab decomposition is design only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    log(f"building workload {a.workload}")
+    reach = build_case(a.workload)
+    n_fluid = int((~reach.solid).sum())
+    log(f"{reach.nx}x{reach.ny}x{reach.nz} = {reach.cells} cells ({n_fluid} fluid); timing {a.steps} steps")
+    r = time_gpu(reach, a.storage, a.steps, a.warmup, a.block_x)
+    log(f"timed: {r['wall']:.3f} s wall, step kernel {r['kernel_ms'] / max(r['launches'], 1):.4f} ms avg")
+    mlups = a.steps * reach.cells / r["wall"] / 1e6
+    k_avg_s = r["kernel_ms"] / max(r["launches"], 1) / 1e3
+    achieved = B_ALG[a.storage] * n_fluid / k_avg_s / 1e9
+    out = {
+        "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(r["wall"] / a.steps * 1e3, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": round(mlups / PUBLISHED_MLUPS, 2), "dtype": "fp32" if a.storage != "f64"
+        else "fp64", "data": "synthetic",
+        "config": {"workload": f"{a.workload}: trapezoidal reach 40 m bed, 1:2 banks, 3 m deep, 150 m at dx 0.1 m, "
+                               f"{reach.nx}x{reach.ny}x{reach.nz} cells, one pusher tug (2 x 1.2 m props, 300 kW)",
+                   "storage": a.storage, "cells": reach.cells, "fluid_cells": n_fluid, "cadence": 20,
+                   "block_x": a.block_x or 128},
+        "mlups_fluid": round(a.steps * n_fluid / r["wall"] / 1e6, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_per_launch(),
+                     "kernel": "k_step", "kernel_ms_avg": round(k_avg_s * 1e3, 5),
+                     "bytes_alg_per_launch": B_ALG[a.storage] * n_fluid},
+        "mass_fp64": r["mass"],
+    }
+    variants = {}
+    for st in [v for v in a.variants.split(",") if v and v != a.storage]:
+        log(f"timing variant {st}")
+        rv = time_gpu(reach, st, a.steps, a.warmup, a.block_x)
+        kv = rv["kernel_ms"] / max(rv["launches"], 1) / 1e3
+        variants[st] = {"mlups": round(a.steps * reach.cells / rv["wall"] / 1e6, 1),
+                        "kernel_ms_avg": round(kv * 1e3, 5),
+                        "roofline_frac": round(B_ALG[st] * n_fluid / kv / 1e9 / PEAK_HBM_GBS, 4),
+                        "speedup_vs_f32": round(r["wall"] / rv["wall"], 3)}
+    if variants:
+        out["variants"] = variants
+    if not a.no_cpu_baseline:
+        log("timing the CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(reach)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,165 @@
+/* C-ABI of the MI355X river-reach lattice-Boltzmann solver (libwfs.so).
+ *
+ * One opaque handle = one solver instance = the state of the reference's
+ * `LBM3D` object (floodsim/lbm3d.py:333-418) held in GPU memory, plus the
+ * device-side pieces of its driver `Run3D` (floodsim/reach3d.py:378-453).
+ * The reference has no FFI of its own: its `LBM3D` methods launch CuPy
+ * kernels directly. Each entry point below replaces the reference code named
+ * in its comment; the Python class water_flow_simulation_amd.lbm3d.LBM3D
+ * binds them with ctypes (see INTEGRATION.md for the binding).
+ *
+ * Conventions
+ *  - Arrays are flat, C order, cell = x + nx*(y + ny*z) (lbm3d.py:9);
+ *    per-row arrays (uin) are indexed j = y + ny*z.
+ *  - Every function returns 0 on success or a negative WFS_E* code;
+ *    wfs_last_error() gives the message of the calling thread's last failure.
+ *  - The caller owns host buffers: inputs are copied in, outputs copied out.
+ *  - Work is queued on the handle's own HIP stream; wfs_step() returns before
+ *    the GPU has finished, every wfs_get_* call waits for it.
+ *  - One handle must not be used from two threads at once; distinct handles
+ *    are independent.
+ */
+#ifndef WFS_H
+#define WFS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WFS_ABI_VERSION 1
+
+enum wfs_status {
+    WFS_OK = 0,
+    WFS_EINVAL = -1,   /* bad argument (maps to TypeError/ValueError) */
+    WFS_ENOMEM = -2,   /* device allocation failed */
+    WFS_EHIP = -3,     /* HIP runtime error */
+    WFS_ENODEV = -4,   /* no usable GPU */
+    WFS_ENONFINITE = -5 /* watchdog: non-finite state found */
+};
+
+/* Population storage. F32 is the reference's layout and precision; the two
+ * 16-bit formats store (f_i - w_i) * 2^8 and compute in FP32; F64 stores and
+ * computes in double (the FP64 run of the same path). */
+enum wfs_storage { WFS_STORE_F32 = 0, WFS_STORE_F16DEV = 1, WFS_STORE_BF16DEV = 2, WFS_STORE_F64 = 3 };
+
+typedef struct wfs_desc {
+    int32_t nx, ny, nz;   /* lattice size; nx >= 3, nx*ny*nz < 2^31 */
+    float tau0;           /* molecular relaxation time (reach3d.py:383-385) */
+    double cs;            /* Smagorinsky constant; the step uses Cs^2 = (float)(cs*cs) */
+    int32_t storage;      /* enum wfs_storage */
+    int32_t block_x;      /* work-group size along x (64..256, multiple of 64); 0 = default */
+    int32_t device;       /* HIP device ordinal */
+} wfs_desc;
+
+typedef struct wfs_solver wfs_solver;
+
+/* Version of this ABI (WFS_ABI_VERSION). */
+int wfs_abi_version(void);
+
+/* Number of visible HIP devices (0 if none). */
+int wfs_device_count(int32_t* count);
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* wfs_last_error(void);
+
+/* Creates a solver (replaces LBM3D.__init__ allocation, lbm3d.py:342-366). */
+int wfs_create(const wfs_desc* desc, wfs_solver** out);
+
+/* Frees all device memory of the solver. NULL is accepted. */
+int wfs_destroy(wfs_solver* h);
+
+/* Uploads the geometry and initial state (lbm3d.py:352-364): solid mask
+ * (u8, N), wall friction coefficients cf (f32, N; NULL = all zero), inlet
+ * velocity uin (f32, ny*nz, lattice units). Populations are set to the
+ * equilibrium at rho = 1, u = (uin broadcast along x, 0, 0), zero velocity in
+ * solid cells; rho = 1, u = 0 for the macroscopic fields; uref = 0; forces 0. */
+int wfs_upload(wfs_solver* h, const uint8_t* solid, const float* cf, const float* uin);
+
+/* Replaces the body-force fields (lattice units, f32, N each) (lbm3d.py:386-388,
+ * and the caller reassignment of Fx/Fy/Fz in reach3d.py:434-436). */
+int wfs_set_force(wfs_solver* h, const float* Fx, const float* Fy, const float* Fz);
+
+/* Scales the forces used by the step: F_used = F * scale in FP32, i.e. the
+ * thrust ramp F = F0 * s of reach3d.py:431-437 without re-uploading F. */
+int wfs_set_force_scale(wfs_solver* h, float scale);
+
+/* Replaces the inlet velocity profile (f32, ny*nz). */
+int wfs_set_uin(wfs_solver* h, const float* uin);
+
+/* Replaces the wall-model reference speed (f32, N) (reach3d.py:418 assignment). */
+int wfs_set_uref(wfs_solver* h, const float* uref);
+
+/* Sets the bed/bank wall mask used by wfs_update_uref (u8, N). */
+int wfs_set_bedwall(wfs_solver* h, const uint8_t* bedwall);
+
+/* Run3D._update_uref (reach3d.py:412-418) on the device: uref = depth-averaged
+ * column speed |(sum_z ux, sum_z uy)| / n_fluid on bed/bank cells, 0 elsewhere. */
+int wfs_update_uref(wfs_solver* h);
+
+/* n solver steps (lbm3d.py:390-412): fused collide-stream, inlet/outlet rows,
+ * buffer swap. Asynchronous. The macroscopic fields hold the last step's
+ * values for every cell when the call's work completes. */
+int wfs_step(wfs_solver* h, int32_t n);
+
+/* Waits for all queued work of the handle. */
+int wfs_synchronize(wfs_solver* h);
+
+/* Number of steps taken (LBM3D.nstep). */
+int64_t wfs_nstep(const wfs_solver* h);
+
+/* Copies the macroscopic fields of the last step (f32, N each; NULL skips). */
+int wfs_get_macro(wfs_solver* h, float* rho, float* ux, float* uy, float* uz);
+int wfs_get_macro_f64(wfs_solver* h, double* rho, double* ux, double* uy, double* uz);
+
+/* Overwrites the macroscopic fields that the next step's wall model reads
+ * as the previous step's values (f32, N each). */
+int wfs_set_macro(wfs_solver* h, const float* rho, const float* ux, const float* uy, const float* uz);
+
+/* Copies uref (f32, N). */
+int wfs_get_uref(wfs_solver* h, float* uref);
+
+/* Populations f[i*N + cell] decoded to f32 / f64 (19*N values), and the reverse. */
+int wfs_get_populations(wfs_solver* h, float* f);
+int wfs_get_populations_f64(wfs_solver* h, double* f);
+int wfs_set_populations(wfs_solver* h, const float* f);
+
+/* Run3D time-mean accumulation (reach3d.py:443-451): adds ux, uy, uz, rho,
+ * |u| and ux^2 of the current state to running sums (storage precision). */
+int wfs_accumulate(wfs_solver* h);
+int wfs_reset_sums(wfs_solver* h);
+/* Copies the six running sums (f32, N each; NULL skips) and the sample count. */
+int wfs_get_sums(wfs_solver* h, float* ux, float* uy, float* uz, float* rho, float* sp, float* uxx,
+                 int32_t* navg);
+
+/* FP64 per-x reductions over fluid cells with z0 <= z < z1:
+ * out[x] = sum_{y, z} field(x, y, z) for field in enum wfs_field. */
+enum wfs_field {
+    WFS_F_RHO = 0, WFS_F_UX = 1, WFS_F_UY = 2, WFS_F_UZ = 3,
+    WFS_F_SUM_UX = 4, WFS_F_SUM_UY = 5, WFS_F_SUM_UZ = 6, WFS_F_SUM_RHO = 7, WFS_F_SUM_SP = 8, WFS_F_SUM_UXX = 9,
+    WFS_F_ONE = 10
+};
+int wfs_profile_x(wfs_solver* h, int32_t field, int32_t z0, int32_t z1, double* out);
+
+/* FP64 total over fluid cells of a field (e.g. mass = total of WFS_F_RHO). */
+int wfs_total(wfs_solver* h, int32_t field, double* out);
+
+/* Watchdog: WFS_ENONFINITE if any fluid cell's rho or u is NaN/Inf. */
+int wfs_check_finite(wfs_solver* h);
+
+/* Kernel timing: while enabled, every launch of the fused step kernel is
+ * bracketed by HIP events on the handle's stream. wfs_timing_read waits for
+ * the stream and returns the summed kernel time (ms) and the number of
+ * launches since the last enable/read, then starts a new tally. */
+int wfs_timing_enable(wfs_solver* h, int32_t on);
+int wfs_timing_read(wfs_solver* h, double* kernel_ms, int64_t* launches);
+
+/* Device memory held by the solver, in bytes. */
+int64_t wfs_device_bytes(const wfs_solver* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WFS_H */

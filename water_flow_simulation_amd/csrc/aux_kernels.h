@@ -1,0 +1,192 @@
+// Set-up, cadence and readout kernels around the step (gfx950).
+//
+// Everything here is either once per case (initial state, cell flags), once
+// per 20-step chunk (uref, time-mean sums) or once per readout (FP64
+// reductions), so these kernels are simple and bandwidth-light; the step
+// kernel in step_kernels.h is the hot path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lattice.h"
+#include "step_kernels.h"
+
+namespace wfs {
+
+// Equilibrium at rho = 1, u = (u0, 0, 0), evaluated in double and stored once
+// through the codec; u0 = uin of the cell's row, 0 in solid cells
+// (floodsim/lbm3d.py:362-364, 377-384).
+template <class Codec>
+__global__ void k_init(typename Codec::T* __restrict__ f, const uint8_t* __restrict__ solid,
+                       const typename Codec::R* __restrict__ uin, int nx, int ny, int nz, long long N) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const long long j = c / nx;  // row index y + ny*z
+    const double u = solid[c] ? 0.0 : (double)uin[j];
+    const double W[3] = {1.0 / 3, 1.0 / 18, 1.0 / 36};
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        const double w = i == 0 ? W[0] : (i < 7 ? W[1] : W[2]);
+        const double cu = 3.0 * kCX[i] * u;
+        const double v = w * (1.0 + cu + 0.5 * cu * cu - 1.5 * u * u);
+        Codec::template st<i>(f + (long long)i * N + c, (typename Codec::R)v);
+    });
+}
+
+// Cell flags and wall-link masks. A link i of fluid cell c is a wall link when
+// its pull source lies inside [0, nx) in x but outside the y range, below the
+// bed (z < 0) or on a solid cell; sources above the lid are mirrored into the
+// same layer first (floodsim/lbm3d.py:50-58 applies the same rules per step).
+template <typename R>
+__global__ void k_flags(const uint8_t* __restrict__ solid, const R* __restrict__ cfw, uint8_t* __restrict__ flag,
+                        uint32_t* __restrict__ wmask, int nx, int ny, int nz) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nx) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const long long c = x + (long long)nx * (y + (long long)ny * z);
+    uint8_t fl = 0;
+    uint32_t wm = 0;
+    if (solid[c]) {
+        fl = FLAG_SOLID;
+    } else {
+        for (int i = 0; i < Q; ++i) {
+            const int xs = x - kCX[i], ys = y - kCY[i];
+            int zs = z - kCZ[i];
+            if (xs < 0 || xs >= nx) continue;
+            if (zs >= nz) zs = z;
+            bool wall = ys < 0 || ys >= ny || zs < 0;
+            if (!wall) wall = solid[xs + (long long)nx * (ys + (long long)ny * zs)] != 0;
+            if (wall) wm |= 1u << i;
+        }
+        if (wm) fl |= FLAG_NEARWALL;
+        if (cfw[c] > R(0)) fl |= FLAG_CF;
+    }
+    flag[c] = fl;
+    wmask[c] = wm;
+}
+
+// FLAG_FORCED where any force component is nonzero.
+template <typename R>
+__global__ void k_forced(uint8_t* __restrict__ flag, const R* __restrict__ Fx, const R* __restrict__ Fy,
+                         const R* __restrict__ Fz, long long N) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    uint8_t fl = flag[c] & uint8_t(~FLAG_FORCED);
+    if (!(fl & FLAG_SOLID) && (Fx[c] != R(0) || Fy[c] != R(0) || Fz[c] != R(0))) fl |= FLAG_FORCED;
+    flag[c] = fl;
+}
+
+// Depth-averaged column speed for the bed/bank wall model
+// (floodsim/reach3d.py:412-418): one work-item per (x, y) column, sums over
+// z = 0..nz-1 in order.
+template <typename R>
+__global__ void k_uref(const R* __restrict__ ux, const R* __restrict__ uy, const uint8_t* __restrict__ flag,
+                       const uint8_t* __restrict__ bedwall, R* __restrict__ uref, int nx, int ny, int nz) {
+    const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long plane = (long long)nx * ny;
+    if (col >= plane) return;
+    R sx = R(0), sy = R(0), nfl = R(0);
+    for (int z = 0; z < nz; ++z) {
+        const long long c = col + plane * z;
+        const R fl = (flag[c] & FLAG_SOLID) ? R(0) : R(1);
+        sx += ux[c] * fl;
+        sy += uy[c] * fl;
+        nfl += fl;
+    }
+    const R ncol = nfl > R(1) ? nfl : R(1);
+    const R U = sqrt(sx * sx + sy * sy) / ncol;
+    for (int z = 0; z < nz; ++z) {
+        const long long c = col + plane * z;
+        uref[c] = bedwall[c] ? U : R(0);
+    }
+}
+
+// Running sums for the time means (floodsim/reach3d.py:443-451).
+template <typename R>
+__global__ void k_accumulate(const R* __restrict__ rho, const R* __restrict__ ux, const R* __restrict__ uy,
+                             const R* __restrict__ uz, R* __restrict__ s_ux, R* __restrict__ s_uy,
+                             R* __restrict__ s_uz, R* __restrict__ s_rho, R* __restrict__ s_sp,
+                             R* __restrict__ s_uxx, long long N) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const R a = ux[c], b = uy[c], d = uz[c];
+    s_ux[c] += a;
+    s_uy[c] += b;
+    s_uz[c] += d;
+    s_rho[c] += rho[c];
+    s_sp[c] += sqrt(a * a + b * b + d * d);
+    s_uxx[c] += a * a;
+}
+
+template <typename R>
+__global__ void k_fill_macro(R* __restrict__ rho, R* __restrict__ ux, R* __restrict__ uy, R* __restrict__ uz,
+                             long long N) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    rho[c] = R(1);
+    ux[c] = R(0);
+    uy[c] = R(0);
+    uz[c] = R(0);
+}
+
+// Population decode/encode between storage and a plain real array.
+template <class Codec, typename O>
+__global__ void k_decode(const typename Codec::T* __restrict__ f, O* __restrict__ out, long long N) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        out[(long long)i * N + c] = (O)Codec::template ld<i>(f + (long long)i * N + c);
+    });
+}
+
+template <class Codec>
+__global__ void k_encode(typename Codec::T* __restrict__ f, const float* __restrict__ in, long long N) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        Codec::template st<i>(f + (long long)i * N + c, (typename Codec::R)in[(long long)i * N + c]);
+    });
+}
+
+// FP64 per-x sums over fluid cells of rows j in [j0, j1) (j = y + ny*z):
+// pass 1 writes one partial per (x, chunk of rows), pass 2 adds the chunks
+// in order. Deterministic (no atomics).
+template <typename R>
+__global__ void k_profile_partial(const R* __restrict__ v, const uint8_t* __restrict__ flag, int nx, int j0,
+                                  int j1, int rows_per_chunk, double* __restrict__ partial) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nx) return;
+    const int chunk = blockIdx.y;
+    const int a = j0 + chunk * rows_per_chunk;
+    const int b = min(a + rows_per_chunk, j1);
+    double s = 0.0;
+    for (int j = a; j < b; ++j) {
+        const long long c = x + (long long)nx * j;
+        if (!(flag[c] & FLAG_SOLID)) s += v ? (double)v[c] : 1.0;
+    }
+    partial[(long long)chunk * nx + x] = s;
+}
+
+__global__ void k_profile_final(const double* __restrict__ partial, int nx, int nchunks, double* __restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nx) return;
+    double s = 0.0;
+    for (int k = 0; k < nchunks; ++k) s += partial[(long long)k * nx + x];
+    out[x] = s;
+}
+
+// Watchdog: counts fluid cells whose macroscopic state is not finite.
+template <typename R>
+__global__ void k_nonfinite(const R* __restrict__ rho, const R* __restrict__ ux, const R* __restrict__ uy,
+                            const R* __restrict__ uz, const uint8_t* __restrict__ flag, long long N,
+                            unsigned int* __restrict__ count) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    if (flag[c] & FLAG_SOLID) return;
+    const bool bad = !isfinite(rho[c]) || !isfinite(ux[c]) || !isfinite(uy[c]) || !isfinite(uz[c]);
+    if (bad) atomicAdd(count, 1u);
+}
+
+}  // namespace wfs

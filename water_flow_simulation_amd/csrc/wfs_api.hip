@@ -1,0 +1,770 @@
+// C-ABI of libwfs.so: solver handles, uploads, stepping and readouts.
+// Declarations and the reference interface each entry point replaces are in
+// include/wfs.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "wfs.h"
+#include "aux_kernels.h"
+#include "step_kernels.h"
+
+using namespace wfs;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define WFS_HIP(expr)                                                                         \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return fail(WFS_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define WFS_LAUNCH_CHECK() WFS_HIP(hipGetLastError())
+
+inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+__global__ void k_fill_zero_u32(uint32_t* p, long long n) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0u;
+}
+
+// Device buffer owned by a solver.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+struct wfs_solver {
+    wfs_desc d{};
+    long long N = 0;
+    int64_t nstep = 0;
+    int32_t navg = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t bytes = 0;
+    bool uploaded = false;
+    bool have_bedwall = false;
+    // step-kernel timing: event pairs recorded around each launch
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    size_t ev_used = 0;
+    double timed_ms = 0.0;
+    int64_t timed_launches = 0;
+    virtual ~wfs_solver() {
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    // Sums the recorded pairs into timed_ms and frees the pool for reuse.
+    int harvest() {
+        if (ev_used == 0) return WFS_OK;
+        WFS_HIP(hipEventSynchronize(events[ev_used - 1]));
+        for (size_t k = 0; k + 1 < ev_used; k += 2) {
+            float ms = 0.f;
+            WFS_HIP(hipEventElapsedTime(&ms, events[k], events[k + 1]));
+            timed_ms += ms;
+            ++timed_launches;
+        }
+        ev_used = 0;
+        return WFS_OK;
+    }
+    // Records one event of a pair on the stream (grows the pool to 4096 events).
+    int mark() {
+        if (ev_used == events.size()) {
+            if (events.size() >= 4096) {
+                int rc = harvest();
+                if (rc) return rc;
+            } else {
+                hipEvent_t e;
+                WFS_HIP(hipEventCreate(&e));
+                events.push_back(e);
+            }
+        }
+        WFS_HIP(hipEventRecord(events[ev_used++], stream));
+        return WFS_OK;
+    }
+    virtual int alloc() = 0;
+    virtual int upload(const uint8_t* solid, const float* cf, const float* uin) = 0;
+    virtual int set_force(const float* Fx, const float* Fy, const float* Fz) = 0;
+    virtual int set_force_scale(float s) = 0;
+    virtual int set_uin(const float* uin) = 0;
+    virtual int set_uref(const float* uref) = 0;
+    virtual int set_bedwall(const uint8_t* bw) = 0;
+    virtual int update_uref() = 0;
+    virtual int step(int n) = 0;
+    virtual int get_macro(void* rho, void* ux, void* uy, void* uz, bool f64) = 0;
+    virtual int get_uref(float* uref) = 0;
+    virtual int set_macro(const float* rho, const float* ux, const float* uy, const float* uz) = 0;
+    virtual int get_pops(void* f, bool f64) = 0;
+    virtual int set_pops(const float* f) = 0;
+    virtual int accumulate() = 0;
+    virtual int reset_sums() = 0;
+    virtual int get_sums(float* const* out) = 0;
+    virtual int profile_x(int field, int z0, int z1, double* out) = 0;
+    virtual int total(int field, double* out) = 0;
+    virtual int check_finite() = 0;
+
+    int alloc_buf(DevBuf& b, size_t bytes_) {
+        bytes_ = (bytes_ + 255) & ~size_t(255);  // whole 256-byte blocks: fills and vector access stay in bounds
+        b.bytes = bytes_;
+        if (hipMalloc(&b.p, bytes_) != hipSuccess) {
+            (void)hipGetLastError();
+            b.p = nullptr;
+            return fail(WFS_ENOMEM, "hipMalloc of " + std::to_string(bytes_) + " bytes failed");
+        }
+        bytes += bytes_;
+        return WFS_OK;
+    }
+};
+
+namespace {
+
+template <class Codec>
+struct Solver final : wfs_solver {
+    using T = typename Codec::T;
+    using R = typename Codec::R;
+    DevBuf f[2], flag, wmask, Fx, Fy, Fz, rho, ux, uy, uz, cfw, uref, uin, bedwall;
+    DevBuf sums[6], partial, out_x, counter;
+    int cur = 0;
+    R fscale = R(1);
+    int bx = 128;
+
+    T* fp(int k) { return static_cast<T*>(f[k].p); }
+    R* rp(DevBuf& b) { return static_cast<R*>(b.p); }
+
+    int alloc() override {
+        const size_t n = (size_t)N;
+        int rc;
+        for (int k = 0; k < 2; ++k)
+            if ((rc = alloc_buf(f[k], n * Q * sizeof(T)))) return rc;
+        if ((rc = alloc_buf(flag, n))) return rc;
+        if ((rc = alloc_buf(wmask, n * sizeof(uint32_t)))) return rc;
+        for (DevBuf* b : {&Fx, &Fy, &Fz, &rho, &ux, &uy, &uz, &cfw, &uref})
+            if ((rc = alloc_buf(*b, n * sizeof(R)))) return rc;
+        if ((rc = alloc_buf(uin, (size_t)d.ny * d.nz * sizeof(R)))) return rc;
+        if ((rc = alloc_buf(bedwall, n))) return rc;
+        for (auto& s : sums)
+            if ((rc = alloc_buf(s, n * sizeof(R)))) return rc;
+        if ((rc = alloc_buf(out_x, (size_t)d.nx * sizeof(double)))) return rc;
+        if ((rc = alloc_buf(partial, (size_t)d.nx * 256 * sizeof(double)))) return rc;
+        if ((rc = alloc_buf(counter, 256))) return rc;
+        return WFS_OK;
+    }
+
+    // host float -> device R
+    int put(DevBuf& b, const float* src, size_t n) {
+        if constexpr (std::is_same<R, float>::value) {
+            WFS_HIP(hipMemcpyAsync(b.p, src, n * sizeof(float), hipMemcpyHostToDevice, stream));
+            WFS_HIP(hipStreamSynchronize(stream));
+        } else {
+            std::vector<R> tmp(src, src + n);
+            WFS_HIP(hipMemcpyAsync(b.p, tmp.data(), n * sizeof(R), hipMemcpyHostToDevice, stream));
+            WFS_HIP(hipStreamSynchronize(stream));
+        }
+        return WFS_OK;
+    }
+
+    // device R -> host float or double
+    int get(DevBuf& b, void* dst, size_t n, bool f64) {
+        if (!dst) return WFS_OK;
+        const bool same = f64 ? std::is_same<R, double>::value : std::is_same<R, float>::value;
+        if (same) {
+            WFS_HIP(hipMemcpyAsync(dst, b.p, n * sizeof(R), hipMemcpyDeviceToHost, stream));
+            WFS_HIP(hipStreamSynchronize(stream));
+        } else {
+            std::vector<R> tmp(n);
+            WFS_HIP(hipMemcpyAsync(tmp.data(), b.p, n * sizeof(R), hipMemcpyDeviceToHost, stream));
+            WFS_HIP(hipStreamSynchronize(stream));
+            if (f64)
+                for (size_t k = 0; k < n; ++k) static_cast<double*>(dst)[k] = (double)tmp[k];
+            else
+                for (size_t k = 0; k < n; ++k) static_cast<float*>(dst)[k] = (float)tmp[k];
+        }
+        return WFS_OK;
+    }
+
+    int zero(DevBuf& b) {
+        // a fill kernel rather than hipMemsetAsync keeps the call graph-capturable
+        const long long n = (long long)(b.bytes / 4);
+        hipLaunchKernelGGL(k_fill_zero_u32, dim3(blocks_for(n, 256)), dim3(256), 0, stream,
+                           static_cast<uint32_t*>(b.p), n);
+        WFS_LAUNCH_CHECK();
+        return WFS_OK;
+    }
+
+    int upload(const uint8_t* solid, const float* cf, const float* uin_h) override {
+        const size_t n = (size_t)N;
+        WFS_HIP(hipMemcpyAsync(bedwall.p, solid, n, hipMemcpyHostToDevice, stream));  // staging for solid
+        if (cf) {
+            int rc = put(cfw, cf, n);
+            if (rc) return rc;
+        } else {
+            int rc = zero(cfw);
+            if (rc) return rc;
+        }
+        int rc = put(uin, uin_h, (size_t)d.ny * d.nz);
+        if (rc) return rc;
+        const dim3 gx(blocks_for(d.nx, 64), d.ny, d.nz);
+        hipLaunchKernelGGL(k_flags<R>, gx, dim3(64), 0, stream, static_cast<const uint8_t*>(bedwall.p), rp(cfw),
+                           static_cast<uint8_t*>(flag.p), static_cast<uint32_t*>(wmask.p), d.nx, d.ny, d.nz);
+        WFS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_init<Codec>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(0),
+                           static_cast<const uint8_t*>(bedwall.p), rp(uin), d.nx, d.ny, d.nz, N);
+        WFS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_init<Codec>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(1),
+                           static_cast<const uint8_t*>(bedwall.p), rp(uin), d.nx, d.ny, d.nz, N);
+        WFS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_fill_macro<R>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(rho), rp(ux),
+                           rp(uy), rp(uz), N);
+        WFS_LAUNCH_CHECK();
+        for (DevBuf* b : {&Fx, &Fy, &Fz, &uref, &bedwall})
+            if ((rc = zero(*b))) return rc;
+        if ((rc = reset_sums())) return rc;
+        WFS_HIP(hipStreamSynchronize(stream));
+        cur = 0;
+        nstep = 0;
+        fscale = R(1);
+        have_bedwall = false;
+        uploaded = true;
+        return WFS_OK;
+    }
+
+    int set_force(const float* a, const float* b, const float* c) override {
+        const size_t n = (size_t)N;
+        int rc;
+        if ((rc = put(Fx, a, n)) || (rc = put(Fy, b, n)) || (rc = put(Fz, c, n))) return rc;
+        hipLaunchKernelGGL(k_forced<R>, dim3(blocks_for(N, 256)), dim3(256), 0, stream,
+                           static_cast<uint8_t*>(flag.p), rp(Fx), rp(Fy), rp(Fz), N);
+        WFS_LAUNCH_CHECK();
+        return WFS_OK;
+    }
+
+    int set_force_scale(float s) override {
+        fscale = R(s);
+        return WFS_OK;
+    }
+
+    int set_uin(const float* u) override { return put(uin, u, (size_t)d.ny * d.nz); }
+    int set_uref(const float* u) override { return put(uref, u, (size_t)N); }
+
+    int set_bedwall(const uint8_t* bw) override {
+        WFS_HIP(hipMemcpyAsync(bedwall.p, bw, (size_t)N, hipMemcpyHostToDevice, stream));
+        WFS_HIP(hipStreamSynchronize(stream));
+        have_bedwall = true;
+        return WFS_OK;
+    }
+
+    int update_uref() override {
+        if (!have_bedwall) return fail(WFS_EINVAL, "wfs_update_uref: no bed/bank mask set (wfs_set_bedwall)");
+        const long long plane = (long long)d.nx * d.ny;
+        hipLaunchKernelGGL(k_uref<R>, dim3(blocks_for(plane, 64)), dim3(64), 0, stream, rp(ux), rp(uy),
+                           static_cast<const uint8_t*>(flag.p), static_cast<const uint8_t*>(bedwall.p), rp(uref),
+                           d.nx, d.ny, d.nz);
+        WFS_LAUNCH_CHECK();
+        return WFS_OK;
+    }
+
+    int step(int n) override {
+        StepArgs<R> a;
+        a.flag = static_cast<const uint8_t*>(flag.p);
+        a.wmask = static_cast<const uint32_t*>(wmask.p);
+        a.Fx = rp(Fx);
+        a.Fy = rp(Fy);
+        a.Fz = rp(Fz);
+        a.rho = rp(rho);
+        a.ux = rp(ux);
+        a.uy = rp(uy);
+        a.uz = rp(uz);
+        a.cfw = rp(cfw);
+        a.uref = rp(uref);
+        a.fscale = fscale;
+        a.tau0 = R(d.tau0);
+        a.tau0sq = a.tau0 * a.tau0;
+        const R cs2 = R((float)(d.cs * d.cs));
+        a.ksmag = R(18) * R(1.41421356) * cs2;
+        a.nx = d.nx;
+        a.ny = d.ny;
+        a.nz = d.nz;
+        a.N = N;
+        const dim3 grid(blocks_for(d.nx, bx), d.ny, d.nz);
+        const int rows = d.ny * d.nz;
+        for (int k = 0; k < n; ++k) {
+            a.write_all = (k == n - 1) ? 1 : 0;
+            if (timing) {
+                if (ev_used % 2) return fail(WFS_EHIP, "step timing: unpaired event");
+                int rc = mark();
+                if (rc) return rc;
+            }
+            hipLaunchKernelGGL(k_step<Codec>, grid, dim3(bx), 0, stream, fp(cur), fp(cur ^ 1), a);
+            WFS_LAUNCH_CHECK();
+            if (timing) {
+                int rc = mark();
+                if (rc) return rc;
+            }
+            hipLaunchKernelGGL(k_bc<Codec>, dim3(blocks_for(rows, 64)), dim3(64), 0, stream, fp(cur ^ 1),
+                               static_cast<const uint8_t*>(flag.p), rp(uin), d.nx, d.ny, d.nz, N, R(1));
+            WFS_LAUNCH_CHECK();
+            cur ^= 1;
+        }
+        nstep += n;
+        return WFS_OK;
+    }
+
+    int get_macro(void* r, void* a, void* b, void* c, bool f64) override {
+        const size_t n = (size_t)N;
+        int rc;
+        if ((rc = get(rho, r, n, f64)) || (rc = get(ux, a, n, f64)) || (rc = get(uy, b, n, f64)) ||
+            (rc = get(uz, c, n, f64)))
+            return rc;
+        return WFS_OK;
+    }
+
+    int get_uref(float* u) override { return get(uref, u, (size_t)N, false); }
+
+    int set_macro(const float* r, const float* a, const float* b, const float* c) override {
+        const size_t n = (size_t)N;
+        int rc;
+        if ((rc = put(rho, r, n)) || (rc = put(ux, a, n)) || (rc = put(uy, b, n)) || (rc = put(uz, c, n))) return rc;
+        return WFS_OK;
+    }
+
+    int get_pops(void* out, bool f64) override {
+        const size_t n = (size_t)N * Q;
+        DevBuf tmp;
+        const size_t esz = f64 ? sizeof(double) : sizeof(float);
+        if (hipMalloc(&tmp.p, n * esz) != hipSuccess) {
+            (void)hipGetLastError();
+            tmp.p = nullptr;
+            return fail(WFS_ENOMEM, "hipMalloc for population readout failed");
+        }
+        if (f64)
+            hipLaunchKernelGGL((k_decode<Codec, double>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
+                               static_cast<double*>(tmp.p), N);
+        else
+            hipLaunchKernelGGL((k_decode<Codec, float>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
+                               static_cast<float*>(tmp.p), N);
+        WFS_LAUNCH_CHECK();
+        WFS_HIP(hipMemcpyAsync(out, tmp.p, n * esz, hipMemcpyDeviceToHost, stream));
+        WFS_HIP(hipStreamSynchronize(stream));
+        return WFS_OK;
+    }
+
+    int set_pops(const float* in) override {
+        const size_t n = (size_t)N * Q;
+        DevBuf tmp;
+        if (hipMalloc(&tmp.p, n * sizeof(float)) != hipSuccess) {
+            (void)hipGetLastError();
+            tmp.p = nullptr;
+            return fail(WFS_ENOMEM, "hipMalloc for population upload failed");
+        }
+        WFS_HIP(hipMemcpyAsync(tmp.p, in, n * sizeof(float), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_encode<Codec>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
+                           static_cast<const float*>(tmp.p), N);
+        WFS_LAUNCH_CHECK();
+        WFS_HIP(hipStreamSynchronize(stream));
+        return WFS_OK;
+    }
+
+    int accumulate() override {
+        hipLaunchKernelGGL(k_accumulate<R>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(rho), rp(ux), rp(uy),
+                           rp(uz), rp(sums[0]), rp(sums[1]), rp(sums[2]), rp(sums[3]), rp(sums[4]), rp(sums[5]), N);
+        WFS_LAUNCH_CHECK();
+        ++navg;
+        return WFS_OK;
+    }
+
+    int reset_sums() override {
+        for (auto& s : sums) {
+            int rc = zero(s);
+            if (rc) return rc;
+        }
+        navg = 0;
+        return WFS_OK;
+    }
+
+    int get_sums(float* const* out) override {
+        for (int k = 0; k < 6; ++k) {
+            int rc = get(sums[k], out[k], (size_t)N, false);
+            if (rc) return rc;
+        }
+        return WFS_OK;
+    }
+
+    const R* field_ptr(int field) {
+        switch (field) {
+            case WFS_F_RHO: return rp(rho);
+            case WFS_F_UX: return rp(ux);
+            case WFS_F_UY: return rp(uy);
+            case WFS_F_UZ: return rp(uz);
+            case WFS_F_SUM_UX: return rp(sums[0]);
+            case WFS_F_SUM_UY: return rp(sums[1]);
+            case WFS_F_SUM_UZ: return rp(sums[2]);
+            case WFS_F_SUM_RHO: return rp(sums[3]);
+            case WFS_F_SUM_SP: return rp(sums[4]);
+            case WFS_F_SUM_UXX: return rp(sums[5]);
+            default: return nullptr;
+        }
+    }
+
+    int profile_dev(int field, int z0, int z1) {
+        if (field < 0 || field > WFS_F_ONE) return fail(WFS_EINVAL, "wfs_profile_x: unknown field");
+        if (z0 < 0 || z1 > d.nz || z0 >= z1) return fail(WFS_EINVAL, "wfs_profile_x: bad z range");
+        const R* v = field_ptr(field);
+        const int j0 = z0 * d.ny, j1 = z1 * d.ny;
+        const int rows = j1 - j0;
+        const int nchunks = std::min(256, std::max(1, rows / 16));
+        const int per = (rows + nchunks - 1) / nchunks;
+        const int used = (rows + per - 1) / per;
+        hipLaunchKernelGGL(k_profile_partial<R>, dim3(blocks_for(d.nx, 64), used), dim3(64), 0, stream, v,
+                           static_cast<const uint8_t*>(flag.p), d.nx, j0, j1, per,
+                           static_cast<double*>(partial.p));
+        WFS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_profile_final, dim3(blocks_for(d.nx, 64)), dim3(64), 0, stream,
+                           static_cast<const double*>(partial.p), d.nx, used, static_cast<double*>(out_x.p));
+        WFS_LAUNCH_CHECK();
+        return WFS_OK;
+    }
+
+    int profile_x(int field, int z0, int z1, double* out) override {
+        int rc = profile_dev(field, z0, z1);
+        if (rc) return rc;
+        WFS_HIP(hipMemcpyAsync(out, out_x.p, (size_t)d.nx * sizeof(double), hipMemcpyDeviceToHost, stream));
+        WFS_HIP(hipStreamSynchronize(stream));
+        return WFS_OK;
+    }
+
+    int total(int field, double* out) override {
+        std::vector<double> px((size_t)d.nx);
+        int rc = profile_x(field, 0, d.nz, px.data());
+        if (rc) return rc;
+        double s = 0.0;
+        for (double v : px) s += v;
+        *out = s;
+        return WFS_OK;
+    }
+
+    int check_finite() override {
+        int rc = zero(counter);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_nonfinite<R>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(rho), rp(ux), rp(uy),
+                           rp(uz), static_cast<const uint8_t*>(flag.p), N, static_cast<unsigned int*>(counter.p));
+        WFS_LAUNCH_CHECK();
+        unsigned int cnt = 0;
+        WFS_HIP(hipMemcpyAsync(&cnt, counter.p, sizeof(cnt), hipMemcpyDeviceToHost, stream));
+        WFS_HIP(hipStreamSynchronize(stream));
+        if (cnt) return fail(WFS_ENONFINITE, std::to_string(cnt) + " fluid cells hold a non-finite state after step " +
+                                                 std::to_string(nstep));
+        return WFS_OK;
+    }
+};
+
+wfs_solver* make_solver(int storage) {
+    switch (storage) {
+        case WFS_STORE_F32: return new Solver<wfs::CodecF32>();
+        case WFS_STORE_F16DEV: return new Solver<wfs::CodecF16Dev>();
+        case WFS_STORE_BF16DEV: return new Solver<wfs::CodecBF16Dev>();
+        case WFS_STORE_F64: return new Solver<wfs::CodecF64>();
+        default: return nullptr;
+    }
+}
+
+int check_handle(const wfs_solver* h, bool need_upload = true) {
+    if (!h) return fail(WFS_EINVAL, "null solver handle");
+    if (need_upload && !h->uploaded) return fail(WFS_EINVAL, "solver has no geometry yet (call wfs_upload first)");
+    return WFS_OK;
+}
+
+// Every call runs with the handle's device current.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int now = -1;
+        if (prev >= 0 && hipGetDevice(&now) == hipSuccess && now != prev) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int wfs_abi_version(void) { return WFS_ABI_VERSION; }
+
+int wfs_device_count(int32_t* count) {
+    if (!count) return fail(WFS_EINVAL, "null count");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    *count = n;
+    return WFS_OK;
+}
+
+const char* wfs_last_error(void) { return g_err.c_str(); }
+
+int wfs_create(const wfs_desc* desc, wfs_solver** out) {
+    if (!desc || !out) return fail(WFS_EINVAL, "wfs_create: null argument");
+    *out = nullptr;
+    const wfs_desc& d = *desc;
+    if (d.nx < 3 || d.ny < 1 || d.nz < 1) return fail(WFS_EINVAL, "wfs_create: need nx >= 3, ny >= 1, nz >= 1");
+    const long long N = (long long)d.nx * d.ny * d.nz;
+    if (N >= (1ll << 31)) return fail(WFS_EINVAL, "wfs_create: nx*ny*nz must be below 2^31 cells");
+    if (d.ny > 65535 || d.nz > 65535) return fail(WFS_EINVAL, "wfs_create: ny and nz must be at most 65535");
+    if (!(d.tau0 > 0.5f) || !std::isfinite(d.tau0)) return fail(WFS_EINVAL, "wfs_create: tau0 must exceed 0.5");
+    if (!(d.cs >= 0.0) || !std::isfinite(d.cs)) return fail(WFS_EINVAL, "wfs_create: cs must be >= 0");
+    const int bx = d.block_x == 0 ? 128 : d.block_x;
+    if (bx < 64 || bx > 256 || bx % 64) return fail(WFS_EINVAL, "wfs_create: block_x must be 64, 128, 192 or 256");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return fail(WFS_ENODEV, "wfs_create: no HIP device visible");
+    }
+    if (d.device < 0 || d.device >= ndev) return fail(WFS_ENODEV, "wfs_create: device ordinal out of range");
+    std::unique_ptr<wfs_solver> h(make_solver(d.storage));
+    if (!h) return fail(WFS_EINVAL, "wfs_create: unknown storage format");
+    h->d = d;
+    h->N = N;
+    h->device = d.device;
+    DeviceGuard g(d.device);
+    WFS_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    int rc = h->alloc();
+    if (rc) return rc;
+    if (auto* s = dynamic_cast<Solver<wfs::CodecF32>*>(h.get())) s->bx = bx;
+    if (auto* s = dynamic_cast<Solver<wfs::CodecF16Dev>*>(h.get())) s->bx = bx;
+    if (auto* s = dynamic_cast<Solver<wfs::CodecBF16Dev>*>(h.get())) s->bx = bx;
+    if (auto* s = dynamic_cast<Solver<wfs::CodecF64>*>(h.get())) s->bx = bx;
+    *out = h.release();
+    return WFS_OK;
+}
+
+int wfs_destroy(wfs_solver* h) {
+    if (!h) return WFS_OK;
+    DeviceGuard g(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h;
+    return WFS_OK;
+}
+
+int wfs_upload(wfs_solver* h, const uint8_t* solid, const float* cf, const float* uin) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (!solid || !uin) return fail(WFS_EINVAL, "wfs_upload: solid and uin are required");
+    DeviceGuard g(h->device);
+    return h->upload(solid, cf, uin);
+}
+
+int wfs_set_force(wfs_solver* h, const float* Fx, const float* Fy, const float* Fz) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!Fx || !Fy || !Fz) return fail(WFS_EINVAL, "wfs_set_force: null force array");
+    DeviceGuard g(h->device);
+    return h->set_force(Fx, Fy, Fz);
+}
+
+int wfs_set_force_scale(wfs_solver* h, float scale) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!std::isfinite(scale)) return fail(WFS_EINVAL, "wfs_set_force_scale: scale must be finite");
+    return h->set_force_scale(scale);
+}
+
+int wfs_set_uin(wfs_solver* h, const float* uin) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!uin) return fail(WFS_EINVAL, "wfs_set_uin: null array");
+    DeviceGuard g(h->device);
+    return h->set_uin(uin);
+}
+
+int wfs_set_uref(wfs_solver* h, const float* uref) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!uref) return fail(WFS_EINVAL, "wfs_set_uref: null array");
+    DeviceGuard g(h->device);
+    return h->set_uref(uref);
+}
+
+int wfs_set_bedwall(wfs_solver* h, const uint8_t* bedwall) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!bedwall) return fail(WFS_EINVAL, "wfs_set_bedwall: null array");
+    DeviceGuard g(h->device);
+    return h->set_bedwall(bedwall);
+}
+
+int wfs_update_uref(wfs_solver* h) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    return h->update_uref();
+}
+
+int wfs_step(wfs_solver* h, int32_t n) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (n < 0) return fail(WFS_EINVAL, "wfs_step: n must be >= 0");
+    if (n == 0) return WFS_OK;
+    DeviceGuard g(h->device);
+    return h->step(n);
+}
+
+int wfs_synchronize(wfs_solver* h) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    WFS_HIP(hipStreamSynchronize(h->stream));
+    return WFS_OK;
+}
+
+int64_t wfs_nstep(const wfs_solver* h) { return h ? h->nstep : -1; }
+
+int wfs_get_macro(wfs_solver* h, float* rho, float* ux, float* uy, float* uz) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    return h->get_macro(rho, ux, uy, uz, false);
+}
+
+int wfs_get_macro_f64(wfs_solver* h, double* rho, double* ux, double* uy, double* uz) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    return h->get_macro(rho, ux, uy, uz, true);
+}
+
+int wfs_set_macro(wfs_solver* h, const float* rho, const float* ux, const float* uy, const float* uz) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!rho || !ux || !uy || !uz) return fail(WFS_EINVAL, "wfs_set_macro: null array");
+    DeviceGuard g(h->device);
+    return h->set_macro(rho, ux, uy, uz);
+}
+
+int wfs_get_uref(wfs_solver* h, float* uref) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!uref) return fail(WFS_EINVAL, "wfs_get_uref: null array");
+    DeviceGuard g(h->device);
+    return h->get_uref(uref);
+}
+
+int wfs_get_populations(wfs_solver* h, float* f) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!f) return fail(WFS_EINVAL, "wfs_get_populations: null array");
+    DeviceGuard g(h->device);
+    return h->get_pops(f, false);
+}
+
+int wfs_get_populations_f64(wfs_solver* h, double* f) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!f) return fail(WFS_EINVAL, "wfs_get_populations_f64: null array");
+    DeviceGuard g(h->device);
+    return h->get_pops(f, true);
+}
+
+int wfs_set_populations(wfs_solver* h, const float* f) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!f) return fail(WFS_EINVAL, "wfs_set_populations: null array");
+    DeviceGuard g(h->device);
+    return h->set_pops(f);
+}
+
+int wfs_accumulate(wfs_solver* h) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    return h->accumulate();
+}
+
+int wfs_reset_sums(wfs_solver* h) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    return h->reset_sums();
+}
+
+int wfs_get_sums(wfs_solver* h, float* ux, float* uy, float* uz, float* rho, float* sp, float* uxx, int32_t* navg) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    float* const out[6] = {ux, uy, uz, rho, sp, uxx};
+    rc = h->get_sums(out);
+    if (rc) return rc;
+    if (navg) *navg = h->navg;
+    return WFS_OK;
+}
+
+int wfs_profile_x(wfs_solver* h, int32_t field, int32_t z0, int32_t z1, double* out) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!out) return fail(WFS_EINVAL, "wfs_profile_x: null output");
+    DeviceGuard g(h->device);
+    return h->profile_x(field, z0, z1, out);
+}
+
+int wfs_total(wfs_solver* h, int32_t field, double* out) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!out) return fail(WFS_EINVAL, "wfs_total: null output");
+    DeviceGuard g(h->device);
+    return h->total(field, out);
+}
+
+int wfs_check_finite(wfs_solver* h) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    return h->check_finite();
+}
+
+int wfs_timing_enable(wfs_solver* h, int32_t on) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    if ((rc = h->harvest())) return rc;
+    h->timed_ms = 0.0;
+    h->timed_launches = 0;
+    h->timing = on != 0;
+    return WFS_OK;
+}
+
+int wfs_timing_read(wfs_solver* h, double* kernel_ms, int64_t* launches) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (!kernel_ms || !launches) return fail(WFS_EINVAL, "wfs_timing_read: null output");
+    DeviceGuard g(h->device);
+    WFS_HIP(hipStreamSynchronize(h->stream));
+    if ((rc = h->harvest())) return rc;
+    *kernel_ms = h->timed_ms;
+    *launches = h->timed_launches;
+    h->timed_ms = 0.0;
+    h->timed_launches = 0;
+    return WFS_OK;
+}
+
+int64_t wfs_device_bytes(const wfs_solver* h) { return h ? (int64_t)h->bytes : -1; }
+
+}  // extern "C"

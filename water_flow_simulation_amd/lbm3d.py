@@ -1,0 +1,258 @@
+"""Drop-in solver object for the 3-D river reach, running on the GPU through
+libwfs.so.
+
+``LBM3D`` keeps the constructor and attribute surface of the reference's
+solver class (floodsim/lbm3d.py:333-418):
+
+    LBM3D(be, solid, Fx, Fy, Fz, uin, tau0, cs=0.17, cf_wall=None)
+
+solid: bool (nz, ny, nx); Fx/Fy/Fz: body force per unit volume in lattice
+units (nz, ny, nx); uin: inlet velocity in lattice units (nz, ny); cf_wall:
+wall friction coefficient of wall-adjacent cells (tau = rho*cf*U^2) or None.
+``rho ux uy uz`` are flat float32 arrays of the last step; ``Fx Fy Fz uin
+uref`` may be replaced by float32 arrays of the right size, checked at the
+next ``step`` with ``TypeError`` like the reference (lbm3d.py:392-394).
+
+Differences by design: the state lives in GPU memory, so the field
+attributes return host copies (a fetch per access) instead of live device
+views; ``be`` is accepted for signature compatibility and only a torch
+backend is rejected, as in the reference (lbm3d.py:349-350). Keyword-only
+extensions choose the population storage, the work-group width and the
+device, and give the device-side cadence helpers used by ``Run3D``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native as nat
+
+CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0])
+CY = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1])
+CZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1])
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+
+class LBM3D:
+    engine = "hip-lbm"
+
+    def __init__(self, be, solid, Fx, Fy, Fz, uin, tau0, cs=0.17, cf_wall=None, *, storage="f32", block_x=0,
+                 device=0):
+        if be is not None and getattr(be, "name", None) == "torch":
+            raise RuntimeError("LBM3D: the 3-D model runs on the HIP library, not on a torch backend")
+        if storage not in nat.STORAGE:
+            raise ValueError(f"LBM3D: storage must be one of {sorted(nat.STORAGE)}")
+        solid = np.asarray(solid)
+        if solid.ndim != 3:
+            raise ValueError("LBM3D: solid must be a (nz, ny, nx) array")
+        self.be = be
+        self.nz, self.ny, self.nx = solid.shape
+        self.N = int(solid.size)
+        self.tau0 = float(tau0)
+        self.cs = float(cs)
+        self.csmag2 = float(cs * cs)
+        self.storage = storage
+        self.nstep = 0
+        L = nat.lib()
+        desc = nat.wfs_desc(self.nx, self.ny, self.nz, self.tau0, self.cs, nat.STORAGE[storage], int(block_x),
+                            int(device))
+        h = ctypes.c_void_p()
+        nat.check(L.wfs_create(ctypes.byref(desc), ctypes.byref(h)), "wfs_create")
+        self._h = h
+        A = lambda a: np.ascontiguousarray(np.asarray(a).ravel(), dtype=np.float32)  # noqa: E731
+        sol = np.ascontiguousarray(solid.ravel(), dtype=np.uint8)
+        cfw = A(cf_wall) if cf_wall is not None else None
+        if cfw is not None and cfw.size != self.N:
+            raise ValueError("LBM3D: cf_wall must have the shape of solid")
+        self._uin = A(uin)
+        if self._uin.size != self.ny * self.nz:
+            raise ValueError("LBM3D: uin must be (nz, ny)")
+        nat.check(L.wfs_upload(h, nat.ptr(sol), nat.ptr(cfw), nat.ptr(self._uin)), "wfs_upload")
+        self._Fx, self._Fy, self._Fz = A(Fx), A(Fy), A(Fz)
+        if not (self._Fx.size == self._Fy.size == self._Fz.size == self.N):
+            raise ValueError("LBM3D: forces must have the shape of solid")
+        nat.check(L.wfs_set_force(h, nat.ptr(self._Fx), nat.ptr(self._Fy), nat.ptr(self._Fz)), "wfs_set_force")
+        self._uref = None          # host value assigned by a caller, uploaded at the next step
+        self._dirty = set()
+
+    # ------------------------------------------------------------ lifetime
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            nat.lib().wfs_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise RuntimeError("LBM3D: solver was closed")
+        return self._h
+
+    # ------------------------------------------------------------ replaceable inputs
+    def _set_input(self, name, value):
+        setattr(self, "_" + name, value)
+        self._dirty.add(name)
+
+    Fx = property(lambda s: s._Fx, lambda s, v: s._set_input("Fx", v))
+    Fy = property(lambda s: s._Fy, lambda s, v: s._set_input("Fy", v))
+    Fz = property(lambda s: s._Fz, lambda s, v: s._set_input("Fz", v))
+    uin = property(lambda s: s._uin, lambda s, v: s._set_input("uin", v))
+
+    @property
+    def uref(self):
+        out = np.empty(self.N, np.float32)
+        nat.check(nat.lib().wfs_get_uref(self.handle, nat.ptr(out)))
+        return out
+
+    @uref.setter
+    def uref(self, v):
+        self._set_input("uref", v)
+
+    def set_force(self, Fx, Fy, Fz):
+        """Replace the body forces (lbm3d.py:386-388)."""
+        A = lambda a: np.ascontiguousarray(np.asarray(a).ravel(), dtype=np.float32)  # noqa: E731
+        self._Fx, self._Fy, self._Fz = A(Fx), A(Fy), A(Fz)
+        self._dirty.update(("Fx", "Fy", "Fz"))
+        self._flush()
+
+    def _flush(self):
+        """Validate and upload replaced inputs (the checks of lbm3d.py:392-394)."""
+        if not self._dirty:
+            return
+        for name in ("Fx", "Fy", "Fz", "uin", "uref"):
+            if name not in self._dirty:
+                continue
+            a = getattr(self, "_" + name)
+            want = self.ny * self.nz if name == "uin" else self.N
+            if not isinstance(a, np.ndarray) or a.dtype != np.float32 or a.size != want:
+                raise TypeError(f"LBM3D: force/inlet arrays must be float32 of the correct size "
+                                f"(got {getattr(a, 'dtype', type(a))}, {getattr(a, 'size', '?')})")
+        L, h = nat.lib(), self.handle
+        c = lambda a: np.ascontiguousarray(a.ravel())  # noqa: E731
+        if self._dirty & {"Fx", "Fy", "Fz"}:
+            self._Fx, self._Fy, self._Fz = c(self._Fx), c(self._Fy), c(self._Fz)
+            nat.check(L.wfs_set_force(h, nat.ptr(self._Fx), nat.ptr(self._Fy), nat.ptr(self._Fz)))
+        if "uin" in self._dirty:
+            self._uin = c(self._uin)
+            nat.check(L.wfs_set_uin(h, nat.ptr(self._uin)))
+        if "uref" in self._dirty:
+            u = c(self._uref)
+            nat.check(L.wfs_set_uref(h, nat.ptr(u)))
+            self._uref = None
+        self._dirty.clear()
+
+    # ------------------------------------------------------------ stepping
+    def step(self, n=1):
+        """Advance n steps (lbm3d.py:390-412). Asynchronous on the GPU."""
+        self._flush()
+        nat.check(nat.lib().wfs_step(self.handle, int(n)), "wfs_step")
+        self.nstep += int(n)
+
+    def synchronize(self):
+        nat.check(nat.lib().wfs_synchronize(self.handle))
+
+    # ------------------------------------------------------------ outputs
+    def macro(self, dtype=np.float32):
+        """(rho, ux, uy, uz) of the last step as flat host arrays."""
+        out = [np.empty(self.N, dtype) for _ in range(4)]
+        fn = nat.lib().wfs_get_macro_f64 if dtype == np.float64 else nat.lib().wfs_get_macro
+        nat.check(fn(self.handle, *(nat.ptr(a) for a in out)))
+        return tuple(out)
+
+    rho = property(lambda s: s.macro()[0])
+    ux = property(lambda s: s.macro()[1])
+    uy = property(lambda s: s.macro()[2])
+    uz = property(lambda s: s.macro()[3])
+
+    def set_macro(self, rho, ux, uy, uz):
+        """Overwrite the macroscopic fields the next step's wall model reads."""
+        A = lambda a: np.ascontiguousarray(np.asarray(a, np.float32).ravel())  # noqa: E731
+        arrs = [A(a) for a in (rho, ux, uy, uz)]
+        if any(a.size != self.N for a in arrs):
+            raise ValueError("LBM3D: macroscopic fields must hold N values")
+        nat.check(nat.lib().wfs_set_macro(self.handle, *(nat.ptr(a) for a in arrs)))
+
+    def populations(self, dtype=np.float32):
+        """Decoded populations f[i*N + cell] (19*N)."""
+        out = np.empty(19 * self.N, dtype)
+        fn = nat.lib().wfs_get_populations_f64 if dtype == np.float64 else nat.lib().wfs_get_populations
+        nat.check(fn(self.handle, nat.ptr(out)))
+        return out
+
+    def set_populations(self, f):
+        f = np.ascontiguousarray(np.asarray(f, np.float32).ravel())
+        if f.size != 19 * self.N:
+            raise ValueError("LBM3D: populations must hold 19*N values")
+        nat.check(nat.lib().wfs_set_populations(self.handle, nat.ptr(f)))
+
+    # ------------------------------------------------------------ device-side cadence helpers
+    def set_force_scale(self, s):
+        """Forces used by the step become F * s (FP32), e.g. a thrust ramp."""
+        self._flush()
+        nat.check(nat.lib().wfs_set_force_scale(self.handle, float(np.float32(s))))
+
+    def set_bedwall(self, bedwall):
+        bw = np.ascontiguousarray(np.asarray(bedwall).ravel(), dtype=np.uint8)
+        if bw.size != self.N:
+            raise ValueError("LBM3D: bedwall must have the shape of solid")
+        nat.check(nat.lib().wfs_set_bedwall(self.handle, nat.ptr(bw)))
+
+    def update_uref(self):
+        """Depth-averaged column speed into uref on bed/bank cells (reach3d.py:412-418), on the GPU."""
+        self._flush()
+        nat.check(nat.lib().wfs_update_uref(self.handle))
+
+    def accumulate(self):
+        nat.check(nat.lib().wfs_accumulate(self.handle))
+
+    def reset_sums(self):
+        nat.check(nat.lib().wfs_reset_sums(self.handle))
+
+    def sums(self):
+        """The six running sums (ux, uy, uz, rho, sp, uxx) and the sample count."""
+        names = ("ux", "uy", "uz", "rho", "sp", "uxx")
+        out = {k: np.empty(self.N, np.float32) for k in names}
+        n = ctypes.c_int32(0)
+        nat.check(nat.lib().wfs_get_sums(self.handle, *(nat.ptr(out[k]) for k in names), ctypes.byref(n)))
+        return out, int(n.value)
+
+    def profile_x(self, field, z0=0, z1=None):
+        """FP64 sum over fluid cells with z0 <= z < z1 of a field, per x."""
+        out = np.empty(self.nx, np.float64)
+        z1 = self.nz if z1 is None else z1
+        nat.check(nat.lib().wfs_profile_x(self.handle, nat.FIELD[field], int(z0), int(z1), nat.ptr(out)))
+        return out
+
+    def total(self, field):
+        out = ctypes.c_double(0.0)
+        nat.check(nat.lib().wfs_total(self.handle, nat.FIELD[field], ctypes.byref(out)))
+        return float(out.value)
+
+    def check_finite(self):
+        """Raises NonFiniteState if any fluid cell holds NaN/Inf."""
+        nat.check(nat.lib().wfs_check_finite(self.handle))
+
+    def timing(self, on=True):
+        """Bracket each step-kernel launch with HIP events on the solver's stream."""
+        nat.check(nat.lib().wfs_timing_enable(self.handle, 1 if on else 0))
+
+    def timing_read(self):
+        """(summed step-kernel milliseconds, launches) since the last read."""
+        ms, n = ctypes.c_double(0.0), ctypes.c_int64(0)
+        nat.check(nat.lib().wfs_timing_read(self.handle, ctypes.byref(ms), ctypes.byref(n)))
+        return float(ms.value), int(n.value)
+
+    # ------------------------------------------------------------ reference helpers
+    def shape3(self, a):
+        return np.asarray(a).reshape(self.nz, self.ny, self.nx)
+
+    def memory_mb(self):
+        """Device memory held by this solver (MB)."""
+        return nat.lib().wfs_device_bytes(self.handle) / 1e6
