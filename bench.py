@@ -125,6 +125,9 @@ def main():
               "(x-slab decomposition is design only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    from water_flow_simulation_amd import _native
+    copy_gbs = _native.bandwidth_probe(2 << 30, 20)
+    log(f"device copy bandwidth {copy_gbs:.0f} GB/s")
     log(f"building workload {a.workload}")
     reach = build_case(a.workload)
     n_fluid = int((~reach.solid).sum())
@@ -147,7 +150,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_per_launch(),
                      "kernel": "k_step", "kernel_ms_avg": round(k_avg_s * 1e3, 5),
-                     "bytes_alg_per_launch": B_ALG[a.storage] * n_fluid},
+                     "bytes_alg_per_launch": B_ALG[a.storage] * n_fluid,
+                     "copy_bw_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
         "mass_fp64": r["mass"],
     }
     variants = {}

@@ -46,8 +46,8 @@ enum wfs_storage { WFS_STORE_F32 = 0, WFS_STORE_F16DEV = 1, WFS_STORE_BF16DEV = 
 
 typedef struct wfs_desc {
     int32_t nx, ny, nz;   /* lattice size; nx >= 3, nx*ny*nz < 2^31 */
-    float tau0;           /* molecular relaxation time (reach3d.py:383-385) */
-    double cs;            /* Smagorinsky constant; the step uses Cs^2 = (float)(cs*cs) */
+    double tau0;          /* molecular relaxation time (reach3d.py:383-385); FP32/16-bit storage rounds it to float */
+    double cs;            /* Smagorinsky constant; Cs^2 = cs*cs, rounded to float for FP32/16-bit storage */
     int32_t storage;      /* enum wfs_storage */
     int32_t block_x;      /* work-group size along x (64..256, multiple of 64); 0 = default */
     int32_t device;       /* HIP device ordinal */
@@ -154,6 +154,13 @@ int wfs_check_finite(wfs_solver* h);
  * launches since the last enable/read, then starts a new tally. */
 int wfs_timing_enable(wfs_solver* h, int32_t on);
 int wfs_timing_read(wfs_solver* h, double* kernel_ms, int64_t* launches);
+
+/* Device copy bandwidth: copies `bytes` bytes between two fresh buffers
+ * `iters` times with a grid-stride 16-byte copy kernel and reports
+ * 2*bytes*iters / elapsed in GB/s (1e9 bytes/s). The practical HBM ceiling
+ * that the step kernel's traffic is compared with; also the known-size copy
+ * used to calibrate hardware byte counters. */
+int wfs_bandwidth_probe(int32_t device, int64_t bytes, int32_t iters, double* gbs);
 
 /* Device memory held by the solver, in bytes. */
 int64_t wfs_device_bytes(const wfs_solver* h);

@@ -34,7 +34,7 @@ class NonFiniteState(WfsError):
 
 class wfs_desc(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
-                ("tau0", ctypes.c_float), ("cs", ctypes.c_double), ("storage", ctypes.c_int32),
+                ("tau0", ctypes.c_double), ("cs", ctypes.c_double), ("storage", ctypes.c_int32),
                 ("block_x", ctypes.c_int32), ("device", ctypes.c_int32)]
 
 
@@ -71,6 +71,7 @@ _SIGS = {
     "wfs_check_finite": ([_P], ctypes.c_int),
     "wfs_timing_enable": ([_P, _I32], ctypes.c_int),
     "wfs_timing_read": ([_P, _P, _P], ctypes.c_int),
+    "wfs_bandwidth_probe": ([_I32, ctypes.c_int64, _I32, _P], ctypes.c_int),
     "wfs_device_bytes": ([_P], ctypes.c_int64),
 }
 
@@ -121,6 +122,13 @@ def device_count() -> int:
     n = ctypes.c_int32(0)
     check(lib().wfs_device_count(ctypes.byref(n)))
     return int(n.value)
+
+
+def bandwidth_probe(nbytes=2 << 30, iters=20, device=0) -> float:
+    """Device-to-device copy bandwidth in GB/s (see wfs_bandwidth_probe)."""
+    out = ctypes.c_double(0.0)
+    check(lib().wfs_bandwidth_probe(int(device), int(nbytes), int(iters), ctypes.byref(out)))
+    return float(out.value)
 
 
 def ptr(a):

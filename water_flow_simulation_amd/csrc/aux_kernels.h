@@ -4,6 +4,10 @@
 // per 20-step chunk (uref, time-mean sums) or once per readout (FP64
 // reductions), so these kernels are simple and bandwidth-light; the step
 // kernel in step_kernels.h is the hot path.
+//
+// Device arrays use the padded layout cell = x + px*(y + ny*z) (px >= nx, a
+// multiple of 64); the host exchanges dense arrays (pitch nx) through
+// k_pad / k_unpad.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,16 +17,36 @@
 
 namespace wfs {
 
+// dense (pitch nx) -> padded (pitch px); padding cells get `fill`.
+template <typename S, typename D>
+__global__ void k_pad(const S* __restrict__ dense, D* __restrict__ padded, int nx, int px, long long Npad, D fill) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Npad) return;
+    const long long j = c / px;
+    const int x = int(c - j * px);
+    padded[c] = x < nx ? (D)dense[x + (long long)nx * j] : fill;
+}
+
+// padded -> dense
+template <typename S, typename D>
+__global__ void k_unpad(const S* __restrict__ padded, D* __restrict__ dense, int nx, int px, long long Ndense) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= Ndense) return;
+    const long long j = k / nx;
+    const int x = int(k - j * nx);
+    dense[k] = (D)padded[x + (long long)px * j];
+}
+
 // Equilibrium at rho = 1, u = (u0, 0, 0), evaluated in double and stored once
 // through the codec; u0 = uin of the cell's row, 0 in solid cells
 // (floodsim/lbm3d.py:362-364, 377-384).
 template <class Codec>
-__global__ void k_init(typename Codec::T* __restrict__ f, const uint8_t* __restrict__ solid,
-                       const typename Codec::R* __restrict__ uin, int nx, int ny, int nz, long long N) {
+__global__ void k_init(typename Codec::T* __restrict__ f, const uint8_t* __restrict__ flag,
+                       const typename Codec::R* __restrict__ uin, int px, long long N) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
-    const long long j = c / nx;  // row index y + ny*z
-    const double u = solid[c] ? 0.0 : (double)uin[j];
+    const long long j = c / px;  // row index y + ny*z
+    const double u = (flag[c] & FLAG_SOLID) ? 0.0 : (double)uin[j];
     const double W[3] = {1.0 / 3, 1.0 / 18, 1.0 / 36};
     static_for<Q>([&](auto I) {
         constexpr int i = I;
@@ -33,20 +57,22 @@ __global__ void k_init(typename Codec::T* __restrict__ f, const uint8_t* __restr
     });
 }
 
-// Cell flags and wall-link masks. A link i of fluid cell c is a wall link when
-// its pull source lies inside [0, nx) in x but outside the y range, below the
-// bed (z < 0) or on a solid cell; sources above the lid are mirrored into the
-// same layer first (floodsim/lbm3d.py:50-58 applies the same rules per step).
+// Cell flags and wall-link masks from the dense solid mask. A link i of fluid
+// cell c is a wall link when its pull source lies inside [0, nx) in x but
+// outside the y range, below the bed (z < 0) or on a solid cell; sources
+// above the lid are mirrored into the same layer first
+// (floodsim/lbm3d.py:50-58 applies the same rules per step). Padding cells
+// (x >= nx) are flagged solid.
 template <typename R>
 __global__ void k_flags(const uint8_t* __restrict__ solid, const R* __restrict__ cfw, uint8_t* __restrict__ flag,
-                        uint32_t* __restrict__ wmask, int nx, int ny, int nz) {
+                        uint32_t* __restrict__ wmask, int nx, int px, int ny, int nz) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= nx) return;
+    if (x >= px) return;
     const int y = blockIdx.y, z = blockIdx.z;
-    const long long c = x + (long long)nx * (y + (long long)ny * z);
+    const long long c = x + (long long)px * (y + (long long)ny * z);
     uint8_t fl = 0;
     uint32_t wm = 0;
-    if (solid[c]) {
+    if (x >= nx || solid[x + (long long)nx * (y + (long long)ny * z)]) {
         fl = FLAG_SOLID;
     } else {
         for (int i = 0; i < Q; ++i) {
@@ -81,10 +107,12 @@ __global__ void k_forced(uint8_t* __restrict__ flag, const R* __restrict__ Fx, c
 // z = 0..nz-1 in order.
 template <typename R>
 __global__ void k_uref(const R* __restrict__ ux, const R* __restrict__ uy, const uint8_t* __restrict__ flag,
-                       const uint8_t* __restrict__ bedwall, R* __restrict__ uref, int nx, int ny, int nz) {
-    const long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long plane = (long long)nx * ny;
-    if (col >= plane) return;
+                       const uint8_t* __restrict__ bedwall, R* __restrict__ uref, int nx, int px, int ny, int nz) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= nx) return;
+    const long long col = x + (long long)px * y;
+    const long long plane = (long long)px * ny;
     R sx = R(0), sy = R(0), nfl = R(0);
     for (int z = 0; z < nz; ++z) {
         const long long c = col + plane * z;
@@ -129,24 +157,30 @@ __global__ void k_fill_macro(R* __restrict__ rho, R* __restrict__ ux, R* __restr
     uz[c] = R(0);
 }
 
-// Population decode/encode between storage and a plain real array.
+// Populations: padded storage -> dense real array f[i*Nd + k], and back.
 template <class Codec, typename O>
-__global__ void k_decode(const typename Codec::T* __restrict__ f, O* __restrict__ out, long long N) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
+__global__ void k_decode(const typename Codec::T* __restrict__ f, O* __restrict__ out, int nx, int px,
+                         long long Npad, long long Nd) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= Nd) return;
+    const long long j = k / nx;
+    const long long c = (k - j * nx) + (long long)px * j;
     static_for<Q>([&](auto I) {
         constexpr int i = I;
-        out[(long long)i * N + c] = (O)Codec::template ld<i>(f + (long long)i * N + c);
+        out[(long long)i * Nd + k] = (O)Codec::template ld<i>(f + (long long)i * Npad + c);
     });
 }
 
 template <class Codec>
-__global__ void k_encode(typename Codec::T* __restrict__ f, const float* __restrict__ in, long long N) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
+__global__ void k_encode(typename Codec::T* __restrict__ f, const float* __restrict__ in, int nx, int px,
+                         long long Npad, long long Nd) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= Nd) return;
+    const long long j = k / nx;
+    const long long c = (k - j * nx) + (long long)px * j;
     static_for<Q>([&](auto I) {
         constexpr int i = I;
-        Codec::template st<i>(f + (long long)i * N + c, (typename Codec::R)in[(long long)i * N + c]);
+        Codec::template st<i>(f + (long long)i * Npad + c, (typename Codec::R)in[(long long)i * Nd + k]);
     });
 }
 
@@ -154,7 +188,7 @@ __global__ void k_encode(typename Codec::T* __restrict__ f, const float* __restr
 // pass 1 writes one partial per (x, chunk of rows), pass 2 adds the chunks
 // in order. Deterministic (no atomics).
 template <typename R>
-__global__ void k_profile_partial(const R* __restrict__ v, const uint8_t* __restrict__ flag, int nx, int j0,
+__global__ void k_profile_partial(const R* __restrict__ v, const uint8_t* __restrict__ flag, int nx, int px, int j0,
                                   int j1, int rows_per_chunk, double* __restrict__ partial) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= nx) return;
@@ -163,7 +197,7 @@ __global__ void k_profile_partial(const R* __restrict__ v, const uint8_t* __rest
     const int b = min(a + rows_per_chunk, j1);
     double s = 0.0;
     for (int j = a; j < b; ++j) {
-        const long long c = x + (long long)nx * j;
+        const long long c = x + (long long)px * j;
         if (!(flag[c] & FLAG_SOLID)) s += v ? (double)v[c] : 1.0;
     }
     partial[(long long)chunk * nx + x] = s;

@@ -4,7 +4,10 @@
 // single (y, z) row, so every population load and store of a wavefront is a
 // contiguous 64-cell run (pull streaming: the x-shift of a neighbour row only
 // moves the run by one element). Populations are structure-of-arrays
-// f[i*N + cell], read from `fs` and written to `fd` (A-B ping-pong).
+// f[i*N + cell], read from `fs` and written to `fd` (A-B ping-pong), with
+// cell = x + px*(y + ny*z): rows are padded to a pitch px that is a multiple
+// of 64 cells, so every row starts on a 256-byte boundary (padding cells are
+// flagged solid and never touched by the step).
 //
 // Per cell (floodsim/lbm3d.py:38-137 describes the same update):
 //   1. pull the 19 populations; x-edge cells keep their own value for sources
@@ -142,9 +145,24 @@ struct StepArgs {
     R tau0sq;      // tau0 * tau0
     R ksmag;       // (18 * 1.41421356) * Cs^2
     int nx, ny, nz;
-    long long N;
+    int px;        // row pitch in cells (nx rounded up to a multiple of 64)
+    int rows;      // ny * nz
+    int tiles;     // x-tiles per row (ceil(nx / blockDim.x))
+    long long N;   // cells per population array, px * ny * nz
     int write_all;
 };
+
+// Work-group -> (row, x-tile) map. Work-groups are dispatched round-robin over
+// the 8 XCDs (group L runs on XCD L % 8), each with its own L2. All x-tiles of
+// a row get the same L % 8, so the lines a row's tiles share at their x-edges
+// (the +-1 pull shift) and its partial-line writes meet in one L2.
+__device__ __forceinline__ bool tile_of(int tiles, int rows, int& row, int& tile) {
+    const unsigned L = blockIdx.x;
+    const unsigned q = L >> 3;
+    tile = int(q % unsigned(tiles));
+    row = int((q / unsigned(tiles)) * 8u + (L & 7u));
+    return row < rows;
+}
 
 // ---------------------------------------------------------------- the step
 template <class Codec>
@@ -153,41 +171,52 @@ __global__ void __launch_bounds__(256) k_step(const typename Codec::T* __restric
                                               StepArgs<typename Codec::R> a) {
     using R = typename Codec::R;
     using T = typename Codec::T;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int row, tile;
+    if (!tile_of(a.tiles, a.rows, row, tile)) return;
+    const int x = tile * int(blockDim.x) + int(threadIdx.x);
     if (x >= a.nx) return;
-    const int y = blockIdx.y, z = blockIdx.z;
-    const int nx = a.nx;
-    const int plane = a.nx * a.ny;
-    const int c = x + nx * (y + a.ny * z);
+    const int z = row / a.ny;
+    const int px = a.px;
+    const int plane = px * a.ny;
+    const int c = x + px * row;
     const uint8_t fl = a.flag[c];
     if (fl & FLAG_SOLID) return;
     const uint32_t wm = (fl & FLAG_NEARWALL) ? a.wmask[c] : 0u;
-    const bool xlo = x == 0, xhi = x == nx - 1, top = z == a.nz - 1;
+    const bool xlo = x == 0, xhi = x == a.nx - 1, top = z == a.nz - 1;
     const long long N = a.N;
 
     // 1. pull streaming
     R f[Q];
-    static_for<Q>([&](auto I) {
-        constexpr int i = I;
-        constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
-        const bool xout = (cx == 1 && xlo) || (cx == -1 && xhi);
-        const bool wall = (wm >> i) & 1u;
-        int dir = i, s = c;
-        if (xout) {
-            dir = i;
-            s = c;
-        } else if (wall) {
-            dir = kOPP[i];
-            s = c;
-        } else if (cz == -1 && top) {
-            dir = kMIRZ[i];
-            s = c - cx - nx * cy;
-        } else {
-            s = c - cx - nx * cy - plane * cz;
-        }
-        // i, OPP[i] and MIRZ[i] share one lattice weight, so one decoder serves all three.
-        f[i] = Codec::template ld<i>(fs + (long long)dir * N + s);
-    });
+    if (!(fl & FLAG_NEARWALL) && !xlo && !xhi && !top) {
+        // interior: every source is the plain upstream neighbour
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            const int s = c - kCX[i] - px * kCY[i] - plane * kCZ[i];
+            f[i] = Codec::template ld<i>(fs + (long long)i * N + s);
+        });
+    } else {
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+            const bool xout = (cx == 1 && xlo) || (cx == -1 && xhi);
+            const bool wall = (wm >> i) & 1u;
+            int dir = i, s = c;
+            if (xout) {
+                dir = i;
+                s = c;
+            } else if (wall) {
+                dir = kOPP[i];
+                s = c;
+            } else if (cz == -1 && top) {
+                dir = kMIRZ[i];
+                s = c - cx - px * cy;
+            } else {
+                s = c - cx - px * cy - plane * cz;
+            }
+            // i, OPP[i] and MIRZ[i] share one lattice weight, so one decoder serves all three.
+            f[i] = Codec::template ld<i>(fs + (long long)dir * N + s);
+        });
+    }
 
     // 2. wall function
     if (wm != 0u && (fl & FLAG_CF)) {
@@ -341,12 +370,12 @@ __device__ __forceinline__ typename Codec::R feq_generic(int i, typename Codec::
 
 template <class Codec>
 __global__ void __launch_bounds__(256) k_bc(typename Codec::T* __restrict__ fd, const uint8_t* __restrict__ flag,
-                                            const typename Codec::R* __restrict__ uin, int nx, int ny, int nz,
+                                            const typename Codec::R* __restrict__ uin, int nx, int px, int rows,
                                             long long N, typename Codec::R rho_out) {
     using R = typename Codec::R;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ny * nz) return;
-    const long long row = (long long)nx * j;  // j = y + ny*z
+    if (j >= rows) return;
+    const long long row = (long long)px * j;  // j = y + ny*z
     for (int side = 0; side < 2; ++side) {
         const long long c0 = side == 0 ? row : row + nx - 1;
         const long long c1 = side == 0 ? row + 1 : row + nx - 2;

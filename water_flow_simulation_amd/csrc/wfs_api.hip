@@ -35,6 +35,12 @@ int fail(int code, const std::string& msg) {
 
 inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+__global__ void __launch_bounds__(256) k_copy16(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                long long n) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) dst[k] = src[k];
+}
+
 __global__ void k_fill_zero_u32(uint32_t* p, long long n) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = 0u;
@@ -53,7 +59,9 @@ struct DevBuf {
 
 struct wfs_solver {
     wfs_desc d{};
-    long long N = 0;
+    long long N = 0;    // dense cell count nx*ny*nz (host arrays)
+    long long Np = 0;   // device cell count px*ny*nz (rows padded to px)
+    int px = 0;         // row pitch: nx rounded up to a multiple of 64
     int64_t nstep = 0;
     int32_t navg = 0;
     int device = 0;
@@ -140,16 +148,17 @@ struct Solver final : wfs_solver {
     using T = typename Codec::T;
     using R = typename Codec::R;
     DevBuf f[2], flag, wmask, Fx, Fy, Fz, rho, ux, uy, uz, cfw, uref, uin, bedwall;
-    DevBuf sums[6], partial, out_x, counter;
+    DevBuf sums[6], partial, out_x, counter, stage;
     int cur = 0;
     R fscale = R(1);
     int bx = 128;
 
     T* fp(int k) { return static_cast<T*>(f[k].p); }
     R* rp(DevBuf& b) { return static_cast<R*>(b.p); }
+    const uint8_t* flags() { return static_cast<const uint8_t*>(flag.p); }
 
     int alloc() override {
-        const size_t n = (size_t)N;
+        const size_t n = (size_t)Np;
         int rc;
         for (int k = 0; k < 2; ++k)
             if ((rc = alloc_buf(f[k], n * Q * sizeof(T)))) return rc;
@@ -157,45 +166,62 @@ struct Solver final : wfs_solver {
         if ((rc = alloc_buf(wmask, n * sizeof(uint32_t)))) return rc;
         for (DevBuf* b : {&Fx, &Fy, &Fz, &rho, &ux, &uy, &uz, &cfw, &uref})
             if ((rc = alloc_buf(*b, n * sizeof(R)))) return rc;
-        if ((rc = alloc_buf(uin, (size_t)d.ny * d.nz * sizeof(R)))) return rc;
+        if ((rc = alloc_buf(uin, (size_t)rows() * sizeof(R)))) return rc;
         if ((rc = alloc_buf(bedwall, n))) return rc;
         for (auto& s : sums)
             if ((rc = alloc_buf(s, n * sizeof(R)))) return rc;
         if ((rc = alloc_buf(out_x, (size_t)d.nx * sizeof(double)))) return rc;
         if ((rc = alloc_buf(partial, (size_t)d.nx * 256 * sizeof(double)))) return rc;
         if ((rc = alloc_buf(counter, 256))) return rc;
+        if ((rc = alloc_buf(stage, (size_t)N * sizeof(double)))) return rc;  // dense host <-> device staging
         return WFS_OK;
     }
 
-    // host float -> device R
-    int put(DevBuf& b, const float* src, size_t n) {
-        if constexpr (std::is_same<R, float>::value) {
-            WFS_HIP(hipMemcpyAsync(b.p, src, n * sizeof(float), hipMemcpyHostToDevice, stream));
-            WFS_HIP(hipStreamSynchronize(stream));
-        } else {
-            std::vector<R> tmp(src, src + n);
-            WFS_HIP(hipMemcpyAsync(b.p, tmp.data(), n * sizeof(R), hipMemcpyHostToDevice, stream));
-            WFS_HIP(hipStreamSynchronize(stream));
-        }
+    int rows() const { return d.ny * d.nz; }
+
+    // host dense float cells -> device padded R cells
+    int put_cells(DevBuf& b, const float* src) {
+        WFS_HIP(hipMemcpyAsync(stage.p, src, (size_t)N * sizeof(float), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL((k_pad<float, R>), dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
+                           static_cast<const float*>(stage.p), rp(b), d.nx, px, Np, R(0));
+        WFS_LAUNCH_CHECK();
+        WFS_HIP(hipStreamSynchronize(stream));
         return WFS_OK;
     }
 
-    // device R -> host float or double
-    int get(DevBuf& b, void* dst, size_t n, bool f64) {
+    // host dense u8 cells -> device padded u8 cells
+    int put_bytes(DevBuf& b, const uint8_t* src) {
+        WFS_HIP(hipMemcpyAsync(stage.p, src, (size_t)N, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL((k_pad<uint8_t, uint8_t>), dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
+                           static_cast<const uint8_t*>(stage.p), static_cast<uint8_t*>(b.p), d.nx, px, Np,
+                           uint8_t(0));
+        WFS_LAUNCH_CHECK();
+        WFS_HIP(hipStreamSynchronize(stream));
+        return WFS_OK;
+    }
+
+    // host float rows (ny*nz) -> device R rows
+    int put_rows(DevBuf& b, const float* src) {
+        const size_t n = (size_t)rows();
+        std::vector<R> tmp(src, src + n);
+        WFS_HIP(hipMemcpyAsync(b.p, tmp.data(), n * sizeof(R), hipMemcpyHostToDevice, stream));
+        WFS_HIP(hipStreamSynchronize(stream));
+        return WFS_OK;
+    }
+
+    // device padded R cells -> host dense float or double cells
+    int get_cells(DevBuf& b, void* dst, bool f64) {
         if (!dst) return WFS_OK;
-        const bool same = f64 ? std::is_same<R, double>::value : std::is_same<R, float>::value;
-        if (same) {
-            WFS_HIP(hipMemcpyAsync(dst, b.p, n * sizeof(R), hipMemcpyDeviceToHost, stream));
-            WFS_HIP(hipStreamSynchronize(stream));
-        } else {
-            std::vector<R> tmp(n);
-            WFS_HIP(hipMemcpyAsync(tmp.data(), b.p, n * sizeof(R), hipMemcpyDeviceToHost, stream));
-            WFS_HIP(hipStreamSynchronize(stream));
-            if (f64)
-                for (size_t k = 0; k < n; ++k) static_cast<double*>(dst)[k] = (double)tmp[k];
-            else
-                for (size_t k = 0; k < n; ++k) static_cast<float*>(dst)[k] = (float)tmp[k];
-        }
+        if (f64)
+            hipLaunchKernelGGL((k_unpad<R, double>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(b),
+                               static_cast<double*>(stage.p), d.nx, px, N);
+        else
+            hipLaunchKernelGGL((k_unpad<R, float>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(b),
+                               static_cast<float*>(stage.p), d.nx, px, N);
+        WFS_LAUNCH_CHECK();
+        WFS_HIP(hipMemcpyAsync(dst, stage.p, (size_t)N * (f64 ? sizeof(double) : sizeof(float)),
+                               hipMemcpyDeviceToHost, stream));
+        WFS_HIP(hipStreamSynchronize(stream));
         return WFS_OK;
     }
 
@@ -209,29 +235,26 @@ struct Solver final : wfs_solver {
     }
 
     int upload(const uint8_t* solid, const float* cf, const float* uin_h) override {
-        const size_t n = (size_t)N;
-        WFS_HIP(hipMemcpyAsync(bedwall.p, solid, n, hipMemcpyHostToDevice, stream));  // staging for solid
+        int rc;
         if (cf) {
-            int rc = put(cfw, cf, n);
-            if (rc) return rc;
-        } else {
-            int rc = zero(cfw);
-            if (rc) return rc;
+            if ((rc = put_cells(cfw, cf))) return rc;
+        } else if ((rc = zero(cfw))) {
+            return rc;
         }
-        int rc = put(uin, uin_h, (size_t)d.ny * d.nz);
-        if (rc) return rc;
-        const dim3 gx(blocks_for(d.nx, 64), d.ny, d.nz);
-        hipLaunchKernelGGL(k_flags<R>, gx, dim3(64), 0, stream, static_cast<const uint8_t*>(bedwall.p), rp(cfw),
-                           static_cast<uint8_t*>(flag.p), static_cast<uint32_t*>(wmask.p), d.nx, d.ny, d.nz);
+        if ((rc = put_rows(uin, uin_h))) return rc;
+        // the dense solid mask stays in the staging buffer while the flags are built
+        WFS_HIP(hipMemcpyAsync(stage.p, solid, (size_t)N, hipMemcpyHostToDevice, stream));
+        const dim3 gx(blocks_for(px, 64), d.ny, d.nz);
+        hipLaunchKernelGGL(k_flags<R>, gx, dim3(64), 0, stream, static_cast<const uint8_t*>(stage.p), rp(cfw),
+                           static_cast<uint8_t*>(flag.p), static_cast<uint32_t*>(wmask.p), d.nx, px, d.ny, d.nz);
         WFS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_init<Codec>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(0),
-                           static_cast<const uint8_t*>(bedwall.p), rp(uin), d.nx, d.ny, d.nz, N);
-        WFS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_init<Codec>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(1),
-                           static_cast<const uint8_t*>(bedwall.p), rp(uin), d.nx, d.ny, d.nz, N);
-        WFS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_fill_macro<R>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(rho), rp(ux),
-                           rp(uy), rp(uz), N);
+        for (int k = 0; k < 2; ++k) {
+            hipLaunchKernelGGL(k_init<Codec>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, fp(k), flags(),
+                               rp(uin), px, Np);
+            WFS_LAUNCH_CHECK();
+        }
+        hipLaunchKernelGGL(k_fill_macro<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, rp(rho), rp(ux),
+                           rp(uy), rp(uz), Np);
         WFS_LAUNCH_CHECK();
         for (DevBuf* b : {&Fx, &Fy, &Fz, &uref, &bedwall})
             if ((rc = zero(*b))) return rc;
@@ -246,11 +269,10 @@ struct Solver final : wfs_solver {
     }
 
     int set_force(const float* a, const float* b, const float* c) override {
-        const size_t n = (size_t)N;
         int rc;
-        if ((rc = put(Fx, a, n)) || (rc = put(Fy, b, n)) || (rc = put(Fz, c, n))) return rc;
-        hipLaunchKernelGGL(k_forced<R>, dim3(blocks_for(N, 256)), dim3(256), 0, stream,
-                           static_cast<uint8_t*>(flag.p), rp(Fx), rp(Fy), rp(Fz), N);
+        if ((rc = put_cells(Fx, a)) || (rc = put_cells(Fy, b)) || (rc = put_cells(Fz, c))) return rc;
+        hipLaunchKernelGGL(k_forced<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
+                           static_cast<uint8_t*>(flag.p), rp(Fx), rp(Fy), rp(Fz), Np);
         WFS_LAUNCH_CHECK();
         return WFS_OK;
     }
@@ -260,29 +282,27 @@ struct Solver final : wfs_solver {
         return WFS_OK;
     }
 
-    int set_uin(const float* u) override { return put(uin, u, (size_t)d.ny * d.nz); }
-    int set_uref(const float* u) override { return put(uref, u, (size_t)N); }
+    int set_uin(const float* u) override { return put_rows(uin, u); }
+    int set_uref(const float* u) override { return put_cells(uref, u); }
 
     int set_bedwall(const uint8_t* bw) override {
-        WFS_HIP(hipMemcpyAsync(bedwall.p, bw, (size_t)N, hipMemcpyHostToDevice, stream));
-        WFS_HIP(hipStreamSynchronize(stream));
+        int rc = put_bytes(bedwall, bw);
+        if (rc) return rc;
         have_bedwall = true;
         return WFS_OK;
     }
 
     int update_uref() override {
         if (!have_bedwall) return fail(WFS_EINVAL, "wfs_update_uref: no bed/bank mask set (wfs_set_bedwall)");
-        const long long plane = (long long)d.nx * d.ny;
-        hipLaunchKernelGGL(k_uref<R>, dim3(blocks_for(plane, 64)), dim3(64), 0, stream, rp(ux), rp(uy),
-                           static_cast<const uint8_t*>(flag.p), static_cast<const uint8_t*>(bedwall.p), rp(uref),
-                           d.nx, d.ny, d.nz);
+        hipLaunchKernelGGL(k_uref<R>, dim3(blocks_for(d.nx, 64), d.ny), dim3(64), 0, stream, rp(ux), rp(uy), flags(),
+                           static_cast<const uint8_t*>(bedwall.p), rp(uref), d.nx, px, d.ny, d.nz);
         WFS_LAUNCH_CHECK();
         return WFS_OK;
     }
 
     int step(int n) override {
         StepArgs<R> a;
-        a.flag = static_cast<const uint8_t*>(flag.p);
+        a.flag = flags();
         a.wmask = static_cast<const uint32_t*>(wmask.p);
         a.Fx = rp(Fx);
         a.Fy = rp(Fy);
@@ -294,16 +314,21 @@ struct Solver final : wfs_solver {
         a.cfw = rp(cfw);
         a.uref = rp(uref);
         a.fscale = fscale;
-        a.tau0 = R(d.tau0);
+        // FP32 solvers see tau0 and Cs^2 rounded to float, like the float kernel arguments of
+        // lbm3d.py:403; the FP64 solver keeps them in double.
+        a.tau0 = std::is_same<R, float>::value ? R((float)d.tau0) : R(d.tau0);
         a.tau0sq = a.tau0 * a.tau0;
-        const R cs2 = R((float)(d.cs * d.cs));
+        const R cs2 = std::is_same<R, float>::value ? R((float)(d.cs * d.cs)) : R(d.cs * d.cs);
         a.ksmag = R(18) * R(1.41421356) * cs2;
         a.nx = d.nx;
         a.ny = d.ny;
         a.nz = d.nz;
-        a.N = N;
-        const dim3 grid(blocks_for(d.nx, bx), d.ny, d.nz);
-        const int rows = d.ny * d.nz;
+        a.px = px;
+        a.rows = rows();
+        a.tiles = (d.nx + bx - 1) / bx;
+        a.N = Np;
+        const long long groups = (long long)((a.rows + 7) / 8) * 8 * a.tiles;
+        const dim3 grid((unsigned)groups);
         for (int k = 0; k < n; ++k) {
             a.write_all = (k == n - 1) ? 1 : 0;
             if (timing) {
@@ -317,8 +342,8 @@ struct Solver final : wfs_solver {
                 int rc = mark();
                 if (rc) return rc;
             }
-            hipLaunchKernelGGL(k_bc<Codec>, dim3(blocks_for(rows, 64)), dim3(64), 0, stream, fp(cur ^ 1),
-                               static_cast<const uint8_t*>(flag.p), rp(uin), d.nx, d.ny, d.nz, N, R(1));
+            hipLaunchKernelGGL(k_bc<Codec>, dim3(blocks_for(a.rows, 64)), dim3(64), 0, stream, fp(cur ^ 1), flags(),
+                               rp(uin), d.nx, px, a.rows, Np, R(1));
             WFS_LAUNCH_CHECK();
             cur ^= 1;
         }
@@ -327,22 +352,21 @@ struct Solver final : wfs_solver {
     }
 
     int get_macro(void* r, void* a, void* b, void* c, bool f64) override {
-        const size_t n = (size_t)N;
         int rc;
-        if ((rc = get(rho, r, n, f64)) || (rc = get(ux, a, n, f64)) || (rc = get(uy, b, n, f64)) ||
-            (rc = get(uz, c, n, f64)))
+        if ((rc = get_cells(rho, r, f64)) || (rc = get_cells(ux, a, f64)) || (rc = get_cells(uy, b, f64)) ||
+            (rc = get_cells(uz, c, f64)))
             return rc;
         return WFS_OK;
     }
 
-    int get_uref(float* u) override { return get(uref, u, (size_t)N, false); }
-
     int set_macro(const float* r, const float* a, const float* b, const float* c) override {
-        const size_t n = (size_t)N;
         int rc;
-        if ((rc = put(rho, r, n)) || (rc = put(ux, a, n)) || (rc = put(uy, b, n)) || (rc = put(uz, c, n))) return rc;
+        if ((rc = put_cells(rho, r)) || (rc = put_cells(ux, a)) || (rc = put_cells(uy, b)) || (rc = put_cells(uz, c)))
+            return rc;
         return WFS_OK;
     }
+
+    int get_uref(float* u) override { return get_cells(uref, u, false); }
 
     int get_pops(void* out, bool f64) override {
         const size_t n = (size_t)N * Q;
@@ -355,10 +379,10 @@ struct Solver final : wfs_solver {
         }
         if (f64)
             hipLaunchKernelGGL((k_decode<Codec, double>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
-                               static_cast<double*>(tmp.p), N);
+                               static_cast<double*>(tmp.p), d.nx, px, Np, N);
         else
             hipLaunchKernelGGL((k_decode<Codec, float>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
-                               static_cast<float*>(tmp.p), N);
+                               static_cast<float*>(tmp.p), d.nx, px, Np, N);
         WFS_LAUNCH_CHECK();
         WFS_HIP(hipMemcpyAsync(out, tmp.p, n * esz, hipMemcpyDeviceToHost, stream));
         WFS_HIP(hipStreamSynchronize(stream));
@@ -375,15 +399,15 @@ struct Solver final : wfs_solver {
         }
         WFS_HIP(hipMemcpyAsync(tmp.p, in, n * sizeof(float), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(k_encode<Codec>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
-                           static_cast<const float*>(tmp.p), N);
+                           static_cast<const float*>(tmp.p), d.nx, px, Np, N);
         WFS_LAUNCH_CHECK();
         WFS_HIP(hipStreamSynchronize(stream));
         return WFS_OK;
     }
 
     int accumulate() override {
-        hipLaunchKernelGGL(k_accumulate<R>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(rho), rp(ux), rp(uy),
-                           rp(uz), rp(sums[0]), rp(sums[1]), rp(sums[2]), rp(sums[3]), rp(sums[4]), rp(sums[5]), N);
+        hipLaunchKernelGGL(k_accumulate<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, rp(rho), rp(ux), rp(uy),
+                           rp(uz), rp(sums[0]), rp(sums[1]), rp(sums[2]), rp(sums[3]), rp(sums[4]), rp(sums[5]), Np);
         WFS_LAUNCH_CHECK();
         ++navg;
         return WFS_OK;
@@ -400,7 +424,7 @@ struct Solver final : wfs_solver {
 
     int get_sums(float* const* out) override {
         for (int k = 0; k < 6; ++k) {
-            int rc = get(sums[k], out[k], (size_t)N, false);
+            int rc = get_cells(sums[k], out[k], false);
             if (rc) return rc;
         }
         return WFS_OK;
@@ -427,13 +451,12 @@ struct Solver final : wfs_solver {
         if (z0 < 0 || z1 > d.nz || z0 >= z1) return fail(WFS_EINVAL, "wfs_profile_x: bad z range");
         const R* v = field_ptr(field);
         const int j0 = z0 * d.ny, j1 = z1 * d.ny;
-        const int rows = j1 - j0;
-        const int nchunks = std::min(256, std::max(1, rows / 16));
-        const int per = (rows + nchunks - 1) / nchunks;
-        const int used = (rows + per - 1) / per;
-        hipLaunchKernelGGL(k_profile_partial<R>, dim3(blocks_for(d.nx, 64), used), dim3(64), 0, stream, v,
-                           static_cast<const uint8_t*>(flag.p), d.nx, j0, j1, per,
-                           static_cast<double*>(partial.p));
+        const int nrows = j1 - j0;
+        const int nchunks = std::min(256, std::max(1, nrows / 16));
+        const int per = (nrows + nchunks - 1) / nchunks;
+        const int used = (nrows + per - 1) / per;
+        hipLaunchKernelGGL(k_profile_partial<R>, dim3(blocks_for(d.nx, 64), used), dim3(64), 0, stream, v, flags(),
+                           d.nx, px, j0, j1, per, static_cast<double*>(partial.p));
         WFS_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_profile_final, dim3(blocks_for(d.nx, 64)), dim3(64), 0, stream,
                            static_cast<const double*>(partial.p), d.nx, used, static_cast<double*>(out_x.p));
@@ -450,11 +473,11 @@ struct Solver final : wfs_solver {
     }
 
     int total(int field, double* out) override {
-        std::vector<double> px((size_t)d.nx);
-        int rc = profile_x(field, 0, d.nz, px.data());
+        std::vector<double> pxs((size_t)d.nx);
+        int rc = profile_x(field, 0, d.nz, pxs.data());
         if (rc) return rc;
         double s = 0.0;
-        for (double v : px) s += v;
+        for (double v : pxs) s += v;
         *out = s;
         return WFS_OK;
     }
@@ -462,8 +485,8 @@ struct Solver final : wfs_solver {
     int check_finite() override {
         int rc = zero(counter);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_nonfinite<R>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(rho), rp(ux), rp(uy),
-                           rp(uz), static_cast<const uint8_t*>(flag.p), N, static_cast<unsigned int*>(counter.p));
+        hipLaunchKernelGGL(k_nonfinite<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, rp(rho), rp(ux), rp(uy),
+                           rp(uz), flags(), Np, static_cast<unsigned int*>(counter.p));
         WFS_LAUNCH_CHECK();
         unsigned int cnt = 0;
         WFS_HIP(hipMemcpyAsync(&cnt, counter.p, sizeof(cnt), hipMemcpyDeviceToHost, stream));
@@ -530,7 +553,7 @@ int wfs_create(const wfs_desc* desc, wfs_solver** out) {
     const long long N = (long long)d.nx * d.ny * d.nz;
     if (N >= (1ll << 31)) return fail(WFS_EINVAL, "wfs_create: nx*ny*nz must be below 2^31 cells");
     if (d.ny > 65535 || d.nz > 65535) return fail(WFS_EINVAL, "wfs_create: ny and nz must be at most 65535");
-    if (!(d.tau0 > 0.5f) || !std::isfinite(d.tau0)) return fail(WFS_EINVAL, "wfs_create: tau0 must exceed 0.5");
+    if (!(d.tau0 > 0.5) || !std::isfinite(d.tau0)) return fail(WFS_EINVAL, "wfs_create: tau0 must exceed 0.5");
     if (!(d.cs >= 0.0) || !std::isfinite(d.cs)) return fail(WFS_EINVAL, "wfs_create: cs must be >= 0");
     const int bx = d.block_x == 0 ? 128 : d.block_x;
     if (bx < 64 || bx > 256 || bx % 64) return fail(WFS_EINVAL, "wfs_create: block_x must be 64, 128, 192 or 256");
@@ -544,6 +567,9 @@ int wfs_create(const wfs_desc* desc, wfs_solver** out) {
     if (!h) return fail(WFS_EINVAL, "wfs_create: unknown storage format");
     h->d = d;
     h->N = N;
+    h->px = (d.nx + 63) / 64 * 64;
+    h->Np = (long long)h->px * d.ny * d.nz;
+    if (h->Np >= (1ll << 31)) return fail(WFS_EINVAL, "wfs_create: padded lattice exceeds 2^31 cells");
     h->device = d.device;
     DeviceGuard g(d.device);
     WFS_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -762,6 +788,54 @@ int wfs_timing_read(wfs_solver* h, double* kernel_ms, int64_t* launches) {
     *launches = h->timed_launches;
     h->timed_ms = 0.0;
     h->timed_launches = 0;
+    return WFS_OK;
+}
+
+int wfs_bandwidth_probe(int32_t device, int64_t bytes, int32_t iters, double* gbs) {
+    if (!gbs || bytes < 16 * 1024 || iters < 1) return fail(WFS_EINVAL, "wfs_bandwidth_probe: bad arguments");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        (void)hipGetLastError();
+        return fail(WFS_ENODEV, "wfs_bandwidth_probe: no such device");
+    }
+    DeviceGuard g(device);
+    const long long n = bytes / 16;
+    DevBuf a, b;
+    if (hipMalloc(&a.p, (size_t)n * 16) != hipSuccess || hipMalloc(&b.p, (size_t)n * 16) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(WFS_ENOMEM, "wfs_bandwidth_probe: allocation failed");
+    }
+    hipStream_t st;
+    WFS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{st};
+    int ncu = 256;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+    const unsigned grid = (unsigned)std::min<long long>((n + 255) / 256, (long long)ncu * 16);
+    hipLaunchKernelGGL(k_fill_zero_u32, dim3(blocks_for(n * 4, 256)), dim3(256), 0, st, static_cast<uint32_t*>(a.p),
+                       n * 4);
+    WFS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, static_cast<const float4*>(a.p),
+                       static_cast<float4*>(b.p), n);  // warm-up
+    WFS_LAUNCH_CHECK();
+    hipEvent_t e0, e1;
+    WFS_HIP(hipEventCreate(&e0));
+    WFS_HIP(hipEventCreate(&e1));
+    WFS_HIP(hipEventRecord(e0, st));
+    for (int k = 0; k < iters; ++k) {
+        hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, static_cast<const float4*>(a.p),
+                           static_cast<float4*>(b.p), n);
+        WFS_LAUNCH_CHECK();
+    }
+    WFS_HIP(hipEventRecord(e1, st));
+    WFS_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    WFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbs = 2.0 * (double)n * 16 * iters / (ms * 1e-3) / 1e9;
     return WFS_OK;
 }
 
