@@ -11,7 +11,8 @@ import re
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
-LIB_PATH = os.path.join(PKG, "libwfs.so")
+# WFS_LIB selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("WFS_LIB") or os.path.join(PKG, "libwfs.so")
 HEADER = os.path.join(ROOT, "include", "wfs.h")
 
 WFS_OK, WFS_EINVAL, WFS_ENOMEM, WFS_EHIP, WFS_ENODEV, WFS_ENONFINITE = 0, -1, -2, -3, -4, -5

@@ -43,13 +43,31 @@ namespace wfs {
 
 // ---------------------------------------------------------------- storage codecs
 // Populations are stored as T and computed in R.
+// Population stores carry the non-temporal hint: the next step reads them
+// back only after the whole lattice (GBs) has streamed through the caches,
+// so allocating them in L2 only evicts the pull halo lines still in use.
+#ifndef WFS_NT_STORE
+#define WFS_NT_STORE 1
+#endif
+
+template <typename T>
+__device__ __forceinline__ void store_pop(T* p, T v) {
+#if WFS_NT_STORE
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
 struct CodecF32 {
     using T = float;
     using R = float;
     template <int I>
     __device__ static R ld(const T* p) { return *p; }
     template <int I>
-    __device__ static void st(T* p, R v) { *p = v; }
+    __device__ static void st(T* p, R v) { store_pop(p, v); }
+    template <int I>
+    __device__ static R rt(R v) { return v; }   // value after a store/load round trip
 };
 
 struct CodecF64 {
@@ -58,7 +76,9 @@ struct CodecF64 {
     template <int I>
     __device__ static R ld(const T* p) { return *p; }
     template <int I>
-    __device__ static void st(T* p, R v) { *p = v; }
+    __device__ static void st(T* p, R v) { store_pop(p, v); }
+    template <int I>
+    __device__ static R rt(R v) { return v; }
 };
 
 // 16-bit deviation storage: T holds (f_i - w_i) * 2^SHIFT, decoded as
@@ -75,7 +95,14 @@ struct CodecF16Dev {
     }
     template <int I>
     __device__ static void st(T* p, R v) {
-        *p = __float2half_rn((v - weight<float>(I)) * float(1 << kDevShift));
+        const __half h = __float2half_rn((v - weight<float>(I)) * float(1 << kDevShift));
+        store_pop(reinterpret_cast<unsigned short*>(p), __half_as_ushort(h));
+    }
+    template <int I>
+    __device__ static R rt(R v) {
+        return weight<float>(I) +
+               __half2float(__float2half_rn((v - weight<float>(I)) * float(1 << kDevShift))) *
+                   (1.0f / float(1 << kDevShift));
     }
 };
 
@@ -88,7 +115,14 @@ struct CodecBF16Dev {
     }
     template <int I>
     __device__ static void st(T* p, R v) {
-        *p = __float2bfloat16((v - weight<float>(I)) * float(1 << kDevShift));
+        const __hip_bfloat16 h = __float2bfloat16((v - weight<float>(I)) * float(1 << kDevShift));
+        store_pop(reinterpret_cast<unsigned short*>(p), __bfloat16_as_ushort(h));
+    }
+    template <int I>
+    __device__ static R rt(R v) {
+        return weight<float>(I) +
+               __bfloat162float(__float2bfloat16((v - weight<float>(I)) * float(1 << kDevShift))) *
+                   (1.0f / float(1 << kDevShift));
     }
 };
 
@@ -149,6 +183,8 @@ struct StepArgs {
     int rows;      // ny * nz
     int tiles;     // x-tiles per row (ceil(nx / blockDim.x))
     long long N;   // cells per population array, px * ny * nz
+    const R* __restrict__ uin;  // inlet velocity per row (ny*nz)
+    int bc_out_fused;           // outlet row inside the step ((nx-1) % 64 != 0)
     int write_all;
 };
 
@@ -165,193 +201,265 @@ __device__ __forceinline__ bool tile_of(int tiles, int rows, int& row, int& tile
 }
 
 // ---------------------------------------------------------------- the step
+// Generic equilibrium with full lattice products, the form the boundary rows use.
+template <typename R>
+__device__ __forceinline__ R feq_full(int i, R r, R ux, R uy, R uz) {
+    const R cu = R(3) * (R(kCX[i]) * ux + R(kCY[i]) * uy + R(kCZ[i]) * uz);
+    return weight<R>(i) * r * (R(1) + cu + R(0.5) * cu * cu - R(1.5) * (ux * ux + uy * uy + uz * uz));
+}
+
+// Inlet (side 0) or outlet (side 1) row by non-equilibrium extrapolation from
+// the neighbour's post-collision populations v (as stored):
+//   inlet  f(0)    = feq(rho1, (uin, 0, 0)) + v - feq(rho1, u1)
+//   outlet f(nx-1) = feq(rho_out, u1)      + v - feq(rho1, u1)
+// with rho1, u1 the moments of v (floodsim/lbm3d.py:145-175 defines these rows).
+template <typename R>
+__device__ __forceinline__ void boundary_row(R* f, const R* v, int side, R uin, R rho_out) {
+    R r = R(0), mx = R(0), my = R(0), mz = R(0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        r += v[i];
+        mx += v[i] * R(kCX[i]);
+        my += v[i] * R(kCY[i]);
+        mz += v[i] * R(kCZ[i]);
+    }
+    const R ux = mx / r, uy = my / r, uz = mz / r;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const R e = side == 0 ? feq_full<R>(i, r, uin, R(0), R(0)) : feq_full<R>(i, rho_out, ux, uy, uz);
+        f[i] = e + (v[i] - feq_full<R>(i, r, ux, uy, uz));
+    }
+}
+
+#ifndef WFS_STEP_WAVES_PER_EU
+#define WFS_STEP_WAVES_PER_EU 1   // minimum resident waves per SIMD requested from the register allocator
+#endif
+
 template <class Codec>
-__global__ void __launch_bounds__(256) k_step(const typename Codec::T* __restrict__ fs,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WFS_STEP_WAVES_PER_EU)))
+k_step(const typename Codec::T* __restrict__ fs,
                                               typename Codec::T* __restrict__ fd,
                                               StepArgs<typename Codec::R> a) {
     using R = typename Codec::R;
-    using T = typename Codec::T;
     int row, tile;
-    if (!tile_of(a.tiles, a.rows, row, tile)) return;
+    if (!tile_of(a.tiles, a.rows, row, tile)) return;   // whole work-group
     const int x = tile * int(blockDim.x) + int(threadIdx.x);
-    if (x >= a.nx) return;
+    const bool inrow = x < a.nx;
     const int z = row / a.ny;
     const int px = a.px;
     const int plane = px * a.ny;
     const int c = x + px * row;
-    const uint8_t fl = a.flag[c];
-    if (fl & FLAG_SOLID) return;
-    const uint32_t wm = (fl & FLAG_NEARWALL) ? a.wmask[c] : 0u;
-    const bool xlo = x == 0, xhi = x == a.nx - 1, top = z == a.nz - 1;
+    // Lanes outside the row or on solid cells do no work but stay in the wave
+    // for the boundary-row shuffles below.
+    const uint8_t fl = inrow ? a.flag[c] : uint8_t(FLAG_SOLID);
+    const bool active = !(fl & FLAG_SOLID);
     const long long N = a.N;
-
-    // 1. pull streaming
     R f[Q];
-    if (!(fl & FLAG_NEARWALL) && !xlo && !xhi && !top) {
-        // interior: every source is the plain upstream neighbour
-        static_for<Q>([&](auto I) {
-            constexpr int i = I;
-            const int s = c - kCX[i] - px * kCY[i] - plane * kCZ[i];
-            f[i] = Codec::template ld<i>(fs + (long long)i * N + s);
-        });
-    } else {
-        static_for<Q>([&](auto I) {
-            constexpr int i = I;
-            constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
-            const bool xout = (cx == 1 && xlo) || (cx == -1 && xhi);
-            const bool wall = (wm >> i) & 1u;
-            int dir = i, s = c;
-            if (xout) {
-                dir = i;
-                s = c;
-            } else if (wall) {
-                dir = kOPP[i];
-                s = c;
-            } else if (cz == -1 && top) {
-                dir = kMIRZ[i];
-                s = c - cx - px * cy;
-            } else {
-                s = c - cx - px * cy - plane * cz;
-            }
-            // i, OPP[i] and MIRZ[i] share one lattice weight, so one decoder serves all three.
-            f[i] = Codec::template ld<i>(fs + (long long)dir * N + s);
-        });
-    }
 
-    // 2. wall function
-    if (wm != 0u && (fl & FLAG_CF)) {
-        const R cf = a.cfw[c];
-        int inx = 0, iny = 0, inz = 0;
-        static_for<Q>([&](auto I) {
-            constexpr int i = I;
-            if ((wm >> i) & 1u) { inx -= kCX[i]; iny -= kCY[i]; inz -= kCZ[i]; }
-        });
-        const R nwx = R(inx), nwy = R(iny), nwz = R(inz);
-        const R nn = sqrt(nwx * nwx + nwy * nwy + nwz * nwz);
-        const R afx = R(((wm >> 1) & 1u) + ((wm >> 2) & 1u));
-        const R afy = R(((wm >> 3) & 1u) + ((wm >> 4) & 1u));
-        const R afz = R(((wm >> 5) & 1u) + ((wm >> 6) & 1u));
-        const R area = sqrt(afx * afx + afy * afy + afz * afz);
-        if (nn > R(0) && area > R(0)) {
-            const R ex = nwx / nn, ey = nwy / nn, ez = nwz / nn;
-            const R px = a.ux[c], py = a.uy[c], pz = a.uz[c];
-            const R un = px * ex + py * ey + pz * ez;
-            R tx = px - un * ex, ty = py - un * ey, tz = pz - un * ez;
-            const R ut = sqrt(tx * tx + ty * ty + tz * tz);
-            if (ut > R(1e-7)) {
-                tx /= ut; ty /= ut; tz /= ut;
-                const R rw = a.rho[c];
-                R D = R(0), K = R(0);
-                static_for<Q>([&](auto I) {
-                    constexpr int i = I;
-                    if ((wm >> i) & 1u) {
-                        const R ct = dot3<kCX[i], kCY[i], kCZ[i]>(tx, ty, tz);
-                        D += ct * f[i];
-                        K += weight<R>(i) * ct * ct;
-                    }
-                });
-                if (K > R(0)) {
-                    const R ur = a.uref[c];
-                    const R U = ur > R(0) ? ur : ut;
-                    const R us0 = (R(-2) * D - rw * cf * U * U * area) / (R(6) * rw * K);
-                    const R us = fmin(fmax(us0, -ut), ut);
+    if (active) {
+        const uint32_t wm = (fl & FLAG_NEARWALL) ? a.wmask[c] : 0u;
+        const bool xlo = x == 0, xhi = x == a.nx - 1, top = z == a.nz - 1;
+        // 1. pull streaming
+        if (!(fl & FLAG_NEARWALL) && !xlo && !xhi && !top) {
+            // interior: every source is the plain upstream neighbour
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                const int s = c - kCX[i] - px * kCY[i] - plane * kCZ[i];
+                f[i] = Codec::template ld<i>(fs + (long long)i * N + s);
+            });
+        } else {
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+                const bool xout = (cx == 1 && xlo) || (cx == -1 && xhi);
+                const bool wall = (wm >> i) & 1u;
+                int dir = i, s = c;
+                if (xout) {
+                    dir = i;
+                    s = c;
+                } else if (wall) {
+                    dir = kOPP[i];
+                    s = c;
+                } else if (cz == -1 && top) {
+                    dir = kMIRZ[i];
+                    s = c - cx - px * cy;
+                } else {
+                    s = c - cx - px * cy - plane * cz;
+                }
+                // i, OPP[i] and MIRZ[i] share one lattice weight, so one decoder serves all three.
+                f[i] = Codec::template ld<i>(fs + (long long)dir * N + s);
+            });
+        }
+
+        // 2. wall function
+        if (wm != 0u && (fl & FLAG_CF)) {
+            const R cf = a.cfw[c];
+            int inx = 0, iny = 0, inz = 0;
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                if ((wm >> i) & 1u) { inx -= kCX[i]; iny -= kCY[i]; inz -= kCZ[i]; }
+            });
+            const R nwx = R(inx), nwy = R(iny), nwz = R(inz);
+            const R nn = sqrt(nwx * nwx + nwy * nwy + nwz * nwz);
+            const R afx = R(((wm >> 1) & 1u) + ((wm >> 2) & 1u));
+            const R afy = R(((wm >> 3) & 1u) + ((wm >> 4) & 1u));
+            const R afz = R(((wm >> 5) & 1u) + ((wm >> 6) & 1u));
+            const R area = sqrt(afx * afx + afy * afy + afz * afz);
+            if (nn > R(0) && area > R(0)) {
+                const R ex = nwx / nn, ey = nwy / nn, ez = nwz / nn;
+                const R ox = a.ux[c], oy = a.uy[c], oz = a.uz[c];   // previous step, own cell
+                const R un = ox * ex + oy * ey + oz * ez;
+                R tx = ox - un * ex, ty = oy - un * ey, tz = oz - un * ez;
+                const R ut = sqrt(tx * tx + ty * ty + tz * tz);
+                if (ut > R(1e-7)) {
+                    tx /= ut; ty /= ut; tz /= ut;
+                    const R rw = a.rho[c];
+                    R D = R(0), K = R(0);
                     static_for<Q>([&](auto I) {
                         constexpr int i = I;
                         if ((wm >> i) & 1u) {
-                            constexpr R c6w = R(6) * weight<R>(i);
-                            f[i] = f[i] + c6w * rw * dot3<kCX[i], kCY[i], kCZ[i]>(tx, ty, tz) * us;
+                            const R ct = dot3<kCX[i], kCY[i], kCZ[i]>(tx, ty, tz);
+                            D += ct * f[i];
+                            K += weight<R>(i) * ct * ct;
                         }
                     });
+                    if (K > R(0)) {
+                        const R ur = a.uref[c];
+                        const R U = ur > R(0) ? ur : ut;
+                        const R us0 = (R(-2) * D - rw * cf * U * U * area) / (R(6) * rw * K);
+                        const R us = fmin(fmax(us0, -ut), ut);
+                        static_for<Q>([&](auto I) {
+                            constexpr int i = I;
+                            if ((wm >> i) & 1u) {
+                                constexpr R c6w = R(6) * weight<R>(i);
+                                f[i] = f[i] + c6w * rw * dot3<kCX[i], kCY[i], kCZ[i]>(tx, ty, tz) * us;
+                            }
+                        });
+                    }
                 }
             }
         }
-    }
 
-    // 3. moments (Guo half-force shift)
-    R r = f[0], mx = R(0), my = R(0), mz = R(0);
-    bool mx0 = true, my0 = true, mz0 = true;
-    static_for<Q - 1>([&](auto J) {
-        constexpr int i = J + 1;
-        r = r + f[i];
-        // first nonzero term starts the sum exactly (0 + v == v)
-        if constexpr (kCX[i] != 0) { mx = mx0 ? (kCX[i] > 0 ? f[i] : -f[i]) : accum<kCX[i]>(mx, f[i]); mx0 = false; }
-        if constexpr (kCY[i] != 0) { my = my0 ? (kCY[i] > 0 ? f[i] : -f[i]) : accum<kCY[i]>(my, f[i]); my0 = false; }
-        if constexpr (kCZ[i] != 0) { mz = mz0 ? (kCZ[i] > 0 ? f[i] : -f[i]) : accum<kCZ[i]>(mz, f[i]); mz0 = false; }
-    });
-    const bool forced = (fl & FLAG_FORCED) != 0;
-    R fx = R(0), fy = R(0), fz = R(0);
-    R ux, uy, uz;
-    if (forced) {
-        fx = a.Fx[c] * a.fscale; fy = a.Fy[c] * a.fscale; fz = a.Fz[c] * a.fscale;
-        ux = (mx + R(0.5) * fx) / r; uy = (my + R(0.5) * fy) / r; uz = (mz + R(0.5) * fz) / r;
-    } else {
-        ux = mx / r; uy = my / r; uz = mz / r;
-    }
-    const R usq = ux * ux + uy * uy + uz * uz;
+        // 3. moments (Guo half-force shift)
+        R r = f[0], mx = R(0), my = R(0), mz = R(0);
+        bool mx0 = true, my0 = true, mz0 = true;
+        static_for<Q - 1>([&](auto J) {
+            constexpr int i = J + 1;
+            r = r + f[i];
+            // first nonzero term starts the sum exactly (0 + v == v)
+            if constexpr (kCX[i] != 0) { mx = mx0 ? (kCX[i] > 0 ? f[i] : -f[i]) : accum<kCX[i]>(mx, f[i]); mx0 = false; }
+            if constexpr (kCY[i] != 0) { my = my0 ? (kCY[i] > 0 ? f[i] : -f[i]) : accum<kCY[i]>(my, f[i]); my0 = false; }
+            if constexpr (kCZ[i] != 0) { mz = mz0 ? (kCZ[i] > 0 ? f[i] : -f[i]) : accum<kCZ[i]>(mz, f[i]); mz0 = false; }
+        });
+        const bool forced = (fl & FLAG_FORCED) != 0;
+        R fx = R(0), fy = R(0), fz = R(0);
+        R ux, uy, uz;
+        if (forced) {
+            fx = a.Fx[c] * a.fscale; fy = a.Fy[c] * a.fscale; fz = a.Fz[c] * a.fscale;
+            ux = (mx + R(0.5) * fx) / r; uy = (my + R(0.5) * fy) / r; uz = (mz + R(0.5) * fz) / r;
+        } else {
+            ux = mx / r; uy = my / r; uz = mz / r;
+        }
+        const R usq = ux * ux + uy * uy + uz * uz;
 
-    // non-equilibrium stress
-    R feq[Q];
-    R pxx = R(0), pyy = R(0), pzz = R(0), pxy = R(0), pxz = R(0), pyz = R(0);
-    bool pxx0 = true, pyy0 = true, pzz0 = true, pxy0 = true, pxz0 = true, pyz0 = true;
-    static_for<Q>([&](auto I) {
-        constexpr int i = I;
-        constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
-        const R cu = R(3) * dot3<cx, cy, cz>(ux, uy, uz);
-        feq[i] = weight<R>(i) * r * (R(1) + cu + R(0.5) * cu * cu - R(1.5) * usq);
-        const R fn = f[i] - feq[i];
-        if constexpr (cx != 0) { pxx = pxx0 ? fn : pxx + fn; pxx0 = false; }
-        if constexpr (cy != 0) { pyy = pyy0 ? fn : pyy + fn; pyy0 = false; }
-        if constexpr (cz != 0) { pzz = pzz0 ? fn : pzz + fn; pzz0 = false; }
-        if constexpr (cx * cy != 0) { pxy = pxy0 ? (cx * cy > 0 ? fn : -fn) : accum<cx * cy>(pxy, fn); pxy0 = false; }
-        if constexpr (cx * cz != 0) { pxz = pxz0 ? (cx * cz > 0 ? fn : -fn) : accum<cx * cz>(pxz, fn); pxz0 = false; }
-        if constexpr (cy * cz != 0) { pyz = pyz0 ? (cy * cz > 0 ? fn : -fn) : accum<cy * cz>(pyz, fn); pyz0 = false; }
-    });
-
-    // Smagorinsky relaxation time
-    const R qn = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
-    const R tau = R(0.5) * (a.tau0 + sqrt(a.tau0sq + a.ksmag * qn / r));
-    const R om = R(1) / tau;
-    const R om1 = R(1) - om;
-
-    // regularized post-collision populations
-    T* const out = fd + c;
-    if (forced) {
-        const R sf = R(1) - R(0.5) * om;
+        // non-equilibrium stress
+        R feq[Q];
+        R pxx = R(0), pyy = R(0), pzz = R(0), pxy = R(0), pxz = R(0), pyz = R(0);
+        bool pxx0 = true, pyy0 = true, pzz0 = true, pxy0 = true, pxz0 = true, pyz0 = true;
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
-            constexpr R third = R(1) / R(3);
-            constexpr R cxx = R(cx * cx) - third, cyy = R(cy * cy) - third, czz = R(cz * cz) - third;
-            R sp = cxx * pxx + cyy * pyy + czz * pzz;
-            if constexpr (cx * cy != 0) sp = sp + R(2) * (cx * cy > 0 ? pxy : -pxy);
-            else if constexpr (cx * cz != 0) sp = sp + R(2) * (cx * cz > 0 ? pxz : -pxz);
-            else if constexpr (cy * cz != 0) sp = sp + R(2) * (cy * cz > 0 ? pyz : -pyz);
-            constexpr R c45w = R(4.5) * weight<R>(i);
-            const R fneq = c45w * sp;
-            const R cu = dot3<cx, cy, cz>(ux, uy, uz);
-            const R cF = dot3<cx, cy, cz>(fx, fy, fz);
-            const R src = sf * weight<R>(i) *
-                          (R(3) * ((R(cx) - ux) * fx + (R(cy) - uy) * fy + (R(cz) - uz) * fz) + R(9) * cu * cF);
-            constexpr R cm15w = R(-1.5) * weight<R>(i);
-            const R fneq1 = cm15w * cF;
-            Codec::template st<i>(out + (long long)i * N, feq[i] + om1 * (fneq + fneq1) + src);
+            const R cu = R(3) * dot3<cx, cy, cz>(ux, uy, uz);
+            feq[i] = weight<R>(i) * r * (R(1) + cu + R(0.5) * cu * cu - R(1.5) * usq);
+            const R fn = f[i] - feq[i];
+            if constexpr (cx != 0) { pxx = pxx0 ? fn : pxx + fn; pxx0 = false; }
+            if constexpr (cy != 0) { pyy = pyy0 ? fn : pyy + fn; pyy0 = false; }
+            if constexpr (cz != 0) { pzz = pzz0 ? fn : pzz + fn; pzz0 = false; }
+            if constexpr (cx * cy != 0) { pxy = pxy0 ? (cx * cy > 0 ? fn : -fn) : accum<cx * cy>(pxy, fn); pxy0 = false; }
+            if constexpr (cx * cz != 0) { pxz = pxz0 ? (cx * cz > 0 ? fn : -fn) : accum<cx * cz>(pxz, fn); pxz0 = false; }
+            if constexpr (cy * cz != 0) { pyz = pyz0 ? (cy * cz > 0 ? fn : -fn) : accum<cy * cz>(pyz, fn); pyz0 = false; }
         });
-    } else {
+
+        // Smagorinsky relaxation time
+        const R qn = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
+        const R tau = R(0.5) * (a.tau0 + sqrt(a.tau0sq + a.ksmag * qn / r));
+        const R om = R(1) / tau;
+        const R om1 = R(1) - om;
+
+        // regularized post-collision populations (kept in f until the store)
+        if (forced) {
+            const R sf = R(1) - R(0.5) * om;
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+                constexpr R third = R(1) / R(3);
+                constexpr R cxx = R(cx * cx) - third, cyy = R(cy * cy) - third, czz = R(cz * cz) - third;
+                R sp = cxx * pxx + cyy * pyy + czz * pzz;
+                if constexpr (cx * cy != 0) sp = sp + R(2) * (cx * cy > 0 ? pxy : -pxy);
+                else if constexpr (cx * cz != 0) sp = sp + R(2) * (cx * cz > 0 ? pxz : -pxz);
+                else if constexpr (cy * cz != 0) sp = sp + R(2) * (cy * cz > 0 ? pyz : -pyz);
+                constexpr R c45w = R(4.5) * weight<R>(i);
+                const R fneq = c45w * sp;
+                const R cu = dot3<cx, cy, cz>(ux, uy, uz);
+                const R cF = dot3<cx, cy, cz>(fx, fy, fz);
+                const R src = sf * weight<R>(i) *
+                              (R(3) * ((R(cx) - ux) * fx + (R(cy) - uy) * fy + (R(cz) - uz) * fz) + R(9) * cu * cF);
+                constexpr R cm15w = R(-1.5) * weight<R>(i);
+                const R fneq1 = cm15w * cF;
+                f[i] = feq[i] + om1 * (fneq + fneq1) + src;
+            });
+        } else {
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+                constexpr R third = R(1) / R(3);
+                constexpr R cxx = R(cx * cx) - third, cyy = R(cy * cy) - third, czz = R(cz * cz) - third;
+                R sp = cxx * pxx + cyy * pyy + czz * pzz;
+                if constexpr (cx * cy != 0) sp = sp + R(2) * (cx * cy > 0 ? pxy : -pxy);
+                else if constexpr (cx * cz != 0) sp = sp + R(2) * (cx * cz > 0 ? pxz : -pxz);
+                else if constexpr (cy * cz != 0) sp = sp + R(2) * (cy * cz > 0 ? pyz : -pyz);
+                constexpr R c45w = R(4.5) * weight<R>(i);
+                f[i] = feq[i] + om1 * (c45w * sp);
+            });
+        }
+        if (a.write_all || (wm != 0u && (fl & FLAG_CF))) {
+            a.rho[c] = r; a.ux[c] = ux; a.uy[c] = uy; a.uz[c] = uz;
+        }
+    }
+
+    // Inlet and outlet rows, fused: the boundary lane takes its neighbour's
+    // post-collision populations (rounded through the storage format, as a
+    // separate pass would read them) by a cross-lane shift. The outlet is done
+    // here when x = nx-2 and nx-1 share a wavefront; otherwise by k_bc.
+    const int xw = tile * int(blockDim.x) + (int(threadIdx.x) & ~63);   // first x of this wavefront
+    const bool in_wave = xw == 0;
+    const bool out_wave = a.bc_out_fused && xw <= a.nx - 1 && a.nx - 1 < xw + 64;
+    if (in_wave || out_wave) {
+        R v[Q];
+        if (in_wave) {
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                v[i] = __shfl_down(Codec::template rt<i>(f[i]), 1);
+            });
+            const bool nb = __shfl_down(int(active), 1) != 0;
+            if (x == 0 && active && nb) boundary_row<R>(f, v, 0, a.uin[row], R(1));
+        }
+        if (out_wave) {
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                v[i] = __shfl_up(Codec::template rt<i>(f[i]), 1);
+            });
+            const bool nb = __shfl_up(int(active), 1) != 0;
+            if (x == a.nx - 1 && active && nb) boundary_row<R>(f, v, 1, R(0), R(1));
+        }
+    }
+
+    if (active) {
+        typename Codec::T* const out = fd + c;
         static_for<Q>([&](auto I) {
             constexpr int i = I;
-            constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
-            constexpr R third = R(1) / R(3);
-            constexpr R cxx = R(cx * cx) - third, cyy = R(cy * cy) - third, czz = R(cz * cz) - third;
-            R sp = cxx * pxx + cyy * pyy + czz * pzz;
-            if constexpr (cx * cy != 0) sp = sp + R(2) * (cx * cy > 0 ? pxy : -pxy);
-            else if constexpr (cx * cz != 0) sp = sp + R(2) * (cx * cz > 0 ? pxz : -pxz);
-            else if constexpr (cy * cz != 0) sp = sp + R(2) * (cy * cz > 0 ? pyz : -pyz);
-            constexpr R c45w = R(4.5) * weight<R>(i);
-            Codec::template st<i>(out + (long long)i * N, feq[i] + om1 * (c45w * sp));
+            Codec::template st<i>(out + (long long)i * N, f[i]);
         });
-    }
-    if (a.write_all || (wm != 0u && (fl & FLAG_CF))) {
-        a.rho[c] = r; a.ux[c] = ux; a.uy[c] = uy; a.uz[c] = uz;
     }
 }
 
@@ -371,12 +479,13 @@ __device__ __forceinline__ typename Codec::R feq_generic(int i, typename Codec::
 template <class Codec>
 __global__ void __launch_bounds__(256) k_bc(typename Codec::T* __restrict__ fd, const uint8_t* __restrict__ flag,
                                             const typename Codec::R* __restrict__ uin, int nx, int px, int rows,
-                                            long long N, typename Codec::R rho_out) {
+                                            long long N, typename Codec::R rho_out, int sides) {
     using R = typename Codec::R;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= rows) return;
     const long long row = (long long)px * j;  // j = y + ny*z
     for (int side = 0; side < 2; ++side) {
+        if (!((sides >> side) & 1)) continue;
         const long long c0 = side == 0 ? row : row + nx - 1;
         const long long c1 = side == 0 ? row + 1 : row + nx - 2;
         if ((flag[c0] | flag[c1]) & FLAG_SOLID) continue;

@@ -35,10 +35,15 @@ int fail(int code, const std::string& msg) {
 
 inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// Streaming copy, four 16-byte elements per work-item (a 4 KB run per wave per pass).
 __global__ void __launch_bounds__(256) k_copy16(const float4* __restrict__ src, float4* __restrict__ dst,
                                                 long long n) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) dst[k] = src[k];
+    const long long base = ((long long)blockIdx.x * blockDim.x) * 4 + threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const long long k = base + (long long)u * blockDim.x;
+        if (k < n) dst[k] = src[k];
+    }
 }
 
 __global__ void k_fill_zero_u32(uint32_t* p, long long n) {
@@ -327,6 +332,8 @@ struct Solver final : wfs_solver {
         a.rows = rows();
         a.tiles = (d.nx + bx - 1) / bx;
         a.N = Np;
+        a.uin = rp(uin);
+        a.bc_out_fused = (d.nx - 1) % 64 != 0 ? 1 : 0;
         const long long groups = (long long)((a.rows + 7) / 8) * 8 * a.tiles;
         const dim3 grid((unsigned)groups);
         for (int k = 0; k < n; ++k) {
@@ -342,9 +349,11 @@ struct Solver final : wfs_solver {
                 int rc = mark();
                 if (rc) return rc;
             }
-            hipLaunchKernelGGL(k_bc<Codec>, dim3(blocks_for(a.rows, 64)), dim3(64), 0, stream, fp(cur ^ 1), flags(),
-                               rp(uin), d.nx, px, a.rows, Np, R(1));
-            WFS_LAUNCH_CHECK();
+            if (!a.bc_out_fused) {   // x = nx-1 opens a wavefront: outlet row in its own pass
+                hipLaunchKernelGGL(k_bc<Codec>, dim3(blocks_for(a.rows, 64)), dim3(64), 0, stream, fp(cur ^ 1),
+                                   flags(), rp(uin), d.nx, px, a.rows, Np, R(1), 2);
+                WFS_LAUNCH_CHECK();
+            }
             cur ^= 1;
         }
         nstep += n;
@@ -811,9 +820,7 @@ int wfs_bandwidth_probe(int32_t device, int64_t bytes, int32_t iters, double* gb
         hipStream_t s;
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } sg{st};
-    int ncu = 256;
-    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
-    const unsigned grid = (unsigned)std::min<long long>((n + 255) / 256, (long long)ncu * 16);
+    const unsigned grid = (unsigned)((n + 1023) / 1024);
     hipLaunchKernelGGL(k_fill_zero_u32, dim3(blocks_for(n * 4, 256)), dim3(256), 0, st, static_cast<uint32_t*>(a.p),
                        n * 4);
     WFS_LAUNCH_CHECK();
