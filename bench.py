@@ -32,7 +32,8 @@ if ROOT not in sys.path:
 METRIC = "MLUPS & % HBM roofline, plain FP32 vs FP16-store/batched/learned SGS"
 PEAK_HBM_GBS = 8000.0                       # MI355X HBM3E
 # algorithmic bytes per fluid-cell update: 19 populations read + 19 written + 1 flag byte
-B_ALG = {"f32": 19 * 4 * 2 + 1, "f16dev": 19 * 2 * 2 + 1, "bf16dev": 19 * 2 * 2 + 1, "f64": 19 * 8 * 2 + 1}
+B_ALG = {"f32": 19 * 4 * 2 + 1, "f32fast": 19 * 4 * 2 + 1, "f16dev": 19 * 2 * 2 + 1, "bf16dev": 19 * 2 * 2 + 1,
+         "f64": 19 * 8 * 2 + 1}
 PUBLISHED_MLUPS = 900.0                     # reference CUDA path, RTX 5070 Laptop (top of 550-900)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_f32_trapezoid_tug_traffic.txt")
 
@@ -117,7 +118,7 @@ def main():
     ap.add_argument("--workload", default="trapezoid_tug")
     ap.add_argument("--storage", default="f32", choices=sorted(B_ALG))
     ap.add_argument("--block-x", type=int, default=0)
-    ap.add_argument("--variants", default="f16dev,bf16dev", help="extra storage formats timed (comma list)")
+    ap.add_argument("--variants", default="f32fast,f16dev,bf16dev", help="extra storage formats timed (comma list)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

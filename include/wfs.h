@@ -39,10 +39,14 @@ enum wfs_status {
     WFS_ENONFINITE = -5 /* watchdog: non-finite state found */
 };
 
-/* Population storage. F32 is the reference's layout and precision; the two
- * 16-bit formats store (f_i - w_i) * 2^8 and compute in FP32; F64 stores and
- * computes in double (the FP64 run of the same path). */
-enum wfs_storage { WFS_STORE_F32 = 0, WFS_STORE_F16DEV = 1, WFS_STORE_BF16DEV = 2, WFS_STORE_F64 = 3 };
+/* Population storage. F32 is the reference's layout and precision, with the
+ * reference's operation order (bit-exact against an IEEE FP32 evaluation);
+ * the two 16-bit formats store (f_i - w_i) * 2^8 and compute in FP32; F64
+ * stores and computes in double (the FP64 run of the same path); F32_FAST
+ * stores FP32 and uses the same opposite-pair collision as the 16-bit
+ * formats (agreement within rounding, not bit for bit). */
+enum wfs_storage { WFS_STORE_F32 = 0, WFS_STORE_F16DEV = 1, WFS_STORE_BF16DEV = 2, WFS_STORE_F64 = 3,
+                   WFS_STORE_F32_FAST = 4 };
 
 typedef struct wfs_desc {
     int32_t nx, ny, nz;   /* lattice size; nx >= 3, nx*ny*nz < 2^31 */

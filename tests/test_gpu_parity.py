@@ -197,3 +197,58 @@ def test_type_errors_like_reference():
     with pytest.raises(TypeError):
         g.step(1)
     g.close()
+
+
+@pytest.mark.parametrize("case", ["a", "b"])
+def test_pair_form_collision_matches_fp64_oracle_on_fixture(case):
+    """The opposite-pair collision (f32fast storage) against the FP64 oracle
+    on the reference fixtures: one step, FP32 rounding level."""
+    from tests.helpers import oracle_raw_steps
+    inp, ref = step_fixture(case)
+    orc = oracle_raw_steps(inp, 1, "f64")
+    L = _lbm_from_fixture(inp, storage="f32fast")
+    fl = inp["solid"] == 0
+    m19 = np.tile(fl, 19)
+    L.step(1)
+    f1 = L.populations()
+    rho, ux, uy, uz = L.macro()
+    assert np.abs(f1[m19] - orc["fd_step1"][m19]).max() < 1e-6
+    assert np.abs(ux[fl] - orc["ux1"][fl]).max() < 1e-6
+    assert np.abs(f1[m19] - ref["fd_step1"][m19]).max() < 1e-6
+    L.close()
+
+
+def _fp64_reference_c1(nsteps):
+    reach = channel128()
+    gpu, orc = _run_pair(reach, nsteps, storage="f64", precision="f64")
+    gpu.close()
+    return reach, orc
+
+
+@pytest.mark.parametrize("storage,vel_tol,eta_tol", [
+    ("f32fast", 1e-4, 1e-3),
+    ("f16dev", 2e-2, 1e-3),
+    ("bf16dev", 2e-1, 5e-3),
+])
+def test_storage_formats_against_fp64_after_1000_steps(storage, vel_tol, eta_tol):
+    """Velocity relative L2 and level-profile error of each storage format
+    against the FP64 oracle on the 128x32x16 channel after 1000 steps."""
+    reach, ref = _fp64_reference_c1(1000)
+    args = solver_inputs(reach)
+    g = LBM3D(None, args["solid"], args["Fx"], args["Fy"], args["Fz"], args["uin"], args["tau0"], args["cs"],
+              args["cf_wall"], storage=storage)
+    g.set_bedwall(reach.bedwall)
+    for _ in range(50):
+        g.update_uref()
+        g.step(20)
+    rho, ux, uy, uz = g.macro()
+    fl = ~reach.solid.ravel()
+    vel = np.concatenate([ux[fl], uy[fl], uz[fl]])
+    vref = np.concatenate([ref.ux[fl], ref.uy[fl], ref.uz[fl]])
+    err_v = rel_l2(vel, vref)
+    err_eta = np.abs(_level_profile(reach, rho) - _level_profile(reach, ref.rho)).max()
+    print(f"{storage}: velocity rel L2 {err_v:.3e}, level max error {err_eta * 1e3:.4f} mm")
+    assert np.isfinite(vel).all()
+    assert err_v <= vel_tol
+    assert err_eta <= eta_tol
+    g.close()

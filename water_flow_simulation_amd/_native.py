@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("WFS_LIB") or os.path.join(PKG, "libwfs.so")
 HEADER = os.path.join(ROOT, "include", "wfs.h")
 
 WFS_OK, WFS_EINVAL, WFS_ENOMEM, WFS_EHIP, WFS_ENODEV, WFS_ENONFINITE = 0, -1, -2, -3, -4, -5
-STORAGE = {"f32": 0, "f16dev": 1, "bf16dev": 2, "f64": 3}
+STORAGE = {"f32": 0, "f16dev": 1, "bf16dev": 2, "f64": 3, "f32fast": 4}
 FIELD = {"rho": 0, "ux": 1, "uy": 2, "uz": 3, "sum_ux": 4, "sum_uy": 5, "sum_uz": 6, "sum_rho": 7,
          "sum_sp": 8, "sum_uxx": 9, "one": 10}
 

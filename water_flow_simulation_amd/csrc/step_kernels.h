@@ -62,6 +62,7 @@ __device__ __forceinline__ void store_pop(T* p, T v) {
 struct CodecF32 {
     using T = float;
     using R = float;
+    static constexpr bool kExact = true;   // operand order of the reference update, bit-exact
     template <int I>
     __device__ static R ld(const T* p) { return *p; }
     template <int I>
@@ -70,9 +71,15 @@ struct CodecF32 {
     __device__ static R rt(R v) { return v; }   // value after a store/load round trip
 };
 
+// FP32 storage with the pair-form collision (not bit-exact; judged by tolerance).
+struct CodecF32Fast : CodecF32 {
+    static constexpr bool kExact = false;
+};
+
 struct CodecF64 {
     using T = double;
     using R = double;
+    static constexpr bool kExact = true;
     template <int I>
     __device__ static R ld(const T* p) { return *p; }
     template <int I>
@@ -89,40 +96,43 @@ constexpr int kDevShift = 8;
 struct CodecF16Dev {
     using T = __half;
     using R = float;
+    static constexpr bool kExact = false;
     template <int I>
     __device__ static R ld(const T* p) {
-        return weight<float>(I) + __half2float(*p) * (1.0f / float(1 << kDevShift));
+        return fmaf(__half2float(*p), 1.0f / float(1 << kDevShift), weight<float>(I));
     }
     template <int I>
     __device__ static void st(T* p, R v) {
-        const __half h = __float2half_rn((v - weight<float>(I)) * float(1 << kDevShift));
+        const __half h = __float2half_rn(fmaf(v, float(1 << kDevShift), -weight<float>(I) * float(1 << kDevShift)));
         store_pop(reinterpret_cast<unsigned short*>(p), __half_as_ushort(h));
     }
     template <int I>
     __device__ static R rt(R v) {
-        return weight<float>(I) +
-               __half2float(__float2half_rn((v - weight<float>(I)) * float(1 << kDevShift))) *
-                   (1.0f / float(1 << kDevShift));
+        return fmaf(__half2float(__float2half_rn(fmaf(v, float(1 << kDevShift),
+                                                         -weight<float>(I) * float(1 << kDevShift)))),
+                    1.0f / float(1 << kDevShift), weight<float>(I));
     }
 };
 
 struct CodecBF16Dev {
     using T = __hip_bfloat16;
     using R = float;
+    static constexpr bool kExact = false;
     template <int I>
     __device__ static R ld(const T* p) {
-        return weight<float>(I) + __bfloat162float(*p) * (1.0f / float(1 << kDevShift));
+        return fmaf(__bfloat162float(*p), 1.0f / float(1 << kDevShift), weight<float>(I));
     }
     template <int I>
     __device__ static void st(T* p, R v) {
-        const __hip_bfloat16 h = __float2bfloat16((v - weight<float>(I)) * float(1 << kDevShift));
+        const __hip_bfloat16 h =
+            __float2bfloat16(fmaf(v, float(1 << kDevShift), -weight<float>(I) * float(1 << kDevShift)));
         store_pop(reinterpret_cast<unsigned short*>(p), __bfloat16_as_ushort(h));
     }
     template <int I>
     __device__ static R rt(R v) {
-        return weight<float>(I) +
-               __bfloat162float(__float2bfloat16((v - weight<float>(I)) * float(1 << kDevShift))) *
-                   (1.0f / float(1 << kDevShift));
+        return fmaf(__bfloat162float(__float2bfloat16(fmaf(v, float(1 << kDevShift),
+                                                           -weight<float>(I) * float(1 << kDevShift)))),
+                    1.0f / float(1 << kDevShift), weight<float>(I));
     }
 };
 
@@ -229,6 +239,93 @@ __device__ __forceinline__ void boundary_row(R* f, const R* v, int side, R uin, 
         const R e = side == 0 ? feq_full<R>(i, r, uin, R(0), R(0)) : feq_full<R>(i, rho_out, ux, uy, uz);
         f[i] = e + (v[i] - feq_full<R>(i, r, ux, uy, uz));
     }
+}
+
+
+// Regularized BGK + Smagorinsky + Guo forcing in pair form, for the storage
+// formats that are judged against an FP64 run rather than bit for bit.
+// Opposite directions i / opp(i) share the even part of their update and
+// differ in the sign of the odd part:
+//   feq_i     = w rho (1 - 1.5 u^2 + 4.5 (c.u)^2)  +  3 w rho (c.u)
+//   fneq_i    = 4.5 w (Q_i : Pi_neq),  Q_i = c c - I/3      (even)
+//   source    = w sf (9 (c.u)(c.F) - 3 u.F)  +  w (c.F)(3 sf - 1.5 (1 - omega))
+// and Pi_neq comes from the second moment of f minus rho (u u + I/3), the
+// second moment of the equilibrium, so no per-direction equilibrium is formed
+// for the stress. Same mathematics as the per-direction form; different
+// rounding (FMAs, one reciprocal).
+template <typename R>
+__device__ __forceinline__ void collide_pairs(R* f, bool forced, R fx, R fy, R fz, R tau0, R tau0sq, R ksmag,
+                                              R& r, R& ux, R& uy, R& uz) {
+    // pair sums and differences (i, i+1) for i = 1, 3, ..., 17
+    R sp[9], dp[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        sp[k] = f[2 * k + 1] + f[2 * k + 2];
+        dp[k] = f[2 * k + 1] - f[2 * k + 2];
+    }
+    // pairs: 0:x 1:y 2:z 3:(1,1,0) 4:(1,-1,0) 5:(1,0,1) 6:(1,0,-1) 7:(0,1,1) 8:(0,1,-1)
+    r = f[0] + ((sp[0] + sp[1] + sp[2]) + (sp[3] + sp[4] + sp[5]) + (sp[6] + sp[7] + sp[8]));
+    const R mx = dp[0] + dp[3] + dp[4] + dp[5] + dp[6];
+    const R my = dp[1] + dp[3] - dp[4] + dp[7] + dp[8];
+    const R mz = dp[2] + dp[5] - dp[6] + dp[7] - dp[8];
+    const R ri = R(1) / r;
+    if (forced) {
+        ux = fma(R(0.5), fx, mx) * ri; uy = fma(R(0.5), fy, my) * ri; uz = fma(R(0.5), fz, mz) * ri;
+    } else {
+        ux = mx * ri; uy = my * ri; uz = mz * ri;
+    }
+    const R rux = r * ux, ruy = r * uy, ruz = r * uz, r3 = r * (R(1) / R(3));
+    const R pxx = (sp[0] + sp[3] + sp[4] + sp[5] + sp[6]) - fma(rux, ux, r3);
+    const R pyy = (sp[1] + sp[3] + sp[4] + sp[7] + sp[8]) - fma(ruy, uy, r3);
+    const R pzz = (sp[2] + sp[5] + sp[6] + sp[7] + sp[8]) - fma(ruz, uz, r3);
+    const R pxy = (sp[3] - sp[4]) - rux * uy;
+    const R pxz = (sp[5] - sp[6]) - rux * uz;
+    const R pyz = (sp[7] - sp[8]) - ruy * uz;
+    const R qn = sqrt(fma(pxx, pxx, fma(pyy, pyy, pzz * pzz)) + R(2) * fma(pxy, pxy, fma(pxz, pxz, pyz * pyz)));
+    const R tau = R(0.5) * (tau0 + sqrt(fma(ksmag * qn, ri, tau0sq)));
+    const R om = R(1) / tau;
+    const R om1 = R(1) - om;
+    const R t3 = (pxx + pyy + pzz) * (R(1) / R(3));
+    const R usq = fma(ux, ux, fma(uy, uy, uz * uz));
+    const R k0 = fma(R(-1.5), usq, R(1));
+    const R wr0 = r * (R(1) / R(3)), wr1 = r * (R(1) / R(18)), wr2 = r * (R(1) / R(36));
+    const R g1 = om1 * R(0.25), g2 = om1 * R(0.125);   // om1 * 4.5 w for w = 1/18, 1/36
+    R sf = R(0), uF = R(0), h1 = R(0), h2 = R(0);
+    if (forced) {
+        sf = R(1) - R(0.5) * om;
+        uF = fma(ux, fx, fma(uy, fy, uz * fz));
+        const R odd = fma(R(3), sf, R(-1.5) * om1);   // (3 sf - 1.5 om1)
+        h1 = odd * (R(1) / R(18));
+        h2 = odd * (R(1) / R(36));
+    }
+    // rest population
+    {
+        R e = fma(om1 * R(1.5), -t3, wr0 * k0);
+        if (forced) e = fma(sf * (R(1) / R(3)), R(-3) * uF, e);
+        f[0] = e;
+    }
+    auto pair = [&](int k, R cu, R nq, R cF, R wr, R g, R w, R h) {
+        const R cu2 = cu * cu;
+        R e = fma(g, nq, wr * fma(R(4.5), cu2, k0));
+        R o = R(3) * wr * cu;
+        if (forced) {
+            e = fma(w * sf, fma(R(9) * cu, cF, R(-3) * uF), e);
+            o = fma(h, cF, o);
+        }
+        f[2 * k + 1] = e + o;
+        f[2 * k + 2] = e - o;
+    };
+    const R dxx = pxx - t3, dyy = pyy - t3, dzz = pzz - t3;
+    pair(0, ux, dxx, fx, wr1, g1, R(1) / R(18), h1);
+    pair(1, uy, dyy, fy, wr1, g1, R(1) / R(18), h1);
+    pair(2, uz, dzz, fz, wr1, g1, R(1) / R(18), h1);
+    const R sxy = pxx + pyy - t3, sxz = pxx + pzz - t3, syz = pyy + pzz - t3;
+    pair(3, ux + uy, fma(R(2), pxy, sxy), fx + fy, wr2, g2, R(1) / R(36), h2);
+    pair(4, ux - uy, fma(R(-2), pxy, sxy), fx - fy, wr2, g2, R(1) / R(36), h2);
+    pair(5, ux + uz, fma(R(2), pxz, sxz), fx + fz, wr2, g2, R(1) / R(36), h2);
+    pair(6, ux - uz, fma(R(-2), pxz, sxz), fx - fz, wr2, g2, R(1) / R(36), h2);
+    pair(7, uy + uz, fma(R(2), pyz, syz), fy + fz, wr2, g2, R(1) / R(36), h2);
+    pair(8, uy - uz, fma(R(-2), pyz, syz), fy - fz, wr2, g2, R(1) / R(36), h2);
 }
 
 #ifndef WFS_STEP_WAVES_PER_EU
@@ -340,87 +437,95 @@ k_step(const typename Codec::T* __restrict__ fs,
             }
         }
 
-        // 3. moments (Guo half-force shift)
-        R r = f[0], mx = R(0), my = R(0), mz = R(0);
-        bool mx0 = true, my0 = true, mz0 = true;
-        static_for<Q - 1>([&](auto J) {
-            constexpr int i = J + 1;
-            r = r + f[i];
-            // first nonzero term starts the sum exactly (0 + v == v)
-            if constexpr (kCX[i] != 0) { mx = mx0 ? (kCX[i] > 0 ? f[i] : -f[i]) : accum<kCX[i]>(mx, f[i]); mx0 = false; }
-            if constexpr (kCY[i] != 0) { my = my0 ? (kCY[i] > 0 ? f[i] : -f[i]) : accum<kCY[i]>(my, f[i]); my0 = false; }
-            if constexpr (kCZ[i] != 0) { mz = mz0 ? (kCZ[i] > 0 ? f[i] : -f[i]) : accum<kCZ[i]>(mz, f[i]); mz0 = false; }
-        });
-        const bool forced = (fl & FLAG_FORCED) != 0;
-        R fx = R(0), fy = R(0), fz = R(0);
-        R ux, uy, uz;
-        if (forced) {
-            fx = a.Fx[c] * a.fscale; fy = a.Fy[c] * a.fscale; fz = a.Fz[c] * a.fscale;
-            ux = (mx + R(0.5) * fx) / r; uy = (my + R(0.5) * fy) / r; uz = (mz + R(0.5) * fz) / r;
-        } else {
-            ux = mx / r; uy = my / r; uz = mz / r;
-        }
-        const R usq = ux * ux + uy * uy + uz * uz;
+        R r, ux, uy, uz;
+        if constexpr (Codec::kExact) {
+            // 3. moments (Guo half-force shift)
+            r = f[0];
+            R mx = R(0), my = R(0), mz = R(0);
+            bool mx0 = true, my0 = true, mz0 = true;
+            static_for<Q - 1>([&](auto J) {
+                constexpr int i = J + 1;
+                r = r + f[i];
+                // first nonzero term starts the sum exactly (0 + v == v)
+                if constexpr (kCX[i] != 0) { mx = mx0 ? (kCX[i] > 0 ? f[i] : -f[i]) : accum<kCX[i]>(mx, f[i]); mx0 = false; }
+                if constexpr (kCY[i] != 0) { my = my0 ? (kCY[i] > 0 ? f[i] : -f[i]) : accum<kCY[i]>(my, f[i]); my0 = false; }
+                if constexpr (kCZ[i] != 0) { mz = mz0 ? (kCZ[i] > 0 ? f[i] : -f[i]) : accum<kCZ[i]>(mz, f[i]); mz0 = false; }
+            });
+            const bool forced = (fl & FLAG_FORCED) != 0;
+            R fx = R(0), fy = R(0), fz = R(0);
+            if (forced) {
+                fx = a.Fx[c] * a.fscale; fy = a.Fy[c] * a.fscale; fz = a.Fz[c] * a.fscale;
+                ux = (mx + R(0.5) * fx) / r; uy = (my + R(0.5) * fy) / r; uz = (mz + R(0.5) * fz) / r;
+            } else {
+                ux = mx / r; uy = my / r; uz = mz / r;
+            }
+            const R usq = ux * ux + uy * uy + uz * uz;
 
-        // non-equilibrium stress
-        R feq[Q];
-        R pxx = R(0), pyy = R(0), pzz = R(0), pxy = R(0), pxz = R(0), pyz = R(0);
-        bool pxx0 = true, pyy0 = true, pzz0 = true, pxy0 = true, pxz0 = true, pyz0 = true;
-        static_for<Q>([&](auto I) {
-            constexpr int i = I;
-            constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
-            const R cu = R(3) * dot3<cx, cy, cz>(ux, uy, uz);
-            feq[i] = weight<R>(i) * r * (R(1) + cu + R(0.5) * cu * cu - R(1.5) * usq);
-            const R fn = f[i] - feq[i];
-            if constexpr (cx != 0) { pxx = pxx0 ? fn : pxx + fn; pxx0 = false; }
-            if constexpr (cy != 0) { pyy = pyy0 ? fn : pyy + fn; pyy0 = false; }
-            if constexpr (cz != 0) { pzz = pzz0 ? fn : pzz + fn; pzz0 = false; }
-            if constexpr (cx * cy != 0) { pxy = pxy0 ? (cx * cy > 0 ? fn : -fn) : accum<cx * cy>(pxy, fn); pxy0 = false; }
-            if constexpr (cx * cz != 0) { pxz = pxz0 ? (cx * cz > 0 ? fn : -fn) : accum<cx * cz>(pxz, fn); pxz0 = false; }
-            if constexpr (cy * cz != 0) { pyz = pyz0 ? (cy * cz > 0 ? fn : -fn) : accum<cy * cz>(pyz, fn); pyz0 = false; }
-        });
-
-        // Smagorinsky relaxation time
-        const R qn = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
-        const R tau = R(0.5) * (a.tau0 + sqrt(a.tau0sq + a.ksmag * qn / r));
-        const R om = R(1) / tau;
-        const R om1 = R(1) - om;
-
-        // regularized post-collision populations (kept in f until the store)
-        if (forced) {
-            const R sf = R(1) - R(0.5) * om;
+            // non-equilibrium stress
+            R feq[Q];
+            R pxx = R(0), pyy = R(0), pzz = R(0), pxy = R(0), pxz = R(0), pyz = R(0);
+            bool pxx0 = true, pyy0 = true, pzz0 = true, pxy0 = true, pxz0 = true, pyz0 = true;
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
                 constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
-                constexpr R third = R(1) / R(3);
-                constexpr R cxx = R(cx * cx) - third, cyy = R(cy * cy) - third, czz = R(cz * cz) - third;
-                R sp = cxx * pxx + cyy * pyy + czz * pzz;
-                if constexpr (cx * cy != 0) sp = sp + R(2) * (cx * cy > 0 ? pxy : -pxy);
-                else if constexpr (cx * cz != 0) sp = sp + R(2) * (cx * cz > 0 ? pxz : -pxz);
-                else if constexpr (cy * cz != 0) sp = sp + R(2) * (cy * cz > 0 ? pyz : -pyz);
-                constexpr R c45w = R(4.5) * weight<R>(i);
-                const R fneq = c45w * sp;
-                const R cu = dot3<cx, cy, cz>(ux, uy, uz);
-                const R cF = dot3<cx, cy, cz>(fx, fy, fz);
-                const R src = sf * weight<R>(i) *
-                              (R(3) * ((R(cx) - ux) * fx + (R(cy) - uy) * fy + (R(cz) - uz) * fz) + R(9) * cu * cF);
-                constexpr R cm15w = R(-1.5) * weight<R>(i);
-                const R fneq1 = cm15w * cF;
-                f[i] = feq[i] + om1 * (fneq + fneq1) + src;
+                const R cu = R(3) * dot3<cx, cy, cz>(ux, uy, uz);
+                feq[i] = weight<R>(i) * r * (R(1) + cu + R(0.5) * cu * cu - R(1.5) * usq);
+                const R fn = f[i] - feq[i];
+                if constexpr (cx != 0) { pxx = pxx0 ? fn : pxx + fn; pxx0 = false; }
+                if constexpr (cy != 0) { pyy = pyy0 ? fn : pyy + fn; pyy0 = false; }
+                if constexpr (cz != 0) { pzz = pzz0 ? fn : pzz + fn; pzz0 = false; }
+                if constexpr (cx * cy != 0) { pxy = pxy0 ? (cx * cy > 0 ? fn : -fn) : accum<cx * cy>(pxy, fn); pxy0 = false; }
+                if constexpr (cx * cz != 0) { pxz = pxz0 ? (cx * cz > 0 ? fn : -fn) : accum<cx * cz>(pxz, fn); pxz0 = false; }
+                if constexpr (cy * cz != 0) { pyz = pyz0 ? (cy * cz > 0 ? fn : -fn) : accum<cy * cz>(pyz, fn); pyz0 = false; }
             });
+
+            // Smagorinsky relaxation time
+            const R qn = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
+            const R tau = R(0.5) * (a.tau0 + sqrt(a.tau0sq + a.ksmag * qn / r));
+            const R om = R(1) / tau;
+            const R om1 = R(1) - om;
+
+            // regularized post-collision populations (kept in f until the store)
+            if (forced) {
+                const R sf = R(1) - R(0.5) * om;
+                static_for<Q>([&](auto I) {
+                    constexpr int i = I;
+                    constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+                    constexpr R third = R(1) / R(3);
+                    constexpr R cxx = R(cx * cx) - third, cyy = R(cy * cy) - third, czz = R(cz * cz) - third;
+                    R sp = cxx * pxx + cyy * pyy + czz * pzz;
+                    if constexpr (cx * cy != 0) sp = sp + R(2) * (cx * cy > 0 ? pxy : -pxy);
+                    else if constexpr (cx * cz != 0) sp = sp + R(2) * (cx * cz > 0 ? pxz : -pxz);
+                    else if constexpr (cy * cz != 0) sp = sp + R(2) * (cy * cz > 0 ? pyz : -pyz);
+                    constexpr R c45w = R(4.5) * weight<R>(i);
+                    const R fneq = c45w * sp;
+                    const R cu = dot3<cx, cy, cz>(ux, uy, uz);
+                    const R cF = dot3<cx, cy, cz>(fx, fy, fz);
+                    const R src = sf * weight<R>(i) *
+                                  (R(3) * ((R(cx) - ux) * fx + (R(cy) - uy) * fy + (R(cz) - uz) * fz) + R(9) * cu * cF);
+                    constexpr R cm15w = R(-1.5) * weight<R>(i);
+                    const R fneq1 = cm15w * cF;
+                    f[i] = feq[i] + om1 * (fneq + fneq1) + src;
+                });
+            } else {
+                static_for<Q>([&](auto I) {
+                    constexpr int i = I;
+                    constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+                    constexpr R third = R(1) / R(3);
+                    constexpr R cxx = R(cx * cx) - third, cyy = R(cy * cy) - third, czz = R(cz * cz) - third;
+                    R sp = cxx * pxx + cyy * pyy + czz * pzz;
+                    if constexpr (cx * cy != 0) sp = sp + R(2) * (cx * cy > 0 ? pxy : -pxy);
+                    else if constexpr (cx * cz != 0) sp = sp + R(2) * (cx * cz > 0 ? pxz : -pxz);
+                    else if constexpr (cy * cz != 0) sp = sp + R(2) * (cy * cz > 0 ? pyz : -pyz);
+                    constexpr R c45w = R(4.5) * weight<R>(i);
+                    f[i] = feq[i] + om1 * (c45w * sp);
+                });
+            }
         } else {
-            static_for<Q>([&](auto I) {
-                constexpr int i = I;
-                constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
-                constexpr R third = R(1) / R(3);
-                constexpr R cxx = R(cx * cx) - third, cyy = R(cy * cy) - third, czz = R(cz * cz) - third;
-                R sp = cxx * pxx + cyy * pyy + czz * pzz;
-                if constexpr (cx * cy != 0) sp = sp + R(2) * (cx * cy > 0 ? pxy : -pxy);
-                else if constexpr (cx * cz != 0) sp = sp + R(2) * (cx * cz > 0 ? pxz : -pxz);
-                else if constexpr (cy * cz != 0) sp = sp + R(2) * (cy * cz > 0 ? pyz : -pyz);
-                constexpr R c45w = R(4.5) * weight<R>(i);
-                f[i] = feq[i] + om1 * (c45w * sp);
-            });
+            const bool forced = (fl & FLAG_FORCED) != 0;
+            R fx = R(0), fy = R(0), fz = R(0);
+            if (forced) { fx = a.Fx[c] * a.fscale; fy = a.Fy[c] * a.fscale; fz = a.Fz[c] * a.fscale; }
+            collide_pairs<R>(f, forced, fx, fy, fz, a.tau0, a.tau0sq, a.ksmag, r, ux, uy, uz);
         }
         if (a.write_all || (wm != 0u && (fl & FLAG_CF))) {
             a.rho[c] = r; a.ux[c] = ux; a.uy[c] = uy; a.uz[c] = uz;

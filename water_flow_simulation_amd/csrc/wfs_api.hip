@@ -512,6 +512,7 @@ wfs_solver* make_solver(int storage) {
         case WFS_STORE_F16DEV: return new Solver<wfs::CodecF16Dev>();
         case WFS_STORE_BF16DEV: return new Solver<wfs::CodecBF16Dev>();
         case WFS_STORE_F64: return new Solver<wfs::CodecF64>();
+        case WFS_STORE_F32_FAST: return new Solver<wfs::CodecF32Fast>();
         default: return nullptr;
     }
 }
@@ -588,6 +589,7 @@ int wfs_create(const wfs_desc* desc, wfs_solver** out) {
     if (auto* s = dynamic_cast<Solver<wfs::CodecF16Dev>*>(h.get())) s->bx = bx;
     if (auto* s = dynamic_cast<Solver<wfs::CodecBF16Dev>*>(h.get())) s->bx = bx;
     if (auto* s = dynamic_cast<Solver<wfs::CodecF64>*>(h.get())) s->bx = bx;
+    if (auto* s = dynamic_cast<Solver<wfs::CodecF32Fast>*>(h.get())) s->bx = bx;
     *out = h.release();
     return WFS_OK;
 }
