@@ -137,6 +137,9 @@ int wfs_reset_sums(wfs_solver* h);
 int wfs_get_sums(wfs_solver* h, float* ux, float* uy, float* uz, float* rho, float* sp, float* uxx,
                  int32_t* navg);
 
+int wfs_get_sums_f64(wfs_solver* h, double* ux, double* uy, double* uz, double* rho, double* sp, double* uxx,
+                     int32_t* navg);
+
 /* FP64 per-x reductions over fluid cells with z0 <= z < z1:
  * out[x] = sum_{y, z} field(x, y, z) for field in enum wfs_field. */
 enum wfs_field {

@@ -91,7 +91,7 @@ class OracleLBM:
         self.N = solid.size
         self.tau0 = float(tau0)
         self.csmag2 = float(cs * cs)
-        A = lambda a: np.ascontiguousarray(np.asarray(a).ravel(), dtype=self.dt)  # noqa: E731
+        A = lambda a: np.array(np.asarray(a).ravel(), dtype=self.dt, copy=True)  # owns its data  # noqa: E731
         self.solid = np.ascontiguousarray(solid.ravel(), dtype=np.uint8)
         self.Fx, self.Fy, self.Fz = A(Fx), A(Fy), A(Fz)
         self.uin = A(uin)
@@ -165,6 +165,8 @@ class OracleRun:
     t_total, t_avg, t_ramp.
     """
 
+    cadence = 20   # steps between ramp/uref updates and averaging samples (reach3d.py:427)
+
     def __init__(self, inputs, precision="f32"):
         self.i = inputs
         self.lbm = OracleLBM(inputs["solid"], inputs["Fx"], inputs["Fy"], inputs["Fz"], inputs["uin_lb"],
@@ -187,7 +189,7 @@ class OracleRun:
         L = self.lbm
         k = 0
         while k < nsteps:
-            m = min(20, nsteps - k)
+            m = min(self.cadence, nsteps - k)
             if not self.ramp_done:
                 f = min(self.t / self.t_ramp, 1.0)
                 f = np.float32(f * f * (3 - 2 * f))

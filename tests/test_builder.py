@@ -162,9 +162,9 @@ def test_run3d_time_scalars_match_reference():
     g = golden("run_toy_boat.npz")
     from water_flow_simulation_amd.reach3d import ReachConfig
     sec = Section(B=2.0, Hb=1.5, z=1.0, Bf=0.0, n=0.03)
-    cfg = ReachConfig(length=4.0, dx=0.25, run_flows=0.02, avg_frac=0.99)
-    tb = replace(R.boat_type("custom"), hull_L=1.0, hull_B=0.6, draft=0.3, n_props=1, prop_D=0.1,
-                 prop_depth=0.45, tilt_deg=10.0, prop_x=0.2, power_kW=0.05)
+    cfg = ReachConfig(length=4.0, dx=0.25, run_flows=1.0, avg_frac=0.99)
+    tb = replace(R.boat_type("custom"), hull_L=1.0, hull_B=0.05, draft=0.05, n_props=1, prop_D=0.1,
+                 prop_depth=0.45, tilt_deg=10.0, prop_x=0.2, power_kW=1.0)
     r = R.build_reach(sec, 1.0, 1.2, 2e-4, R.Fleet(tb, rows=1, per_row=1, x_first=1.0), cfg)
     assert r.dt == float(g["dt"])
     t_total = cfg.run_flows * cfg.length / max(r.V, 0.05)

@@ -252,3 +252,24 @@ def test_storage_formats_against_fp64_after_1000_steps(storage, vel_tol, eta_tol
     assert err_v <= vel_tol
     assert err_eta <= eta_tol
     g.close()
+
+
+@pytest.mark.parametrize("tag", ["boat", "base"])
+def test_run3d_matches_oracle_run_bitwise(tag):
+    """The product run driver (ramp, uref, sums on the GPU) against the
+    oracle's restatement of the reference cadence, on the toy reach whose
+    reference run the CPU tests pin."""
+    from oracle.oracle import OracleRun
+    from tests.test_run_driver import oracle_inputs, toy_reach
+    r = toy_reach(tag == "boat")
+    orc = OracleRun(oracle_inputs(r), precision="f32")
+    run = R.Run3D(toy_reach(tag == "boat"))
+    for m in (40, 30, 30):
+        orc.advance(m)
+        run.advance(m)
+    sums, navg = run.lbm.sums()
+    assert navg == orc.navg == run.navg and run.t == orc.t
+    fl = ~r.solid.ravel()
+    for k in ("ux", "uy", "uz", "rho", "sp", "uxx"):
+        assert np.array_equal(sums[k][fl], orc.sum[k][fl]), k
+    run.close()

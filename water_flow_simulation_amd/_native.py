@@ -67,6 +67,7 @@ _SIGS = {
     "wfs_accumulate": ([_P], ctypes.c_int),
     "wfs_reset_sums": ([_P], ctypes.c_int),
     "wfs_get_sums": ([_P] * 8, ctypes.c_int),
+    "wfs_get_sums_f64": ([_P] * 8, ctypes.c_int),
     "wfs_profile_x": ([_P, _I32, _I32, _I32, _P], ctypes.c_int),
     "wfs_total": ([_P, _I32, _P], ctypes.c_int),
     "wfs_check_finite": ([_P], ctypes.c_int),

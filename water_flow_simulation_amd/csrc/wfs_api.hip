@@ -128,7 +128,7 @@ struct wfs_solver {
     virtual int set_pops(const float* f) = 0;
     virtual int accumulate() = 0;
     virtual int reset_sums() = 0;
-    virtual int get_sums(float* const* out) = 0;
+    virtual int get_sums(void* const* out, bool f64) = 0;
     virtual int profile_x(int field, int z0, int z1, double* out) = 0;
     virtual int total(int field, double* out) = 0;
     virtual int check_finite() = 0;
@@ -431,9 +431,9 @@ struct Solver final : wfs_solver {
         return WFS_OK;
     }
 
-    int get_sums(float* const* out) override {
+    int get_sums(void* const* out, bool f64) override {
         for (int k = 0; k < 6; ++k) {
-            int rc = get_cells(sums[k], out[k], false);
+            int rc = get_cells(sums[k], out[k], f64);
             if (rc) return rc;
         }
         return WFS_OK;
@@ -747,8 +747,20 @@ int wfs_get_sums(wfs_solver* h, float* ux, float* uy, float* uz, float* rho, flo
     int rc = check_handle(h);
     if (rc) return rc;
     DeviceGuard g(h->device);
-    float* const out[6] = {ux, uy, uz, rho, sp, uxx};
-    rc = h->get_sums(out);
+    void* const out[6] = {ux, uy, uz, rho, sp, uxx};
+    rc = h->get_sums(out, false);
+    if (rc) return rc;
+    if (navg) *navg = h->navg;
+    return WFS_OK;
+}
+
+int wfs_get_sums_f64(wfs_solver* h, double* ux, double* uy, double* uz, double* rho, double* sp, double* uxx,
+                     int32_t* navg) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    void* const out[6] = {ux, uy, uz, rho, sp, uxx};
+    rc = h->get_sums(out, true);
     if (rc) return rc;
     if (navg) *navg = h->navg;
     return WFS_OK;

@@ -215,12 +215,13 @@ class LBM3D:
     def reset_sums(self):
         nat.check(nat.lib().wfs_reset_sums(self.handle))
 
-    def sums(self):
+    def sums(self, dtype=np.float32):
         """The six running sums (ux, uy, uz, rho, sp, uxx) and the sample count."""
         names = ("ux", "uy", "uz", "rho", "sp", "uxx")
-        out = {k: np.empty(self.N, np.float32) for k in names}
+        out = {k: np.empty(self.N, dtype) for k in names}
         n = ctypes.c_int32(0)
-        nat.check(nat.lib().wfs_get_sums(self.handle, *(nat.ptr(out[k]) for k in names), ctypes.byref(n)))
+        fn = nat.lib().wfs_get_sums_f64 if dtype == np.float64 else nat.lib().wfs_get_sums
+        nat.check(fn(self.handle, *(nat.ptr(out[k]) for k in names), ctypes.byref(n)))
         return out, int(n.value)
 
     def profile_x(self, field, z0=0, z1=None):

@@ -518,10 +518,11 @@ class Run3D:
         return self.lbm.nstep * self.r.cells / max(self.steps_wall, 1e-9) / 1e6
 
     def _raw(self, name, mean):
+        wide = np.float64 if self.lbm.storage == "f64" else np.float32
         if mean and self.navg > 0:
-            sums, n = self.lbm.sums()
+            sums, n = self.lbm.sums(wide)
             return sums[name].astype(np.float64) / n
-        rho, ux, uy, uz = self.lbm.macro()
+        rho, ux, uy, uz = self.lbm.macro(wide)
         if name == "sp":
             return np.sqrt(ux * ux + uy * uy + uz * uz).astype(np.float64)
         if name == "uxx":
