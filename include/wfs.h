@@ -162,6 +162,27 @@ int wfs_check_finite(wfs_solver* h);
 int wfs_timing_enable(wfs_solver* h, int32_t on);
 int wfs_timing_read(wfs_solver* h, double* kernel_ms, int64_t* launches);
 
+/* ---- Batches: many reaches stepped together (the boat-test sweep).
+ * A batch references solvers that were created and uploaded as usual (same
+ * storage format, block_x and device). Each run step is one launch over every
+ * case; the run driver's 20-step cadence of floodsim/reach3d.py:424-453
+ * (thrust ramp, uref refresh, time-mean sums once t >= t_avg) runs on the
+ * device from per-case schedules, and with use_graph a 20-step chunk is
+ * captured once as a HIP graph and replayed. Replaces the sequential per-case
+ * loop of floodsim/boat_tests.py:79-110 (run_test). While a batch runs, its
+ * cases must not be used directly; wfs_batch_synchronize() hands them back
+ * with nstep, the sample count and the force scale brought up to date. */
+typedef struct wfs_batch wfs_batch;
+int wfs_batch_create(wfs_solver* const* cases, int32_t n, wfs_batch** out);
+int wfs_batch_destroy(wfs_batch* b);
+/* Case k: time step dt (s), ramp length t_ramp (s; <= 0: no ramp), start of
+ * the averaging window t_avg (s), and the case's current time t0 (s). */
+int wfs_batch_set_schedule(wfs_batch* b, int32_t k, double dt, double t_ramp, double t_avg, double t0);
+int wfs_batch_run(wfs_batch* b, int32_t nsteps, int32_t use_graph);
+int wfs_batch_synchronize(wfs_batch* b);
+/* Case k's time and sample count as of the last synchronize. */
+int wfs_batch_time(wfs_batch* b, int32_t k, double* t, int32_t* navg);
+
 /* Device copy bandwidth: copies `bytes` bytes between two fresh buffers
  * `iters` times with a grid-stride 16-byte copy kernel and reports
  * 2*bytes*iters / elapsed in GB/s (1e9 bytes/s). The practical HBM ceiling

@@ -73,6 +73,13 @@ _SIGS = {
     "wfs_check_finite": ([_P], ctypes.c_int),
     "wfs_timing_enable": ([_P, _I32], ctypes.c_int),
     "wfs_timing_read": ([_P, _P, _P], ctypes.c_int),
+    "wfs_batch_create": ([_P, _I32, _P], ctypes.c_int),
+    "wfs_batch_destroy": ([_P], ctypes.c_int),
+    "wfs_batch_set_schedule": ([_P, _I32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+                               ctypes.c_int),
+    "wfs_batch_run": ([_P, _I32, _I32], ctypes.c_int),
+    "wfs_batch_synchronize": ([_P], ctypes.c_int),
+    "wfs_batch_time": ([_P, _I32, _P, _P], ctypes.c_int),
     "wfs_bandwidth_probe": ([_I32, ctypes.c_int64, _I32, _P], ctypes.c_int),
     "wfs_device_bytes": ([_P], ctypes.c_int64),
 }

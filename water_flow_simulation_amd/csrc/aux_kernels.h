@@ -146,6 +146,66 @@ __global__ void k_accumulate(const R* __restrict__ rho, const R* __restrict__ ux
     s_uxx[c] += a * a;
 }
 
+// ---------------------------------------------------------------- batched cadence
+// Per-case run scalars kept on the device by a batch (the state of
+// floodsim/reach3d.py Run3D.advance between chunks: time, ramp, averaging).
+template <typename R>
+struct CaseSched {
+    double t, dt, t_ramp, t_avg;
+    R fscale;          // force scale used by the step
+    int ramp_done;
+    int navg;
+    int accumulate;    // this chunk adds to the time-mean sums
+};
+
+// Before a chunk: thrust ramp F = F0 * s, s = smoothstep(min(t / t_ramp, 1))
+// evaluated in double and rounded to float (reach3d.py:431-437).
+template <typename R>
+__global__ void k_batch_pre(CaseSched<R>* __restrict__ sch, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    CaseSched<R> c = sch[k];
+    if (!c.ramp_done) {
+        double f = c.t / c.t_ramp;
+        f = f < 1.0 ? f : 1.0;
+        const float s = (float)(f * f * (3.0 - 2.0 * f));
+        c.fscale = R(s);
+        c.ramp_done = s >= 1.0f;
+    }
+    sch[k] = c;
+}
+
+// After a chunk of m steps: t += m dt; the time-mean sums take this state
+// once t reaches the averaging window (reach3d.py:441-451).
+template <typename R>
+__global__ void k_batch_post(CaseSched<R>* __restrict__ sch, int n, int m) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    CaseSched<R> c = sch[k];
+    c.t += m * c.dt;
+    c.accumulate = c.t >= c.t_avg;
+    if (c.accumulate) c.navg += 1;
+    sch[k] = c;
+}
+
+// k_accumulate for one case, skipped when its schedule says so.
+template <typename R>
+__global__ void k_accumulate_if(const CaseSched<R>* __restrict__ sch, int k, const R* __restrict__ rho,
+                                const R* __restrict__ ux, const R* __restrict__ uy, const R* __restrict__ uz,
+                                R* __restrict__ s_ux, R* __restrict__ s_uy, R* __restrict__ s_uz,
+                                R* __restrict__ s_rho, R* __restrict__ s_sp, R* __restrict__ s_uxx, long long N) {
+    if (!sch[k].accumulate) return;
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const R a = ux[c], b = uy[c], d = uz[c];
+    s_ux[c] += a;
+    s_uy[c] += b;
+    s_uz[c] += d;
+    s_rho[c] += rho[c];
+    s_sp[c] += sqrt(a * a + b * b + d * d);
+    s_uxx[c] += a * a;
+}
+
 template <typename R>
 __global__ void k_fill_macro(R* __restrict__ rho, R* __restrict__ ux, R* __restrict__ uy, R* __restrict__ uz,
                              long long N) {

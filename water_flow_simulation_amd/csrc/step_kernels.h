@@ -202,8 +202,7 @@ struct StepArgs {
 // the 8 XCDs (group L runs on XCD L % 8), each with its own L2. All x-tiles of
 // a row get the same L % 8, so the lines a row's tiles share at their x-edges
 // (the +-1 pull shift) and its partial-line writes meet in one L2.
-__device__ __forceinline__ bool tile_of(int tiles, int rows, int& row, int& tile) {
-    const unsigned L = blockIdx.x;
+__device__ __forceinline__ bool tile_of(unsigned L, int tiles, int rows, int& row, int& tile) {
     const unsigned q = L >> 3;
     tile = int(q % unsigned(tiles));
     row = int((q / unsigned(tiles)) * 8u + (L & 7u));
@@ -332,14 +331,11 @@ __device__ __forceinline__ void collide_pairs(R* f, bool forced, R fx, R fy, R f
 #define WFS_STEP_WAVES_PER_EU 1   // minimum resident waves per SIMD requested from the register allocator
 #endif
 
+// One x-tile of one row: the whole step for its cells.
 template <class Codec>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WFS_STEP_WAVES_PER_EU)))
-k_step(const typename Codec::T* __restrict__ fs,
-                                              typename Codec::T* __restrict__ fd,
-                                              StepArgs<typename Codec::R> a) {
+__device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ fs, typename Codec::T* __restrict__ fd,
+                                          const StepArgs<typename Codec::R>& a, int row, int tile) {
     using R = typename Codec::R;
-    int row, tile;
-    if (!tile_of(a.tiles, a.rows, row, tile)) return;   // whole work-group
     const int x = tile * int(blockDim.x) + int(threadIdx.x);
     const bool inrow = x < a.nx;
     const int z = row / a.ny;
@@ -566,6 +562,43 @@ k_step(const typename Codec::T* __restrict__ fs,
             Codec::template st<i>(out + (long long)i * N, f[i]);
         });
     }
+}
+
+template <class Codec>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WFS_STEP_WAVES_PER_EU)))
+k_step(const typename Codec::T* __restrict__ fs, typename Codec::T* __restrict__ fd, StepArgs<typename Codec::R> a) {
+    int row, tile;
+    if (!tile_of(blockIdx.x, a.tiles, a.rows, row, tile)) return;   // whole work-group
+    step_tile<Codec>(fs, fd, a, row, tile);
+}
+
+// ---------------------------------------------------------------- many reaches in one launch
+// Per-case view for the batched step: the case's step arguments, its two
+// population buffers, the device scalar that holds its current force scale
+// (written by the cadence kernel), and the first work-group of the case in
+// the flat launch (a multiple of 8, so the XCD map of each case is kept).
+template <class Codec>
+struct CaseArgs {
+    StepArgs<typename Codec::R> a;
+    typename Codec::T* f[2];
+    const typename Codec::R* fscale;
+    unsigned group_begin;
+};
+
+// All cases advance one step: work-group g belongs to case group_case[g]
+// and covers one (row, x-tile) of it. parity selects the source buffer.
+template <class Codec>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WFS_STEP_WAVES_PER_EU)))
+k_step_batch(const CaseArgs<Codec>* __restrict__ cases, const unsigned short* __restrict__ group_case, int parity,
+             int write_all) {
+    const unsigned g = blockIdx.x;
+    const CaseArgs<Codec>& cs = cases[group_case[g]];
+    StepArgs<typename Codec::R> a = cs.a;
+    a.fscale = *cs.fscale;
+    a.write_all = write_all;
+    int row, tile;
+    if (!tile_of(g - cs.group_begin, a.tiles, a.rows, row, tile)) return;   // whole work-group
+    step_tile<Codec>(cs.f[parity], cs.f[parity ^ 1], a, row, tile);
 }
 
 // ---------------------------------------------------------------- inlet/outlet rows

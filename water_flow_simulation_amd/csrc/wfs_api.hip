@@ -305,8 +305,7 @@ struct Solver final : wfs_solver {
         return WFS_OK;
     }
 
-    int step(int n) override {
-        StepArgs<R> a;
+    void fill_args(StepArgs<R>& a) {
         a.flag = flags();
         a.wmask = static_cast<const uint32_t*>(wmask.p);
         a.Fx = rp(Fx);
@@ -334,6 +333,12 @@ struct Solver final : wfs_solver {
         a.N = Np;
         a.uin = rp(uin);
         a.bc_out_fused = (d.nx - 1) % 64 != 0 ? 1 : 0;
+        a.write_all = 0;
+    }
+
+    int step(int n) override {
+        StepArgs<R> a;
+        fill_args(a);
         const long long groups = (long long)((a.rows + 7) / 8) * 8 * a.tiles;
         const dim3 grid((unsigned)groups);
         for (int k = 0; k < n; ++k) {
@@ -502,6 +507,210 @@ struct Solver final : wfs_solver {
         WFS_HIP(hipStreamSynchronize(stream));
         if (cnt) return fail(WFS_ENONFINITE, std::to_string(cnt) + " fluid cells hold a non-finite state after step " +
                                                  std::to_string(nstep));
+        return WFS_OK;
+    }
+};
+
+// ---------------------------------------------------------------- batches
+// A batch steps several uploaded solvers (same storage format, same device)
+// together: one step launch covers every case, and the 20-step cadence of the
+// run driver (ramp, uref, time-mean sums) runs on the device from per-case
+// schedules, so a whole chunk can be captured once as a HIP graph and replayed.
+struct BatchBase {
+    int device = 0;
+    int storage = 0;
+    hipStream_t stream = nullptr;
+    virtual ~BatchBase() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    virtual int set_schedule(int k, double dt, double t_ramp, double t_avg, double t0) = 0;
+    virtual int run(int nsteps, int use_graph) = 0;
+    virtual int synchronize() = 0;
+    virtual int time_of(int k, double* t, int32_t* navg) = 0;
+};
+
+template <class Codec>
+struct Batch final : BatchBase {
+    using R = typename Codec::R;
+    std::vector<Solver<Codec>*> cases;
+    std::vector<CaseArgs<Codec>> args;   // host copy of the per-case step views
+    DevBuf d_args, d_group_case, d_sched;
+    std::vector<CaseSched<R>> sched;     // host mirror, valid after synchronize()
+    unsigned groups = 0;
+    int parity = 0;                      // 0: every case reads its buffer f[0]
+    int64_t pending_steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool dirty_sched = true;
+
+    ~Batch() override {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+
+    int init(wfs_solver* const* hs, int n) {
+        int bx0 = -1;
+        std::vector<unsigned short> gcase;
+        for (int k = 0; k < n; ++k) {
+            auto* c = dynamic_cast<Solver<Codec>*>(hs[k]);
+            if (!c) return fail(WFS_EINVAL, "wfs_batch_create: cases must share one storage format");
+            if (!c->uploaded) return fail(WFS_EINVAL, "wfs_batch_create: case " + std::to_string(k) + " has no geometry");
+            if (c->device != device) return fail(WFS_EINVAL, "wfs_batch_create: cases must be on one device");
+            if (bx0 < 0) bx0 = c->bx;
+            if (c->bx != bx0) return fail(WFS_EINVAL, "wfs_batch_create: cases must share block_x");
+            WFS_HIP(hipStreamSynchronize(c->stream));
+            cases.push_back(c);
+            CaseArgs<Codec> ca{};
+            c->fill_args(ca.a);
+            ca.f[0] = c->fp(c->cur);
+            ca.f[1] = c->fp(c->cur ^ 1);
+            ca.fscale = nullptr;   // set below, once the schedule array exists
+            ca.group_begin = (unsigned)gcase.size();
+            const long long ng = (long long)((ca.a.rows + 7) / 8) * 8 * ca.a.tiles;
+            if ((long long)gcase.size() + ng > (1ll << 31)) return fail(WFS_EINVAL, "wfs_batch_create: too many cells");
+            gcase.insert(gcase.end(), (size_t)ng, (unsigned short)k);
+            args.push_back(ca);
+            CaseSched<R> sc{};
+            sc.t = 0.0;
+            sc.dt = 1.0;
+            sc.t_ramp = 0.0;
+            sc.t_avg = 1e300;
+            sc.fscale = c->fscale;
+            sc.ramp_done = 1;
+            sc.navg = c->navg;
+            sc.accumulate = 0;
+            sched.push_back(sc);
+        }
+        groups = (unsigned)gcase.size();
+        int rc;
+        if ((rc = cases[0]->alloc_buf(d_group_case, gcase.size() * sizeof(unsigned short)))) return rc;
+        if ((rc = cases[0]->alloc_buf(d_sched, sched.size() * sizeof(CaseSched<R>)))) return rc;
+        if ((rc = cases[0]->alloc_buf(d_args, args.size() * sizeof(CaseArgs<Codec>)))) return rc;
+        for (size_t k = 0; k < args.size(); ++k)
+            args[k].fscale = &static_cast<CaseSched<R>*>(d_sched.p)[k].fscale;
+        WFS_HIP(hipMemcpyAsync(d_group_case.p, gcase.data(), gcase.size() * sizeof(unsigned short),
+                               hipMemcpyHostToDevice, stream));
+        WFS_HIP(hipMemcpyAsync(d_args.p, args.data(), args.size() * sizeof(CaseArgs<Codec>), hipMemcpyHostToDevice,
+                               stream));
+        WFS_HIP(hipStreamSynchronize(stream));
+        return WFS_OK;
+    }
+
+    int set_schedule(int k, double dt, double t_ramp, double t_avg, double t0) override {
+        if (k < 0 || k >= (int)cases.size()) return fail(WFS_EINVAL, "wfs_batch_set_schedule: case out of range");
+        if (!(dt > 0.0)) return fail(WFS_EINVAL, "wfs_batch_set_schedule: dt must be positive");
+        CaseSched<R>& c = sched[(size_t)k];
+        c.t = t0;
+        c.dt = dt;
+        c.t_ramp = t_ramp;
+        c.t_avg = t_avg;
+        c.ramp_done = t_ramp <= 0.0;
+        if (!c.ramp_done) c.fscale = R(0);
+        dirty_sched = true;
+        return WFS_OK;
+    }
+
+    // The kernels of one chunk of m steps, queued on the batch stream.
+    int enqueue_chunk(int m, int parity0) {
+        const int n = (int)cases.size();
+        CaseSched<R>* ds = static_cast<CaseSched<R>*>(d_sched.p);
+        hipLaunchKernelGGL(k_batch_pre<R>, dim3(blocks_for(n, 64)), dim3(64), 0, stream, ds, n);
+        WFS_LAUNCH_CHECK();
+        for (auto* c : cases) {
+            if (!c->have_bedwall) continue;
+            hipLaunchKernelGGL(k_uref<R>, dim3(blocks_for(c->d.nx, 64), c->d.ny), dim3(64), 0, stream,
+                               c->rp(c->ux), c->rp(c->uy), c->flags(), static_cast<const uint8_t*>(c->bedwall.p),
+                               c->rp(c->uref), c->d.nx, c->px, c->d.ny, c->d.nz);
+            WFS_LAUNCH_CHECK();
+        }
+        const CaseArgs<Codec>* da = static_cast<const CaseArgs<Codec>*>(d_args.p);
+        int par = parity0;
+        for (int s = 0; s < m; ++s) {
+            hipLaunchKernelGGL(k_step_batch<Codec>, dim3(groups), dim3(cases[0]->bx), 0, stream, da,
+                               static_cast<const unsigned short*>(d_group_case.p), par, s == m - 1 ? 1 : 0);
+            WFS_LAUNCH_CHECK();
+            for (size_t k = 0; k < cases.size(); ++k) {
+                Solver<Codec>* c = cases[k];
+                if ((c->d.nx - 1) % 64 != 0) continue;   // outlet row fused into the step
+                hipLaunchKernelGGL(k_bc<Codec>, dim3(blocks_for(c->rows(), 64)), dim3(64), 0, stream,
+                                   args[k].f[par ^ 1], c->flags(), c->rp(c->uin), c->d.nx, c->px, c->rows(), c->Np,
+                                   R(1), 2);
+                WFS_LAUNCH_CHECK();
+            }
+            par ^= 1;
+        }
+        hipLaunchKernelGGL(k_batch_post<R>, dim3(blocks_for(n, 64)), dim3(64), 0, stream, ds, n, m);
+        WFS_LAUNCH_CHECK();
+        for (size_t k = 0; k < cases.size(); ++k) {
+            Solver<Codec>* c = cases[k];
+            hipLaunchKernelGGL(k_accumulate_if<R>, dim3(blocks_for(c->Np, 256)), dim3(256), 0, stream,
+                               static_cast<const CaseSched<R>*>(d_sched.p), (int)k, c->rp(c->rho), c->rp(c->ux),
+                               c->rp(c->uy), c->rp(c->uz), c->rp(c->sums[0]), c->rp(c->sums[1]), c->rp(c->sums[2]),
+                               c->rp(c->sums[3]), c->rp(c->sums[4]), c->rp(c->sums[5]), c->Np);
+            WFS_LAUNCH_CHECK();
+        }
+        return WFS_OK;
+    }
+
+    int run(int nsteps, int use_graph) override {
+        if (nsteps < 0) return fail(WFS_EINVAL, "wfs_batch_run: nsteps must be >= 0");
+        if (dirty_sched) {
+            WFS_HIP(hipMemcpyAsync(d_sched.p, sched.data(), sched.size() * sizeof(CaseSched<R>),
+                                   hipMemcpyHostToDevice, stream));
+            dirty_sched = false;
+        }
+        const int cadence = 20;
+        const int full = nsteps / cadence, rest = nsteps % cadence;
+        int rc;
+        if (use_graph && full > 0) {
+            if (!graph_exec) {
+                // one chunk of 20 steps (an even count, so it starts and ends on the same parity)
+                WFS_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                rc = enqueue_chunk(cadence, parity);
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(stream, &g);
+                if (rc) return rc;
+                if (e != hipSuccess) return fail(WFS_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+                graph = g;
+                WFS_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+            }
+            for (int k = 0; k < full; ++k) WFS_HIP(hipGraphLaunch(graph_exec, stream));
+        } else {
+            for (int k = 0; k < full; ++k)
+                if ((rc = enqueue_chunk(cadence, parity))) return rc;
+        }
+        if (rest) {
+            if ((rc = enqueue_chunk(rest, parity))) return rc;
+            parity ^= rest & 1;
+            if (graph_exec) {   // the captured chunk assumed the old parity
+                (void)hipGraphExecDestroy(graph_exec);
+                (void)hipGraphDestroy(graph);
+                graph_exec = nullptr;
+                graph = nullptr;
+            }
+        }
+        pending_steps += nsteps;
+        return WFS_OK;
+    }
+
+    int synchronize() override {
+        WFS_HIP(hipStreamSynchronize(stream));
+        WFS_HIP(hipMemcpy(sched.data(), d_sched.p, sched.size() * sizeof(CaseSched<R>), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < cases.size(); ++k) {
+            Solver<Codec>* c = cases[k];
+            c->nstep += pending_steps;
+            c->navg = sched[k].navg;
+            c->fscale = sched[k].fscale;
+            c->cur = (args[k].f[parity] == c->fp(0)) ? 0 : 1;
+        }
+        pending_steps = 0;
+        return WFS_OK;
+    }
+
+    int time_of(int k, double* t, int32_t* navg) override {
+        if (k < 0 || k >= (int)cases.size()) return fail(WFS_EINVAL, "wfs_batch_time: case out of range");
+        *t = sched[(size_t)k].t;
+        *navg = sched[(size_t)k].navg;
         return WFS_OK;
     }
 };
@@ -812,6 +1021,73 @@ int wfs_timing_read(wfs_solver* h, double* kernel_ms, int64_t* launches) {
     h->timed_ms = 0.0;
     h->timed_launches = 0;
     return WFS_OK;
+}
+
+struct wfs_batch {
+    std::unique_ptr<BatchBase> impl;
+};
+
+int wfs_batch_create(wfs_solver* const* cases, int32_t n, wfs_batch** out) {
+    if (!cases || !out || n < 1) return fail(WFS_EINVAL, "wfs_batch_create: need at least one case");
+    if (n > 65535) return fail(WFS_EINVAL, "wfs_batch_create: at most 65535 cases");
+    *out = nullptr;
+    for (int k = 0; k < n; ++k)
+        if (!cases[k]) return fail(WFS_EINVAL, "wfs_batch_create: null case handle");
+    const int storage = cases[0]->d.storage;
+    std::unique_ptr<BatchBase> b;
+    switch (storage) {
+        case WFS_STORE_F32: b.reset(new Batch<wfs::CodecF32>()); break;
+        case WFS_STORE_F16DEV: b.reset(new Batch<wfs::CodecF16Dev>()); break;
+        case WFS_STORE_BF16DEV: b.reset(new Batch<wfs::CodecBF16Dev>()); break;
+        case WFS_STORE_F64: b.reset(new Batch<wfs::CodecF64>()); break;
+        case WFS_STORE_F32_FAST: b.reset(new Batch<wfs::CodecF32Fast>()); break;
+        default: return fail(WFS_EINVAL, "wfs_batch_create: unknown storage");
+    }
+    b->device = cases[0]->device;
+    b->storage = storage;
+    DeviceGuard g(b->device);
+    WFS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    int rc = 0;
+    switch (storage) {
+        case WFS_STORE_F32: rc = static_cast<Batch<wfs::CodecF32>*>(b.get())->init(cases, n); break;
+        case WFS_STORE_F16DEV: rc = static_cast<Batch<wfs::CodecF16Dev>*>(b.get())->init(cases, n); break;
+        case WFS_STORE_BF16DEV: rc = static_cast<Batch<wfs::CodecBF16Dev>*>(b.get())->init(cases, n); break;
+        case WFS_STORE_F64: rc = static_cast<Batch<wfs::CodecF64>*>(b.get())->init(cases, n); break;
+        case WFS_STORE_F32_FAST: rc = static_cast<Batch<wfs::CodecF32Fast>*>(b.get())->init(cases, n); break;
+    }
+    if (rc) return rc;
+    *out = new wfs_batch{std::move(b)};
+    return WFS_OK;
+}
+
+int wfs_batch_destroy(wfs_batch* b) {
+    if (!b) return WFS_OK;
+    DeviceGuard g(b->impl->device);
+    (void)hipStreamSynchronize(b->impl->stream);
+    delete b;
+    return WFS_OK;
+}
+
+int wfs_batch_set_schedule(wfs_batch* b, int32_t k, double dt, double t_ramp, double t_avg, double t0) {
+    if (!b) return fail(WFS_EINVAL, "null batch");
+    return b->impl->set_schedule(k, dt, t_ramp, t_avg, t0);
+}
+
+int wfs_batch_run(wfs_batch* b, int32_t nsteps, int32_t use_graph) {
+    if (!b) return fail(WFS_EINVAL, "null batch");
+    DeviceGuard g(b->impl->device);
+    return b->impl->run(nsteps, use_graph);
+}
+
+int wfs_batch_synchronize(wfs_batch* b) {
+    if (!b) return fail(WFS_EINVAL, "null batch");
+    DeviceGuard g(b->impl->device);
+    return b->impl->synchronize();
+}
+
+int wfs_batch_time(wfs_batch* b, int32_t k, double* t, int32_t* navg) {
+    if (!b || !t || !navg) return fail(WFS_EINVAL, "wfs_batch_time: null argument");
+    return b->impl->time_of(k, t, navg);
 }
 
 int wfs_bandwidth_probe(int32_t device, int64_t bytes, int32_t iters, double* gbs) {
