@@ -41,7 +41,8 @@ enum wfs_status {
 
 /* Population storage. F32 is the reference's layout and precision, with the
  * reference's operation order (bit-exact against an IEEE FP32 evaluation);
- * the two 16-bit formats store (f_i - w_i) * 2^8 and compute in FP32; F64
+ * the two 16-bit formats store f_i - e_i, e_i the equilibrium of the
+ * mean inlet velocity given at wfs_upload, and compute in FP32; F64
  * stores and computes in double (the FP64 run of the same path); F32_FAST
  * stores FP32 and uses the same opposite-pair collision as the 16-bit
  * formats (agreement within rounding, not bit for bit). */

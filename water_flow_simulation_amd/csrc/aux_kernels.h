@@ -42,7 +42,7 @@ __global__ void k_unpad(const S* __restrict__ padded, D* __restrict__ dense, int
 // (floodsim/lbm3d.py:362-364, 377-384).
 template <class Codec>
 __global__ void k_init(typename Codec::T* __restrict__ f, const uint8_t* __restrict__ flag,
-                       const typename Codec::R* __restrict__ uin, int px, long long N) {
+                       const typename Codec::R* __restrict__ uin, int px, long long N, Center ctr) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
     const long long j = c / px;  // row index y + ny*z
@@ -53,7 +53,7 @@ __global__ void k_init(typename Codec::T* __restrict__ f, const uint8_t* __restr
         const double w = i == 0 ? W[0] : (i < 7 ? W[1] : W[2]);
         const double cu = 3.0 * kCX[i] * u;
         const double v = w * (1.0 + cu + 0.5 * cu * cu - 1.5 * u * u);
-        Codec::template st<i>(f + (long long)i * N + c, (typename Codec::R)v);
+        Codec::template st<i>(f + (long long)i * N + c, (typename Codec::R)v, store_key<Codec>(ctr, i));
     });
 }
 
@@ -220,27 +220,28 @@ __global__ void k_fill_macro(R* __restrict__ rho, R* __restrict__ ux, R* __restr
 // Populations: padded storage -> dense real array f[i*Nd + k], and back.
 template <class Codec, typename O>
 __global__ void k_decode(const typename Codec::T* __restrict__ f, O* __restrict__ out, int nx, int px,
-                         long long Npad, long long Nd) {
+                         long long Npad, long long Nd, Center ctr) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= Nd) return;
     const long long j = k / nx;
     const long long c = (k - j * nx) + (long long)px * j;
     static_for<Q>([&](auto I) {
         constexpr int i = I;
-        out[(long long)i * Nd + k] = (O)Codec::template ld<i>(f + (long long)i * Npad + c);
+        out[(long long)i * Nd + k] = (O)Codec::template ld<i>(f + (long long)i * Npad + c, ctr.c[center_class(i)]);
     });
 }
 
 template <class Codec>
 __global__ void k_encode(typename Codec::T* __restrict__ f, const float* __restrict__ in, int nx, int px,
-                         long long Npad, long long Nd) {
+                         long long Npad, long long Nd, Center ctr) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= Nd) return;
     const long long j = k / nx;
     const long long c = (k - j * nx) + (long long)px * j;
     static_for<Q>([&](auto I) {
         constexpr int i = I;
-        Codec::template st<i>(f + (long long)i * Npad + c, (typename Codec::R)in[(long long)i * Nd + k]);
+        Codec::template st<i>(f + (long long)i * Npad + c, (typename Codec::R)in[(long long)i * Nd + k],
+                              store_key<Codec>(ctr, i));
     });
 }
 

@@ -25,6 +25,19 @@ __host__ __device__ constexpr T weight(int i) {
     return i == 0 ? T(1) / T(3) : (i < 7 ? T(1) / T(18) : T(1) / T(36));
 }
 
+// Centres of the 16-bit storage formats, passed by value to kernels. The
+// centre of direction i depends only on its weight and c_ix, so there are
+// seven: class 0 rest; 1..3 axis directions with c_x = 0, +1, -1; 4..6 edge
+// directions with c_x = 0, +1, -1. c = e, n2 = -2 e (exact).
+constexpr int kCenterClasses = 7;
+__host__ __device__ constexpr int center_class(int i) {
+    return i == 0 ? 0 : (i < 7 ? 1 : 4) + (kCX[i] == 0 ? 0 : (kCX[i] > 0 ? 1 : 2));
+}
+struct Center {
+    float c[kCenterClasses];
+    float n2[kCenterClasses];
+};
+
 // Per-cell flag byte.
 enum : uint8_t {
     FLAG_SOLID = 1u << 0,     // solid cell: never updated

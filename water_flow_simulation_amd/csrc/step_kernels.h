@@ -59,16 +59,20 @@ __device__ __forceinline__ void store_pop(T* p, T v) {
 #endif
 }
 
+// Codec interface: ld/st/rt take the centre e of the stored direction (used
+// by the 16-bit formats only): ld decodes, st encodes and stores, rt is the
+// value a store followed by a load gives back.
 struct CodecF32 {
     using T = float;
     using R = float;
     static constexpr bool kExact = true;   // operand order of the reference update, bit-exact
     template <int I>
-    __device__ static R ld(const T* p) { return *p; }
+    __device__ static R ld(const T* p, float) { return *p; }
     template <int I>
-    __device__ static void st(T* p, R v) { store_pop(p, v); }
+    __device__ static void st(T* p, R v, float) { store_pop(p, v); }
     template <int I>
-    __device__ static R rt(R v) { return v; }   // value after a store/load round trip
+    __device__ static R rt(R v, float, float) { return v; }
+    static constexpr bool kStoreUsesN2 = false;
 };
 
 // FP32 storage with the pair-form collision (not bit-exact; judged by tolerance).
@@ -81,37 +85,38 @@ struct CodecF64 {
     using R = double;
     static constexpr bool kExact = true;
     template <int I>
-    __device__ static R ld(const T* p) { return *p; }
+    __device__ static R ld(const T* p, float) { return *p; }
     template <int I>
-    __device__ static void st(T* p, R v) { store_pop(p, v); }
+    __device__ static void st(T* p, R v, float) { store_pop(p, v); }
     template <int I>
-    __device__ static R rt(R v) { return v; }
+    __device__ static R rt(R v, float, float) { return v; }
+    static constexpr bool kStoreUsesN2 = false;
 };
 
-// 16-bit deviation storage: T holds (f_i - w_i) * 2^SHIFT, decoded as
-// w_i + t * 2^-SHIFT. The power-of-two scale is exact; it lifts the small
-// deviations out of the FP16 subnormal range. Arithmetic stays FP32.
-constexpr int kDevShift = 8;
-
+// 16-bit deviation storage. FP16 holds 2 (f_i - e_i), decoded as
+// e_i + t / 2; BF16 holds f_i - e_i. e_i is the equilibrium of the mean
+// inlet flow, e_i = w_i (1 + 3 c_ix U0 + 4.5 (c_ix U0)^2 - 1.5 U0^2) (the
+// lattice weight when U0 = 0). Centring on the mean flow leaves only the
+// departures from it to the 11 (FP16) or 8 (BF16) significand bits. Small FP16
+// departures fall into the subnormal range, whose absolute spacing is finer
+// than that of the normal range the flow needs; the factor 2 (exact) only
+// lets each FP16 decode and encode be one mixed-precision FMA whose constants
+// are an inline 0.5 or 2 and the per-direction centre. Arithmetic is FP32.
 struct CodecF16Dev {
     using T = __half;
     using R = float;
     static constexpr bool kExact = false;
     template <int I>
-    __device__ static R ld(const T* p) {
-        return fmaf(__half2float(*p), 1.0f / float(1 << kDevShift), weight<float>(I));
+    __device__ static R ld(const T* p, float e) { return fmaf(__half2float(*p), 0.5f, e); }
+    // n2 = -2 e
+    __device__ static __half enc(R v, float n2) { return __float2half_rn(fmaf(v, 2.0f, n2)); }
+    template <int I>
+    __device__ static void st(T* p, R v, float n2) {
+        store_pop(reinterpret_cast<unsigned short*>(p), __half_as_ushort(enc(v, n2)));
     }
     template <int I>
-    __device__ static void st(T* p, R v) {
-        const __half h = __float2half_rn(fmaf(v, float(1 << kDevShift), -weight<float>(I) * float(1 << kDevShift)));
-        store_pop(reinterpret_cast<unsigned short*>(p), __half_as_ushort(h));
-    }
-    template <int I>
-    __device__ static R rt(R v) {
-        return fmaf(__half2float(__float2half_rn(fmaf(v, float(1 << kDevShift),
-                                                         -weight<float>(I) * float(1 << kDevShift)))),
-                    1.0f / float(1 << kDevShift), weight<float>(I));
-    }
+    __device__ static R rt(R v, float e, float n2) { return fmaf(__half2float(enc(v, n2)), 0.5f, e); }
+    static constexpr bool kStoreUsesN2 = true;
 };
 
 struct CodecBF16Dev {
@@ -119,22 +124,22 @@ struct CodecBF16Dev {
     using R = float;
     static constexpr bool kExact = false;
     template <int I>
-    __device__ static R ld(const T* p) {
-        return fmaf(__bfloat162float(*p), 1.0f / float(1 << kDevShift), weight<float>(I));
+    __device__ static R ld(const T* p, float e) { return __bfloat162float(*p) + e; }
+    template <int I>
+    __device__ static void st(T* p, R v, float e) {
+        store_pop(reinterpret_cast<unsigned short*>(p), __bfloat16_as_ushort(__float2bfloat16(v - e)));
     }
     template <int I>
-    __device__ static void st(T* p, R v) {
-        const __hip_bfloat16 h =
-            __float2bfloat16(fmaf(v, float(1 << kDevShift), -weight<float>(I) * float(1 << kDevShift)));
-        store_pop(reinterpret_cast<unsigned short*>(p), __bfloat16_as_ushort(h));
-    }
-    template <int I>
-    __device__ static R rt(R v) {
-        return fmaf(__bfloat162float(__float2bfloat16(fmaf(v, float(1 << kDevShift),
-                                                           -weight<float>(I) * float(1 << kDevShift)))),
-                    1.0f / float(1 << kDevShift), weight<float>(I));
-    }
+    __device__ static R rt(R v, float e, float) { return __bfloat162float(__float2bfloat16(v - e)) + e; }
+    static constexpr bool kStoreUsesN2 = false;
 };
+
+// The per-direction constant a codec's store takes: -2 e_i for FP16, e_i otherwise.
+template <class Codec>
+__device__ __forceinline__ float store_key(const Center& ctr, int i) {
+    if constexpr (Codec::kStoreUsesN2) return ctr.n2[center_class(i)];
+    else return ctr.c[center_class(i)];
+}
 
 // ---------------------------------------------------------------- helpers
 // cx*a + cy*b + cz*c for lattice coefficients in {-1, 0, 1}, summing the
@@ -194,6 +199,7 @@ struct StepArgs {
     int tiles;     // x-tiles per row (ceil(nx / blockDim.x))
     long long N;   // cells per population array, px * ny * nz
     const R* __restrict__ uin;  // inlet velocity per row (ny*nz)
+    Center ctr;                 // per-direction centre of the 16-bit storage formats
     int bc_out_fused;           // outlet row inside the step ((nx-1) % 64 != 0)
     int write_all;
 };
@@ -358,7 +364,7 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
                 const int s = c - kCX[i] - px * kCY[i] - plane * kCZ[i];
-                f[i] = Codec::template ld<i>(fs + (long long)i * N + s);
+                f[i] = Codec::template ld<i>(fs + (long long)i * N + s, a.ctr.c[center_class(i)]);
             });
         } else {
             static_for<Q>([&](auto I) {
@@ -379,8 +385,12 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
                 } else {
                     s = c - cx - px * cy - plane * cz;
                 }
-                // i, OPP[i] and MIRZ[i] share one lattice weight, so one decoder serves all three.
-                f[i] = Codec::template ld<i>(fs + (long long)dir * N + s);
+                    // decode with the centre of the direction actually read
+                // (the mirror keeps c_x, so only the opposite direction has another centre)
+                constexpr int ki = center_class(i), ko = center_class(kOPP[i]);
+                const float e_i = a.ctr.c[ki];
+                const float e = (ki != ko && dir == kOPP[i]) ? a.ctr.c[ko] : e_i;
+                f[i] = Codec::template ld<i>(fs + (long long)dir * N + s, e);
             });
         }
 
@@ -540,7 +550,7 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
         if (in_wave) {
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
-                v[i] = __shfl_down(Codec::template rt<i>(f[i]), 1);
+                v[i] = __shfl_down(Codec::template rt<i>(f[i], a.ctr.c[center_class(i)], a.ctr.n2[center_class(i)]), 1);
             });
             const bool nb = __shfl_down(int(active), 1) != 0;
             if (x == 0 && active && nb) boundary_row<R>(f, v, 0, a.uin[row], R(1));
@@ -548,7 +558,7 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
         if (out_wave) {
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
-                v[i] = __shfl_up(Codec::template rt<i>(f[i]), 1);
+                v[i] = __shfl_up(Codec::template rt<i>(f[i], a.ctr.c[center_class(i)], a.ctr.n2[center_class(i)]), 1);
             });
             const bool nb = __shfl_up(int(active), 1) != 0;
             if (x == a.nx - 1 && active && nb) boundary_row<R>(f, v, 1, R(0), R(1));
@@ -559,7 +569,7 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
         typename Codec::T* const out = fd + c;
         static_for<Q>([&](auto I) {
             constexpr int i = I;
-            Codec::template st<i>(out + (long long)i * N, f[i]);
+            Codec::template st<i>(out + (long long)i * N, f[i], store_key<Codec>(a.ctr, i));
         });
     }
 }
@@ -617,7 +627,7 @@ __device__ __forceinline__ typename Codec::R feq_generic(int i, typename Codec::
 template <class Codec>
 __global__ void __launch_bounds__(256) k_bc(typename Codec::T* __restrict__ fd, const uint8_t* __restrict__ flag,
                                             const typename Codec::R* __restrict__ uin, int nx, int px, int rows,
-                                            long long N, typename Codec::R rho_out, int sides) {
+                                            long long N, typename Codec::R rho_out, int sides, Center ctr) {
     using R = typename Codec::R;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= rows) return;
@@ -631,7 +641,7 @@ __global__ void __launch_bounds__(256) k_bc(typename Codec::T* __restrict__ fd, 
         R r = R(0), mx = R(0), my = R(0), mz = R(0);
         static_for<Q>([&](auto I) {
             constexpr int i = I;
-            v[i] = Codec::template ld<i>(fd + (long long)i * N + c1);
+            v[i] = Codec::template ld<i>(fd + (long long)i * N + c1, ctr.c[center_class(i)]);
             r += v[i];
             mx += v[i] * R(kCX[i]);
             my += v[i] * R(kCY[i]);
@@ -643,7 +653,8 @@ __global__ void __launch_bounds__(256) k_bc(typename Codec::T* __restrict__ fd, 
             constexpr int i = I;
             const R e = side == 0 ? feq_generic<Codec>(i, r, uj, R(0), R(0))
                                   : feq_generic<Codec>(i, rho_out, ux, uy, uz);
-            Codec::template st<i>(fd + (long long)i * N + c0, e + (v[i] - feq_generic<Codec>(i, r, ux, uy, uz)));
+            Codec::template st<i>(fd + (long long)i * N + c0, e + (v[i] - feq_generic<Codec>(i, r, ux, uy, uz)),
+                                  store_key<Codec>(ctr, i));
         });
     }
 }

@@ -157,6 +157,7 @@ struct Solver final : wfs_solver {
     int cur = 0;
     R fscale = R(1);
     int bx = 128;
+    Center ctr{};   // storage centres (16-bit formats), fixed at upload
 
     T* fp(int k) { return static_cast<T*>(f[k].p); }
     R* rp(DevBuf& b) { return static_cast<R*>(b.p); }
@@ -247,6 +248,7 @@ struct Solver final : wfs_solver {
             return rc;
         }
         if ((rc = put_rows(uin, uin_h))) return rc;
+        set_center(uin_h);
         // the dense solid mask stays in the staging buffer while the flags are built
         WFS_HIP(hipMemcpyAsync(stage.p, solid, (size_t)N, hipMemcpyHostToDevice, stream));
         const dim3 gx(blocks_for(px, 64), d.ny, d.nz);
@@ -255,7 +257,7 @@ struct Solver final : wfs_solver {
         WFS_LAUNCH_CHECK();
         for (int k = 0; k < 2; ++k) {
             hipLaunchKernelGGL(k_init<Codec>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, fp(k), flags(),
-                               rp(uin), px, Np);
+                               rp(uin), px, Np, ctr);
             WFS_LAUNCH_CHECK();
         }
         hipLaunchKernelGGL(k_fill_macro<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, rp(rho), rp(ux),
@@ -271,6 +273,30 @@ struct Solver final : wfs_solver {
         have_bedwall = false;
         uploaded = true;
         return WFS_OK;
+    }
+
+    // Centre of the 16-bit formats: the equilibrium of the mean inlet velocity U0
+    // (mean of the positive inlet entries), in double, rounded to float.
+    void set_center(const float* uin_h) {
+        double sum = 0.0;
+        long long n = 0;
+        for (int j = 0; j < rows(); ++j)
+            if (uin_h[j] > 0.f) {
+                sum += uin_h[j];
+                ++n;
+            }
+#ifdef WFS_CENTER_AT_WEIGHTS
+        const double u0 = 0.0;
+#else
+        const double u0 = Codec::kExact || n == 0 ? 0.0 : sum / (double)n;
+#endif
+        for (int i = 0; i < Q; ++i) {
+            const double w = i == 0 ? 1.0 / 3 : (i < 7 ? 1.0 / 18 : 1.0 / 36);
+            const double cu = kCX[i] * u0;
+            const int k = center_class(i);
+            ctr.c[k] = (float)(w * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u0 * u0));
+            ctr.n2[k] = -2.0f * ctr.c[k];
+        }
     }
 
     int set_force(const float* a, const float* b, const float* c) override {
@@ -334,6 +360,7 @@ struct Solver final : wfs_solver {
         a.uin = rp(uin);
         a.bc_out_fused = (d.nx - 1) % 64 != 0 ? 1 : 0;
         a.write_all = 0;
+        a.ctr = ctr;
     }
 
     int step(int n) override {
@@ -356,7 +383,7 @@ struct Solver final : wfs_solver {
             }
             if (!a.bc_out_fused) {   // x = nx-1 opens a wavefront: outlet row in its own pass
                 hipLaunchKernelGGL(k_bc<Codec>, dim3(blocks_for(a.rows, 64)), dim3(64), 0, stream, fp(cur ^ 1),
-                                   flags(), rp(uin), d.nx, px, a.rows, Np, R(1), 2);
+                                   flags(), rp(uin), d.nx, px, a.rows, Np, R(1), 2, ctr);
                 WFS_LAUNCH_CHECK();
             }
             cur ^= 1;
@@ -393,10 +420,10 @@ struct Solver final : wfs_solver {
         }
         if (f64)
             hipLaunchKernelGGL((k_decode<Codec, double>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
-                               static_cast<double*>(tmp.p), d.nx, px, Np, N);
+                               static_cast<double*>(tmp.p), d.nx, px, Np, N, ctr);
         else
             hipLaunchKernelGGL((k_decode<Codec, float>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
-                               static_cast<float*>(tmp.p), d.nx, px, Np, N);
+                               static_cast<float*>(tmp.p), d.nx, px, Np, N, ctr);
         WFS_LAUNCH_CHECK();
         WFS_HIP(hipMemcpyAsync(out, tmp.p, n * esz, hipMemcpyDeviceToHost, stream));
         WFS_HIP(hipStreamSynchronize(stream));
@@ -413,7 +440,7 @@ struct Solver final : wfs_solver {
         }
         WFS_HIP(hipMemcpyAsync(tmp.p, in, n * sizeof(float), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(k_encode<Codec>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
-                           static_cast<const float*>(tmp.p), d.nx, px, Np, N);
+                           static_cast<const float*>(tmp.p), d.nx, px, Np, N, ctr);
         WFS_LAUNCH_CHECK();
         WFS_HIP(hipStreamSynchronize(stream));
         return WFS_OK;
@@ -634,7 +661,7 @@ struct Batch final : BatchBase {
                 if ((c->d.nx - 1) % 64 != 0) continue;   // outlet row fused into the step
                 hipLaunchKernelGGL(k_bc<Codec>, dim3(blocks_for(c->rows(), 64)), dim3(64), 0, stream,
                                    args[k].f[par ^ 1], c->flags(), c->rp(c->uin), c->d.nx, c->px, c->rows(), c->Np,
-                                   R(1), 2);
+                                   R(1), 2, c->ctr);
                 WFS_LAUNCH_CHECK();
             }
             par ^= 1;

@@ -31,7 +31,7 @@ def channel128(dt=2.5e-3, with_boat=True):
     return R.build_reach(sec, 4.0, 150.0, 2.2222e-4, fleet, cfg, dt=dt)
 
 
-def trapezoid_tug(with_boat=True, length=150.0, dx=0.1, x_first=40.0):
+def trapezoid_tug(with_boat=True, length=150.0, dx=0.1, x_first=40.0, run_flows=1.0, avg_frac=0.4, dt=None):
     """Trapezoidal reach with one pusher tug (bow at x_first); Q from the
     Manning rule at h = 3 m on the E3 slope; reference time-step rules."""
     sec = Section(B=40.0, Hb=3.0, z=2.0, Bf=0.0, n=0.03)
@@ -39,5 +39,5 @@ def trapezoid_tug(with_boat=True, length=150.0, dx=0.1, x_first=40.0):
     Q = float(q_ref(h, 1.0, E3_SLOPE, sec))
     tug = replace(R.boat_type("tug"), hull_L=12.0, hull_B=4.0, draft=1.5)
     fleet = R.Fleet(tug, rows=1, per_row=1, x_first=x_first, enabled=with_boat)
-    cfg = R.ReachConfig(length=length, dx=dx)
-    return R.build_reach(sec, h, Q, E3_SLOPE, fleet, cfg)
+    cfg = R.ReachConfig(length=length, dx=dx, run_flows=run_flows, avg_frac=avg_frac)
+    return R.build_reach(sec, h, Q, E3_SLOPE, fleet, cfg, dt=dt)
