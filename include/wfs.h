@@ -194,6 +194,67 @@ int wfs_bandwidth_probe(int32_t device, int64_t bytes, int32_t iters, double* gb
 /* Device memory held by the solver, in bytes. */
 int64_t wfs_device_bytes(const wfs_solver* h);
 
+/* ---- Learned sub-grid closure.
+ * No counterpart in the reference, whose only closure is the constant-Cs
+ * Smagorinsky model of floodsim/lbm3d.py:120-123 (kept as the default and the
+ * fallback). A network 8 -> 32 -> 32 -> 1 (ReLU, softplus output) predicts per
+ * cell the coefficient C^2 = WFS_C2_MIN + WFS_C2_SCALE * softplus(y) that
+ * replaces Cs^2 in tau = (tau0 + sqrt(tau0^2 + 18 sqrt(2) C^2 Q / rho)) / 2.
+ * Its inputs are strain-rate invariants from the non-equilibrium moments, the
+ * velocity relative to U0, the wall distance, the height in the column and
+ * the column depth (formulas: water_flow_simulation_amd/csrc/closure.h).
+ * Parameters are WFS_MLP_PARAMS floats in the order
+ *   W1[32][8] (out, in), b1[32], W2[32][32] (out, in), b2[32], w3[32], b3.
+ * Hidden layers run as BF16 MFMA with FP32 accumulation inside the step. */
+#define WFS_MLP_FEATURES 8
+#define WFS_MLP_HIDDEN 32
+#define WFS_MLP_PARAMS 1377
+#define WFS_C2_MIN 0.002
+#define WFS_C2_SCALE 0.0289
+enum wfs_closure { WFS_CLOSURE_SMAGORINSKY = 0, WFS_CLOSURE_MLP = 1 };
+
+/* Selects the closure. kind WFS_CLOSURE_SMAGORINSKY ignores the other
+ * arguments. kind WFS_CLOSURE_MLP (not for FP64 storage) takes the weights,
+ * the per-cell geometry code geo (nz, ny, nx) uint16: round(255/d) |
+ * round(255 z_rel) << 8, the per-column fluid-cell count hcol (ny, nx) uint8,
+ * and u0 > 0, the mean inlet speed in lattice units. */
+int wfs_set_closure(wfs_solver* h, int32_t kind, const float* params, const uint16_t* geo, const uint8_t* hcol,
+                    double u0);
+/* C^2 of every cell as used in the last step of the previous wfs_step call
+ * (learned closure; dense (nz, ny, nx) float). */
+int wfs_get_closure_field(wfs_solver* h, float* c2);
+
+/* Training samples from a fine run (its state after the last wfs_step call):
+ * the velocity is box-filtered 2x2x2 onto the coarse grid ncx x ncy x ncz
+ * (coarse cell (X,Y,Z) covers fine cells 2X..2X+1, 2Y..2Y+1, 2Z..2Z+1) and
+ * scaled by kv into coarse lattice units; geo/hcol (host) describe the coarse
+ * grid as in wfs_set_closure and u0 is the coarse mean inlet speed. Outputs
+ * are DEVICE arrays in dense coarse order: X (nc x 8) features, ABP (nc x 3)
+ * loss coefficients A = |S|^2 S:S, B = |S| (T:S), P = T:T (T the deviatoric
+ * subfilter stress, S the resolved strain by central differences), valid (nc)
+ * 1 where the cell and its face neighbours are fluid. Blocks until done. */
+int wfs_closure_samples(wfs_solver* fine, const uint16_t* geo, const uint8_t* hcol, int32_t ncx, int32_t ncy,
+                        int32_t ncz, double kv, double u0, float* X, float* ABP, uint8_t* valid);
+
+/* Training and evaluation kernels. All pointers are DEVICE pointers on
+ * `device`; work is a scratch array of at least wfs_mlp_work_floats(n)
+ * floats; stream is a hipStream_t (NULL: the null stream); calls are
+ * asynchronous on it. */
+int64_t wfs_mlp_work_floats(int64_t n);
+/* c2[k] = C^2 of the features X[k][0..7] through the in-step MFMA path. */
+int wfs_mlp_eval(int32_t device, const float* X, int64_t n, const float* params, float* c2, float* work,
+                 int64_t work_floats, void* stream);
+/* Loss sum(P + 4 C^2 B + 4 C^4 A) over the n samples and its gradient with
+ * respect to params, divided by sum(P): grad (WFS_MLP_PARAMS floats),
+ * sums[0] = loss sum, sums[1] = sum P (double). The forward pass rounds like
+ * the MFMA path (BF16 inputs, W1, W2 and layer-1 activations); the gradient
+ * treats the rounding as the identity. Deterministic. */
+int wfs_mlp_grad(int32_t device, const float* X, const float* ABP, int64_t n, const float* params, float* grad,
+                 double* sums, float* work, int64_t work_floats, void* stream);
+/* One bias-corrected Adam step (t >= 1) on params with moments m, v. */
+int wfs_mlp_adam(int32_t device, float* params, const float* grad, float* m, float* v, int64_t t, double lr,
+                 double beta1, double beta2, double eps, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,11 +66,12 @@ struct Lbm {
                      Real* rho_o, Real* ux_o, Real* uy_o, Real* uz_o,
                      const Real* cfw, const Real* uref,
                      int nx, int ny, int nz, int64_t N, int x, int y, int z,
-                     Acc tau0, Acc csmag2) {
+                     Acc tau0, Acc csmag2, const Acc* cs2f, Acc* mom) {
         const Real* w = W().w;
         const int64_t c = x + (int64_t)nx * (y + (int64_t)ny * z);
         if (solid[c]) {  // lbm3d.py:44
             rho_o[c] = Real(1); ux_o[c] = Real(0); uy_o[c] = Real(0); uz_o[c] = Real(0);
+            if (mom) for (int k = 0; k < 10; ++k) mom[(int64_t)k * N + c] = Acc(k == 0 ? 1 : 0);
             return;
         }
         Real f[19];
@@ -143,9 +144,17 @@ struct Lbm {
             pxx += (CX[i] * CX[i]) * Acc(fn); pyy += (CY[i] * CY[i]) * Acc(fn); pzz += (CZ[i] * CZ[i]) * Acc(fn);
             pxy += (CX[i] * CY[i]) * Acc(fn); pxz += (CX[i] * CZ[i]) * Acc(fn); pyz += (CY[i] * CZ[i]) * Acc(fn);
         }
-        // Smagorinsky relaxation time (lbm3d.py:120-123).
+        // Optional export of the collision's moments (checks of the learned closure):
+        // rho, u (with the half-force shift), Pi_neq xx yy zz xy xz yz.
+        if (mom) {
+            const Acc v[10] = {r, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz};
+            for (int k = 0; k < 10; ++k) mom[(int64_t)k * N + c] = v[k];
+        }
+        // Smagorinsky relaxation time (lbm3d.py:120-123); with cs2f, a per-cell
+        // coefficient in place of Cs^2 (the learned closure's C^2 field).
+        const Acc cs2 = cs2f ? cs2f[c] : csmag2;
         const Acc qn = std::sqrt(pxx * pxx + pyy * pyy + pzz * pzz + Acc(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
-        const Acc tau = Acc(0.5) * (tau0 + std::sqrt(tau0 * tau0 + Acc(18) * Acc(1.41421356) * csmag2 * qn / r));
+        const Acc tau = Acc(0.5) * (tau0 + std::sqrt(tau0 * tau0 + Acc(18) * Acc(1.41421356) * cs2 * qn / r));
         const Acc om = Acc(1) / tau;
         const Acc sf = Acc(1) - Acc(0.5) * om;
         // Regularized post-collision state with Guo source (lbm3d.py:124-136).
@@ -167,14 +176,15 @@ struct Lbm {
     static void step(const Real* fs, Real* fd, const uint8_t* solid,
                      const Real* Fx, const Real* Fy, const Real* Fz,
                      Real* rho_o, Real* ux_o, Real* uy_o, Real* uz_o,
-                     const Real* cfw, const Real* uref, int nx, int ny, int nz, Acc tau0, Acc csmag2) {
+                     const Real* cfw, const Real* uref, int nx, int ny, int nz, Acc tau0, Acc csmag2,
+                     const Acc* cs2f = nullptr, Acc* mom = nullptr) {
         const int64_t N = (int64_t)nx * ny * nz;
 #pragma omp parallel for collapse(2) schedule(static)
         for (int z = 0; z < nz; ++z)
             for (int y = 0; y < ny; ++y)
                 for (int x = 0; x < nx; ++x)
                     cell(fs, fd, solid, Fx, Fy, Fz, rho_o, ux_o, uy_o, uz_o, cfw, uref,
-                         nx, ny, nz, N, x, y, z, tau0, csmag2);
+                         nx, ny, nz, N, x, y, z, tau0, csmag2, cs2f, mom);
     }
 
     // lbm_bc (lbm3d.py:145-175; Numba lbm3d.py:301-328): inlet velocity and outlet
@@ -242,6 +252,17 @@ int orc_step_f64(const double* fs, double* fd, const uint8_t* solid, const doubl
                  const double* Fz, double* rho, double* ux, double* uy, double* uz, const double* cfw,
                  const double* uref, int nx, int ny, int nz, double tau0, double csmag2) {
     Lbm<double, double>::step(fs, fd, solid, Fx, Fy, Fz, rho, ux, uy, uz, cfw, uref, nx, ny, nz, tau0, csmag2);
+    return 0;
+}
+
+// FP64 step with a per-cell coefficient field cs2f (NULL: csmag2 everywhere) and
+// an optional export mom[10][N] of each cell's collision moments.
+int orc_step_field_f64(const double* fs, double* fd, const uint8_t* solid, const double* Fx, const double* Fy,
+                       const double* Fz, double* rho, double* ux, double* uy, double* uz, const double* cfw,
+                       const double* uref, int nx, int ny, int nz, double tau0, double csmag2, const double* cs2f,
+                       double* mom) {
+    Lbm<double, double>::step(fs, fd, solid, Fx, Fy, Fz, rho, ux, uy, uz, cfw, uref, nx, ny, nz, tau0, csmag2,
+                              cs2f, mom);
     return 0;
 }
 

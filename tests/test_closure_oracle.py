@@ -1,0 +1,234 @@
+"""Pins of the learned-closure oracle (oracle/closure_oracle.py) against
+checks that do not restate its formulas: torch's own bfloat16 rounding, numpy
+matrix products, central finite differences, the full-tensor form of the
+loss, brute-force loops, a linear velocity field's closed form, brute-force
+distances for the geometry codes, and the constant-coefficient special case
+that must reproduce the plain Smagorinsky step of the LBM oracle."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import closure_oracle as C
+from oracle import oracle as O
+from tests.helpers import channel128, solver_inputs
+from water_flow_simulation_amd import _native as nat
+from water_flow_simulation_amd import closure as CL
+
+
+def _sym(rng, n, traceless=True):
+    S = rng.normal(size=(n, 6))
+    return C.deviatoric(S) if traceless else S
+
+
+def _full(V):
+    M = np.empty(V.shape[:-1] + (3, 3))
+    M[..., 0, 0], M[..., 1, 1], M[..., 2, 2] = V[..., 0], V[..., 1], V[..., 2]
+    M[..., 0, 1] = M[..., 1, 0] = V[..., 3]
+    M[..., 0, 2] = M[..., 2, 0] = V[..., 4]
+    M[..., 1, 2] = M[..., 2, 1] = V[..., 5]
+    return M
+
+
+def test_constants_agree_with_library_header():
+    assert C.NPARAMS == nat.MLP_PARAMS == CL.NPARAMS == 1377
+    assert C.NF == nat.MLP_FEATURES
+    assert abs(C.C2_MIN - nat.C2_MIN) < 1e-9 and abs(C.C2_SCALE - nat.C2_SCALE) < 1e-9
+    text = open(nat.HEADER).read()
+    assert "#define WFS_MLP_PARAMS 1377" in text and "#define WFS_C2_MIN 0.002" in text
+
+
+def test_bf16_rounding_matches_torch():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.normal(size=20000).astype(np.float32) * 10.0 ** rng.integers(-6, 4, 20000),
+                        # exact ties between two bfloat16 values, both parities
+                        (np.arange(1, 2001, dtype=np.float32) * 2 ** -7 + 2 ** -16).astype(np.float32)])
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    np.testing.assert_array_equal(C.bf16(x), ref)
+
+
+def test_invariants_against_matrix_products():
+    rng = np.random.default_rng(2)
+    S = _sym(rng, 500)
+    M = _full(S)
+    np.testing.assert_allclose(C.trace_s3(S), np.trace(M @ M @ M, axis1=-2, axis2=-1), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(C.ddot(S, S), np.einsum("...ij,...ij->...", M, M), rtol=1e-13)
+    np.testing.assert_allclose(np.trace(_full(C.deviatoric(_sym(rng, 50, False))), axis1=-2, axis2=-1), 0,
+                               atol=1e-13)
+    u = rng.normal(size=(500, 3))
+    X = C.features(S, u, np.full(500, 255 | (128 << 8), np.uint16), np.full(500, 32, np.uint8), 2.0)
+    smag = np.sqrt(2 * np.einsum("...ij,...ij->...", M, M))
+    np.testing.assert_allclose(X[:, 0], 0.25 * np.log(smag + 1e-7) + 1.5, rtol=1e-12)
+    assert np.all(np.abs(X[:, 1]) <= 1.0) and np.all((X[:, 3] >= 0) & (X[:, 3] <= 1.0 + 1e-12))
+    np.testing.assert_allclose(X[:, 2], np.linalg.norm(u, axis=1) * 2 - 1, rtol=1e-12)
+    np.testing.assert_allclose(X[:, 5:], np.tile([1.0, 128 / 255, 0.5], (500, 1)))
+    # closed forms of the shape invariant: axisymmetric extension = +1, compression = -1, pure shear = 0
+    ax = np.array([[2, -1, -1, 0, 0, 0], [-2, 1, 1, 0, 0, 0], [1, -1, 0, 0, 0, 0], [0, 0, 0, 1, 0, 0.]])
+    X = C.features(ax * 1e-3, np.zeros((4, 3)), np.zeros(4, np.uint16), np.zeros(4, np.uint8), 1.0)
+    np.testing.assert_allclose(X[:, 1], [1, -1, 0, 0], atol=1e-12)
+    np.testing.assert_allclose(X[:, 3], [0, 0, 0, 0], atol=1e-12)
+    X = C.features(np.array([[0, 0, 0, 0, 1e-3, 0.]]), np.zeros((1, 3)), np.zeros(1, np.uint16),
+                   np.zeros(1, np.uint8), 1.0)
+    assert X[0, 3] == pytest.approx(1.0)
+
+
+def test_loss_coefficients_match_tensor_form():
+    rng = np.random.default_rng(3)
+    S, T = _sym(rng, 300) * 1e-3, _sym(rng, 300) * 1e-6
+    c2 = rng.uniform(0, 0.1, 300)
+    ss = C.ddot(S, S)
+    smag = np.sqrt(2 * ss)
+    A, B, P = smag * smag * ss, smag * C.ddot(T, S), C.ddot(T, T)
+    np.testing.assert_allclose(P + 4 * c2 * B + 4 * c2 * c2 * A, C.stress_loss(c2, S, T), rtol=1e-10)
+    # the best constant minimises the summed loss
+    best = -B.sum() / (2 * A.sum())
+    tot = lambda c: C.stress_loss(np.full(300, c), S, T).sum()  # noqa: E731
+    if best > 0:
+        assert tot(best) <= min(tot(best * 0.99), tot(best * 1.01))
+
+
+def _random_batch(rng, n):
+    X = np.column_stack([rng.uniform(-1, 1.5, n), rng.uniform(-1, 1, n), rng.normal(0, 0.3, n),
+                         rng.uniform(0, 1, n), rng.normal(0, 0.1, n), rng.uniform(0, 1, n), rng.uniform(0, 1, n),
+                         rng.uniform(0.2, 1, n)])
+    S, T = _sym(rng, n) * 1e-3, _sym(rng, n) * 1e-6
+    ss = C.ddot(S, S)
+    smag = np.sqrt(2 * ss)
+    ABP = np.column_stack([smag * smag * ss, smag * C.ddot(T, S), C.ddot(T, T)])
+    return X, ABP
+
+
+def test_gradient_against_finite_differences():
+    rng = np.random.default_rng(4)
+    X, ABP = _random_batch(rng, 64)
+    p = CL.init_params(7).astype(np.float64)
+    p[C.OFF_B1:C.OFF_W2] = rng.normal(0, 0.3, C.NH)
+    p[C.OFF_B2:C.OFF_W3] = rng.normal(0, 0.3, C.NH)
+    p[C.OFF_W3:C.OFF_B3] = rng.normal(0, 0.3, C.NH)
+    _, psum, g = C.loss_grad(p, X, ABP, quantize=False)
+    idx = np.concatenate([rng.choice(C.NPARAMS - 1, 80, replace=False), [C.NPARAMS - 1]])
+    h = 1e-6
+    for k in idx:
+        a, b = p.copy(), p.copy()
+        a[k] += h
+        b[k] -= h
+        fd = (C.loss_grad(a, X, ABP, False)[0] - C.loss_grad(b, X, ABP, False)[0]) / (2 * h) / psum
+        assert g[k] == pytest.approx(fd, rel=1e-5, abs=1e-9 * max(1.0, np.abs(g).max())), k
+
+
+def test_box_filter_brute_force():
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=(3, 4, 6, 8))
+    solid = rng.random((4, 6, 8)) < 0.1
+    m, T, fluid = C.coarsen(u, solid, 4, 3, 2, 1.5)
+    pairs = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+    for Z in range(2):
+        for Y in range(3):
+            for X in range(4):
+                cells = [(2 * Z + a, 2 * Y + b, 2 * X + c) for a in (0, 1) for b in (0, 1) for c in (0, 1)]
+                fl = not any(solid[c] for c in cells)
+                assert fluid[Z, Y, X] == fl
+                if not fl:
+                    continue
+                v = np.array([[1.5 * u[d][c] for d in range(3)] for c in cells])
+                mean = v.mean(0)
+                np.testing.assert_allclose(m[:, Z, Y, X], mean, rtol=1e-13)
+                for k, (i, j) in enumerate(pairs):
+                    ref = sum((w[i] - mean[i]) * (w[j] - mean[j]) for w in v) / 8
+                    assert T[Z, Y, X, k] == pytest.approx(ref, rel=1e-12, abs=1e-15)
+
+
+def test_samples_of_a_linear_field():
+    # u_i = G_ij x_j (x in fine cells): filtered strain = 2 sym(G) exactly, and
+    # the subfilter stress of a linear field over a 2x2x2 block is G G^T / 4.
+    rng = np.random.default_rng(6)
+    G = rng.normal(size=(3, 3)) * 1e-3
+    nz, ny, nx = 8, 10, 12
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    pos = np.stack([x, y, z]).astype(np.float64)
+    u = np.einsum("ij,j...->i...", G, pos) + np.array([0.05, 0, 0])[:, None, None, None]
+    solid = np.zeros((nz, ny, nx), bool)
+    geo = np.zeros((4, 5, 6), np.uint16)
+    hcol = np.full((5, 6), 4, np.uint8)
+    X, ABP, valid = C.samples(u, solid, 6, 5, 4, 1.0, geo, hcol, 20.0)
+    valid = valid.reshape(4, 5, 6)
+    assert valid[1:, 1:-1, 1:-1].all() and not valid[0].any() and not valid[:, 0].any() and not valid[..., -1].any()
+    S = C.deviatoric(np.array([[2 * G[0, 0], 2 * G[1, 1], 2 * G[2, 2], G[0, 1] + G[1, 0], G[0, 2] + G[2, 0],
+                                G[1, 2] + G[2, 1]]]))
+    TT = (G @ G.T) / 4
+    T = C.deviatoric(np.array([[TT[0, 0], TT[1, 1], TT[2, 2], TT[0, 1], TT[0, 2], TT[1, 2]]]))
+    ss = C.ddot(S, S)[0]
+    smag = np.sqrt(2 * ss)
+    want = [smag * smag * ss, smag * C.ddot(T, S)[0], C.ddot(T, T)[0]]
+    inner = ABP.reshape(4, 5, 6, 3)[1:3, 1:-1, 1:-1]   # below the top layer (the lid mirror is not linear)
+    np.testing.assert_allclose(inner.reshape(-1, 3), np.tile(want, (inner.size // 3, 1)), rtol=1e-9)
+
+
+def test_geometry_codes_brute_force():
+    rng = np.random.default_rng(8)
+    nz, ny, nx = 5, 7, 6
+    solid = np.zeros((nz, ny, nx), bool)
+    solid[0, :, :] = rng.random((ny, nx)) < 0.5
+    solid[:, 0, :] = True
+    solid[3:, 3, 2:4] = True          # a hull
+    geo, hcol = CL.geometry(solid)
+    walls = [(z, y, x) for z in range(nz) for y in range(ny) for x in range(nx) if solid[z, y, x]]
+    walls += [(-1, y, x) for y in range(-1, ny + 1) for x in range(nx)]
+    walls += [(z, y, x) for z in range(-1, nz) for y in (-1, ny) for x in range(nx)]
+    W = np.array(walls, float)
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                if solid[z, y, x]:
+                    continue
+                d = np.sqrt(((W - [z, y, x]) ** 2).sum(1)).min()
+                assert geo[z, y, x] & 255 == min(255, int(np.rint(255 / max(d, 1.0))))
+    for y in range(ny):
+        for x in range(nx):
+            col = ~solid[:, y, x]
+            assert hcol[y, x] == col.sum()
+            if col.any():
+                zb = int(np.argmax(col))
+                for z in np.nonzero(col)[0]:
+                    zr = min(max((z - zb + 0.5) / col.sum(), 0.0), 1.0)
+                    assert geo[z, y, x] >> 8 == int(np.rint(255 * zr))
+
+
+def test_smagorinsky_weights_give_cs():
+    y, _ = C.forward(CL.smagorinsky_params(0.17), np.random.default_rng(9).normal(size=(10, 8)))
+    np.testing.assert_allclose(C.c2_of(y), 0.17 ** 2, rtol=1e-6)
+
+
+def test_constant_network_step_is_the_smagorinsky_step():
+    """The learned-closure oracle step with the constant network equals the
+    FP64 oracle's plain step (the per-cell field path, features and network
+    plumbing leave the constant-coefficient model unchanged)."""
+    r = channel128()
+    kw = solver_inputs(r)
+    a = O.OracleLBM(precision="f64", **kw)
+    b = O.OracleLBM(precision="f64", **kw)
+    a.step(30)
+    b.step(30)
+    geo, hcol = CL.geometry(r.solid)
+    u0 = float(kw["uin"][kw["uin"] > 0].mean())
+    p = CL.smagorinsky_params(r.cfg.cs)
+    # the plain step with exactly the network's constant (cs^2 up to the float32 output bias)
+    a.csmag2 = float(C.c2_of(np.float64(p[-1])))
+    assert a.csmag2 == pytest.approx(r.cfg.cs ** 2, rel=1e-6)
+    for _ in range(3):
+        a.step(1)
+        c2 = C.learned_step(b, p, geo, hcol, 1.0 / u0)
+    fl = ~r.solid.ravel()
+    np.testing.assert_array_equal(c2[fl], a.csmag2)
+    np.testing.assert_array_equal(b.f, a.f)
+    # a planted mistake (features of the wrong cell) changes a real network's field
+    p2 = CL.init_params(3)
+    b2 = O.OracleLBM(precision="f64", **kw)
+    b2.step(30)
+    mom = C.moments(b2)
+    S, u = C.strain_from_moments(mom, b2.tau0, b2.csmag2)
+    X = C.features(S, u, geo.ravel(), np.broadcast_to(hcol[None], r.solid.shape).ravel(), 1 / u0)
+    y_ok, _ = C.forward(p2, X)
+    y_bad, _ = C.forward(p2, np.roll(X, 1, axis=0))
+    assert np.abs(C.c2_of(y_ok) - C.c2_of(y_bad))[fl].max() > 1e-4

@@ -82,7 +82,22 @@ _SIGS = {
     "wfs_batch_time": ([_P, _I32, _P, _P], ctypes.c_int),
     "wfs_bandwidth_probe": ([_I32, ctypes.c_int64, _I32, _P], ctypes.c_int),
     "wfs_device_bytes": ([_P], ctypes.c_int64),
+    "wfs_set_closure": ([_P, _I32, _P, _P, _P, ctypes.c_double], ctypes.c_int),
+    "wfs_get_closure_field": ([_P, _P], ctypes.c_int),
+    "wfs_closure_samples": ([_P, _P, _P, _I32, _I32, _I32, ctypes.c_double, ctypes.c_double, _P, _P, _P],
+                            ctypes.c_int),
+    "wfs_mlp_work_floats": ([ctypes.c_int64], ctypes.c_int64),
+    "wfs_mlp_eval": ([_I32, _P, ctypes.c_int64, _P, _P, _P, ctypes.c_int64, _P], ctypes.c_int),
+    "wfs_mlp_grad": ([_I32, _P, _P, ctypes.c_int64, _P, _P, _P, _P, ctypes.c_int64, _P], ctypes.c_int),
+    "wfs_mlp_adam": ([_I32, _P, _P, _P, _P, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                      ctypes.c_double, _P], ctypes.c_int),
 }
+
+CLOSURE = {"smagorinsky": 0, "mlp": 1}
+MLP_PARAMS = 1377
+MLP_FEATURES = 8
+C2_MIN = 0.002
+C2_SCALE = 0.0289
 
 _lib = None
 

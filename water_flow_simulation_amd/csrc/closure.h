@@ -1,0 +1,452 @@
+// Learned sub-grid closure: a local Smagorinsky coefficient C^2 predicted per
+// cell by an 8-32-32-1 network.
+//
+// The model. With the resolved deviatoric strain rate S (lattice units, filter
+// width one cell) and |S| = sqrt(2 S:S), the eddy viscosity is
+//     nu_t = C^2 |S|,   C^2 = kC2Min + kC2Scale * softplus(y(x)),
+// y the network output for the feature vector x of the cell. Inside the LBM
+// step nu_t enters exactly like the constant-coefficient Smagorinsky model:
+//     tau = (tau0 + sqrt(tau0^2 + 18 sqrt(2) C^2 Q / rho)) / 2,
+// Q = |Pi_neq|, so C^2 = Cs^2 everywhere is plain Smagorinsky.
+//
+// Features (all cells, lattice units):
+//   x0 = ln(|S| + 1e-7) / 4 + 1.5        strain-rate magnitude
+//   x1 = sqrt(6) tr(S^3) / (S:S)^1.5     strain shape, in [-1, 1]
+//   x2 = |u| / U0 - 1                    speed relative to the mean inlet speed U0
+//   x3 = 2 (Sxz^2 + Syz^2) / S:S         share of vertical shear, in [0, 1]
+//   x4 = u_z / U0                        vertical velocity
+//   x5 = 1 / d                           d: distance to the nearest wall cell centre (cells)
+//   x6 = z_rel                           height above the bed over the column depth
+//   x7 = h / 64                          h: fluid cells in the column
+// x5 and x6 are stored per cell as two bytes (value * 255, rounded), h per
+// column as one byte. Inputs and the two hidden layers' weights are BF16 with
+// FP32 accumulation; the output layer is FP32.
+//
+// Training fits C^2 to box-filtered fine-grid runs in stress space: for a
+// sample with subfilter stress tau (deviatoric part T) and resolved strain S,
+//     loss = |T + 2 C^2 |S| S|^2 = P + 4 C^2 B + 4 C^4 A,
+//     A = |S|^2 S:S,  B = |S| (T:S),  P = T:T,
+// so a sample carries (A, B, P) besides its features.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace wfs {
+
+constexpr int kNF = 8;    // features
+constexpr int kNH = 32;   // hidden width
+// Parameter vector, natural order: W1[32][8] (out, in), b1[32], W2[32][32] (out, in), b2[32], w3[32], b3.
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kNH * kNF;
+constexpr int kOffW2 = kOffB1 + kNH;
+constexpr int kOffB2 = kOffW2 + kNH * kNH;
+constexpr int kOffW3 = kOffB2 + kNH;
+constexpr int kOffB3 = kOffW3 + kNH;
+constexpr int kNParams = kOffB3 + 1;   // 1377
+
+constexpr float kC2Min = 0.002f;    // floor of C^2 (Cs ~ 0.045)
+constexpr float kC2Scale = 0.0289f; // C^2 per unit softplus (0.17^2)
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+
+// Weights in the register layout of the wavefront evaluation below.
+//
+// Both MFMA layers compute the transposed product out^T = W x^T, so the rows
+// of a 32x32 tile are units and the columns are 32 cells. For the 32x32 FP32
+// result, lane l holds column l % 32 and, in accumulator r = 0..15, the unit
+//     row(r, h) = 8 (r / 4) + 4 h + (r % 4),   h = l / 32.
+// A lane's 8 accumulators r = 8s..8s+7 are, after ReLU and rounding, its
+// B-operand slots for k-step s of the next 32x32x16 product; the matching
+// A-operand slots of W2 are packed in the same unit order, so no data moves
+// between lanes from layer 1 to layer 2.
+struct MlpPack {
+    uint32_t w1[64 * 2];       // lane l: BF16 W1[l%32][4h .. 4h+3] (32x32x8 A operand)
+    uint32_t w2[2 * 64 * 4];   // step s, lane l: BF16 W2[l%32][unit(s, h, 0..7)]
+    float b1[2 * 16];          // [h][r]: b1[row(r, h)], the layer-1 accumulator start
+    float b2[2 * 16];          // [h][r]: b2[row(r, h)]
+    float w3[2 * 16];          // [h][r]: w3[row(r, h)]
+    float b3;
+    float pad[3];
+};
+constexpr int kPackWords = (int)(sizeof(MlpPack) / 4);
+
+__host__ __device__ constexpr int mfma_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
+// Hidden unit in B slot j (0..7) of k-step s for lane half h.
+__host__ __device__ constexpr int l2_unit(int s, int h, int j) { return mfma_row(8 * s + j, h); }
+
+__device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
+    bf2v v = {(__bf16)a, (__bf16)b};
+    return __builtin_bit_cast(uint32_t, v);
+}
+__device__ __forceinline__ float bf16_round(float a) { return (float)(__bf16)a; }
+
+__device__ __forceinline__ float softplus_f(float y) {
+    return y > 15.f ? y : __logf(1.f + __expf(y));
+}
+
+// The eight features of one cell (formulas in the header comment).
+__device__ __forceinline__ void closure_features(float sxx, float syy, float szz, float sxy, float sxz, float syz,
+                                                 float ux, float uy, float uz, uint16_t geo, uint8_t hcol,
+                                                 float inv_u0, float* x) {
+    const float od = sxy * sxy + sxz * sxz + syz * syz;
+    const float ss = sxx * sxx + syy * syy + szz * szz + 2.f * od;
+    const float smag = sqrtf(2.f * ss);
+    x[0] = 0.25f * __logf(smag + 1e-7f) + 1.5f;
+    const float t3 = sxx * sxx * sxx + syy * syy * syy + szz * szz * szz + 3.f * sxy * sxy * (sxx + syy) +
+                     3.f * sxz * sxz * (sxx + szz) + 3.f * syz * syz * (syy + szz) + 6.f * sxy * sxz * syz;
+    const bool live = ss > 1e-20f;
+    x[1] = live ? fminf(fmaxf(2.4494897f * t3 / (ss * sqrtf(ss)), -1.f), 1.f) : 0.f;
+    x[2] = sqrtf(ux * ux + uy * uy + uz * uz) * inv_u0 - 1.f;
+    x[3] = live ? 2.f * (sxz * sxz + syz * syz) / ss : 0.f;
+    x[4] = uz * inv_u0;
+    x[5] = float(geo & 255u) * (1.f / 255.f);
+    x[6] = float(geo >> 8) * (1.f / 255.f);
+    x[7] = float(hcol) * (1.f / 64.f);
+}
+
+// Network output y for the cell of this lane. All 64 lanes of the wavefront
+// must call this together (it exchanges data between lanes and runs MFMA).
+// P points to the pack in LDS.
+__device__ __forceinline__ float mlp_wave(const float* x, const MlpPack* P) {
+    const int lane = int(threadIdx.x) & 63;
+    const int h = lane >> 5;
+    const uint32_t xp0 = pk_bf16(x[0], x[1]), xp1 = pk_bf16(x[2], x[3]);
+    const uint32_t xp2 = pk_bf16(x[4], x[5]), xp3 = pk_bf16(x[6], x[7]);
+    // Tile b holds cells 32b .. 32b+31 of the wavefront; lane l supplies
+    // features 4h .. 4h+3 of cell 32b + l%32. Each lane sends the half it does
+    // not use itself to its partner lane l ^ 32.
+    const uint32_t r0 = __shfl_xor(h ? xp0 : xp2, 32);
+    const uint32_t r1 = __shfl_xor(h ? xp1 : xp3, 32);
+    const u2v own = h ? u2v{xp2, xp3} : u2v{xp0, xp1};
+    const u2v other = u2v{r0, r1};
+    const s4v a1 = __builtin_bit_cast(s4v, u2v{P->w1[2 * lane], P->w1[2 * lane + 1]});
+    const bf8v a2s0 = __builtin_bit_cast(bf8v, *reinterpret_cast<const u4v*>(&P->w2[(0 * 64 + lane) * 4]));
+    const bf8v a2s1 = __builtin_bit_cast(bf8v, *reinterpret_cast<const u4v*>(&P->w2[(1 * 64 + lane) * 4]));
+    float yb[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        const u2v xb = (b == h) ? own : other;
+        f16v acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = P->b1[h * 16 + r];
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, __builtin_bit_cast(s4v, xb), acc, 0, 0, 0);
+        u4v hq0, hq1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            hq0[q] = pk_bf16(fmaxf(acc[2 * q], 0.f), fmaxf(acc[2 * q + 1], 0.f));
+            hq1[q] = pk_bf16(fmaxf(acc[8 + 2 * q], 0.f), fmaxf(acc[8 + 2 * q + 1], 0.f));
+        }
+        f16v acc2;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc2[r] = P->b2[h * 16 + r];
+        acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2s0, __builtin_bit_cast(bf8v, hq0), acc2, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2s1, __builtin_bit_cast(bf8v, hq1), acc2, 0, 0, 0);
+        float part = 0.f;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) part = fmaf(P->w3[h * 16 + r], fmaxf(acc2[r], 0.f), part);
+        yb[b] = part + __shfl_xor(part, 32) + P->b3;
+    }
+    return h ? yb[1] : yb[0];
+}
+
+// Copies the pack (global) into LDS; every thread of the block calls it.
+__device__ __forceinline__ void stage_pack(const MlpPack* __restrict__ g, MlpPack* s) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(g);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(s);
+    for (int k = int(threadIdx.x); k < kPackWords; k += int(blockDim.x)) dst[k] = src[k];
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- kernels
+
+// Natural parameter vector -> pack. One block of 256 threads.
+__global__ void __launch_bounds__(256) k_mlp_pack(const float* __restrict__ p, MlpPack* __restrict__ out) {
+    const int t = int(threadIdx.x);
+    if (t < 64) {
+        const int i = t & 31, h = t >> 5;
+        const float* w = p + kOffW1 + i * kNF + 4 * h;
+        out->w1[2 * t] = pk_bf16(w[0], w[1]);
+        out->w1[2 * t + 1] = pk_bf16(w[2], w[3]);
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float* w2 = p + kOffW2 + i * kNH;
+                out->w2[(s * 64 + t) * 4 + q] = pk_bf16(w2[l2_unit(s, h, 2 * q)], w2[l2_unit(s, h, 2 * q + 1)]);
+            }
+    } else if (t < 96) {
+        const int h = (t - 64) >> 4, r = (t - 64) & 15;
+        const int u = mfma_row(r, h);
+        out->b1[h * 16 + r] = p[kOffB1 + u];
+        out->b2[h * 16 + r] = p[kOffB2 + u];
+        out->w3[h * 16 + r] = p[kOffW3 + u];
+    } else if (t == 96) {
+        out->b3 = p[kOffB3];
+        out->pad[0] = out->pad[1] = out->pad[2] = 0.f;
+    }
+}
+
+// C^2 of n feature vectors (row-major n x 8) through the wavefront MFMA path.
+__global__ void __launch_bounds__(256) k_mlp_eval(const float* __restrict__ X, long long n,
+                                                  const MlpPack* __restrict__ pack, float* __restrict__ c2) {
+    __shared__ MlpPack P;
+    stage_pack(pack, &P);
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    float x[kNF];
+#pragma unroll
+    for (int j = 0; j < kNF; ++j) x[j] = k < n ? X[k * kNF + j] : 0.f;
+    const float y = mlp_wave(x, &P);
+    if (k < n) c2[k] = kC2Min + kC2Scale * softplus_f(y);
+}
+
+// Training: loss and gradient of a batch, one sample per thread, FP32 with
+// the forward pass rounded like the MFMA path (BF16 inputs, W1, W2 and
+// layer-1 activations); the gradient treats the rounding as the identity.
+// Each block writes its partial gradient and loss sums (kNParams + 2 floats)
+// to part[blockIdx.x]; k_mlp_reduce adds the blocks in a fixed order.
+constexpr int kGradBlock = 64;
+constexpr int kPartStride = kNParams + 3;   // grad, sum loss, sum P, pad
+
+__global__ void __launch_bounds__(kGradBlock) k_mlp_grad(const float* __restrict__ X, const float* __restrict__ ABP,
+                                                          long long n, const float* __restrict__ p,
+                                                          float* __restrict__ part) {
+    __shared__ float W[kNParams];
+    __shared__ float sh1[kNH][kGradBlock];   // layer-1 activations (rounded)
+    __shared__ float sd1[kNH][kGradBlock];   // layer-1 deltas
+    __shared__ float sh2[kNH][kGradBlock];   // layer-2 activations
+    __shared__ float sd2[kNH][kGradBlock];   // layer-2 deltas
+    __shared__ float sx[kNF][kGradBlock];    // inputs (rounded)
+    __shared__ float sdy[kGradBlock];
+    __shared__ float sl[2][kGradBlock];
+    const int t = int(threadIdx.x);
+    for (int k = t; k < kNParams; k += kGradBlock) {
+        const bool q = (k < kOffB1) || (k >= kOffW2 && k < kOffB2);
+        W[k] = q ? bf16_round(p[k]) : p[k];
+    }
+    __syncthreads();
+    const long long s = (long long)blockIdx.x * kGradBlock + t;
+    const bool ok = s < n;
+    float x[kNF];
+#pragma unroll
+    for (int j = 0; j < kNF; ++j) x[j] = ok ? bf16_round(X[s * kNF + j]) : 0.f;
+    float z1[kNH], h1[kNH], z2[kNH];
+#pragma unroll
+    for (int i = 0; i < kNH; ++i) {
+        float a = W[kOffB1 + i];
+#pragma unroll
+        for (int j = 0; j < kNF; ++j) a = fmaf(W[kOffW1 + i * kNF + j], x[j], a);
+        z1[i] = a;
+        h1[i] = bf16_round(fmaxf(a, 0.f));
+    }
+    float y = W[kOffB3];
+#pragma unroll
+    for (int i = 0; i < kNH; ++i) {
+        float a = W[kOffB2 + i];
+#pragma unroll
+        for (int j = 0; j < kNH; ++j) a = fmaf(W[kOffW2 + i * kNH + j], h1[j], a);
+        z2[i] = a;
+        y = fmaf(W[kOffW3 + i], fmaxf(a, 0.f), y);
+    }
+    float dy = 0.f, loss = 0.f, pp = 0.f;
+    if (ok) {
+        const float A = ABP[s * 3], B = ABP[s * 3 + 1], P = ABP[s * 3 + 2];
+        const float sp = y > 15.f ? y : log1pf(expf(y));
+        const float sg = 1.f / (1.f + expf(-y));
+        const float c2 = kC2Min + kC2Scale * sp;
+        loss = P + 4.f * c2 * B + 4.f * c2 * c2 * A;
+        pp = P;
+        dy = (4.f * B + 8.f * c2 * A) * kC2Scale * sg;
+    }
+    float d2[kNH];
+#pragma unroll
+    for (int i = 0; i < kNH; ++i) d2[i] = z2[i] > 0.f ? dy * W[kOffW3 + i] : 0.f;
+#pragma unroll
+    for (int j = 0; j < kNH; ++j) {
+        float a = 0.f;
+#pragma unroll
+        for (int i = 0; i < kNH; ++i) a = fmaf(W[kOffW2 + i * kNH + j], d2[i], a);
+        sd1[j][t] = z1[j] > 0.f ? a : 0.f;
+        sh1[j][t] = h1[j];
+        sh2[j][t] = fmaxf(z2[j], 0.f);
+        sd2[j][t] = d2[j];
+    }
+#pragma unroll
+    for (int j = 0; j < kNF; ++j) sx[j][t] = x[j];
+    sdy[t] = dy;
+    sl[0][t] = loss;
+    sl[1][t] = pp;
+    __syncthreads();
+    float* out = part + (long long)blockIdx.x * kPartStride;
+    for (int k = t; k < kNParams + 2; k += kGradBlock) {
+        float g = 0.f;
+        if (k < kOffB1) {
+            const int i = k / kNF, j = k % kNF;
+            for (int m = 0; m < kGradBlock; ++m) g = fmaf(sd1[i][m], sx[j][m], g);
+        } else if (k < kOffW2) {
+            const int i = k - kOffB1;
+            for (int m = 0; m < kGradBlock; ++m) g += sd1[i][m];
+        } else if (k < kOffB2) {
+            const int i = (k - kOffW2) / kNH, j = (k - kOffW2) % kNH;
+            for (int m = 0; m < kGradBlock; ++m) g = fmaf(sd2[i][m], sh1[j][m], g);
+        } else if (k < kOffW3) {
+            const int i = k - kOffB2;
+            for (int m = 0; m < kGradBlock; ++m) g += sd2[i][m];
+        } else if (k < kOffB3) {
+            const int i = k - kOffW3;
+            for (int m = 0; m < kGradBlock; ++m) g = fmaf(sdy[m], sh2[i][m], g);
+        } else if (k == kOffB3) {
+            for (int m = 0; m < kGradBlock; ++m) g += sdy[m];
+        } else {
+            const int q = k - kNParams;
+            for (int m = 0; m < kGradBlock; ++m) g += sl[q][m];
+        }
+        out[k] = g;
+    }
+}
+
+// Sum of the block partials in block order (FP64). grad gets the mean over
+// n samples divided by the mean P of the batch (the relative stress error's
+// gradient); sums[0..1] get the loss and P sums.
+__global__ void __launch_bounds__(256) k_mlp_reduce(const float* __restrict__ part, int blocks, long long n,
+                                                    float* __restrict__ grad, double* __restrict__ sums) {
+    __shared__ double psum;
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int b = 0; b < blocks; ++b) a += (double)part[(long long)b * kPartStride + kNParams + 1];
+        psum = a;
+    }
+    __syncthreads();
+    const double norm = psum > 0.0 ? 1.0 / psum : 0.0;
+    for (int k = int(threadIdx.x); k < kNParams + 2; k += int(blockDim.x)) {
+        double a = 0.0;
+        for (int b = 0; b < blocks; ++b) a += (double)part[(long long)b * kPartStride + k];
+        if (k < kNParams) grad[k] = (float)(a * norm);
+        else sums[k - kNParams] = a;
+    }
+}
+
+// Adam step (bias-corrected), t >= 1.
+__global__ void __launch_bounds__(256) k_adam(float* __restrict__ p, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v, int n, float lr,
+                                              float b1, float b2, float eps, float c1, float c2) {
+    const int k = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= n) return;
+    const float gk = g[k];
+    const float mk = b1 * m[k] + (1.f - b1) * gk;
+    const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    p[k] -= lr * (mk * c1) / (sqrtf(vk * c2) + eps);
+}
+
+// ---------------------------------------------------------------- training samples
+// A fine run is box-filtered 2x2x2 onto the coarse grid (coarse cell (X,Y,Z)
+// covers fine cells 2X..2X+1, 2Y..2Y+1, 2Z..2Z+1). Velocities are scaled by
+// kv into coarse lattice units.
+
+// Pass 1: filtered velocity and subfilter stress mean((u - ubar)(u - ubar)).
+template <typename R>
+__global__ void __launch_bounds__(256) k_coarsen(const R* __restrict__ ux, const R* __restrict__ uy,
+                                                 const R* __restrict__ uz, const uint8_t* __restrict__ flag,
+                                                 int pxf, int nyf, int ncx, int ncy, int ncz, float kv,
+                                                 float* __restrict__ ubar, float* __restrict__ tsub,
+                                                 uint8_t* __restrict__ cfluid) {
+    const long long nc = (long long)ncx * ncy * ncz;
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int X = int(c % ncx), Y = int((c / ncx) % ncy), Z = int(c / ((long long)ncx * ncy));
+    float u[8][3];
+    bool fluid = true;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const long long f = (2 * X + (k & 1)) + (long long)pxf * ((2 * Y + ((k >> 1) & 1)) +
+                                                                   (long long)nyf * (2 * Z + (k >> 2)));
+        fluid = fluid && !(flag[f] & 1u);
+        u[k][0] = kv * float(ux[f]);
+        u[k][1] = kv * float(uy[f]);
+        u[k][2] = kv * float(uz[f]);
+    }
+    float m[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) m[d] += u[k][d];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) m[d] *= 0.125f;
+    float t[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float a = u[k][0] - m[0], b = u[k][1] - m[1], e = u[k][2] - m[2];
+        t[0] += a * a; t[1] += b * b; t[2] += e * e; t[3] += a * b; t[4] += a * e; t[5] += b * e;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ubar[d * nc + c] = fluid ? m[d] : 0.f;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) tsub[d * nc + c] = fluid ? t[d] * 0.125f : 0.f;
+    cfluid[c] = fluid ? 1 : 0;
+}
+
+// Pass 2: resolved strain by central differences of the filtered velocity
+// (the rigid lid mirrors the cell below it: ghost (ux, uy, -uz)), then the
+// features and the loss coefficients. A sample is valid when the cell and its
+// six face neighbours are fluid (or the lid above) and 0 < X < ncx-1.
+__global__ void __launch_bounds__(256) k_samples(const float* __restrict__ ubar, const float* __restrict__ tsub,
+                                                 const uint8_t* __restrict__ cfluid, const uint16_t* __restrict__ geo,
+                                                 const uint8_t* __restrict__ hcol, int ncx, int ncy, int ncz,
+                                                 float inv_u0, float* __restrict__ X, float* __restrict__ ABP,
+                                                 uint8_t* __restrict__ valid) {
+    const long long nc = (long long)ncx * ncy * ncz;
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int x = int(c % ncx), y = int((c / ncx) % ncy), z = int(c / ((long long)ncx * ncy));
+    const long long sx = 1, sy = ncx, sz = (long long)ncx * ncy;
+    bool ok = cfluid[c] && x > 0 && x < ncx - 1 && y > 0 && y < ncy - 1 && z > 0;
+    if (ok) ok = cfluid[c - sx] && cfluid[c + sx] && cfluid[c - sy] && cfluid[c + sy] && cfluid[c - sz] &&
+                 (z == ncz - 1 || cfluid[c + sz]);
+    valid[c] = ok ? 1 : 0;
+    float feat[kNF] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float A = 0.f, B = 0.f, P = 0.f;
+    if (ok) {
+        float g[3][3];   // g[i][j] = d u_i / d x_j
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float* u = ubar + i * nc;
+            g[i][0] = 0.5f * (u[c + sx] - u[c - sx]);
+            g[i][1] = 0.5f * (u[c + sy] - u[c - sy]);
+            const float up = z == ncz - 1 ? (i == 2 ? -u[c] : u[c]) : u[c + sz];
+            g[i][2] = 0.5f * (up - u[c - sz]);
+        }
+        const float tr = (g[0][0] + g[1][1] + g[2][2]) * (1.f / 3.f);
+        const float sxx = g[0][0] - tr, syy = g[1][1] - tr, szz = g[2][2] - tr;
+        const float sxy = 0.5f * (g[0][1] + g[1][0]), sxz = 0.5f * (g[0][2] + g[2][0]),
+                    syz = 0.5f * (g[1][2] + g[2][1]);
+        const float* t = tsub + c;
+        const float tt = (t[0] + t[nc] + t[2 * nc]) * (1.f / 3.f);
+        const float Txx = t[0] - tt, Tyy = t[nc] - tt, Tzz = t[2 * nc] - tt;
+        const float Txy = t[3 * nc], Txz = t[4 * nc], Tyz = t[5 * nc];
+        const float ss = sxx * sxx + syy * syy + szz * szz + 2.f * (sxy * sxy + sxz * sxz + syz * syz);
+        const float smag = sqrtf(2.f * ss);
+        const float ts = Txx * sxx + Tyy * syy + Tzz * szz + 2.f * (Txy * sxy + Txz * sxz + Tyz * syz);
+        A = smag * smag * ss;
+        B = smag * ts;
+        P = Txx * Txx + Tyy * Tyy + Tzz * Tzz + 2.f * (Txy * Txy + Txz * Txz + Tyz * Tyz);
+        closure_features(sxx, syy, szz, sxy, sxz, syz, ubar[c], ubar[nc + c], ubar[2 * nc + c], geo[c],
+                         hcol[x + (long long)ncx * y], inv_u0, feat);
+#pragma unroll
+        for (int j = 0; j < kNF; ++j) feat[j] = bf16_round(feat[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kNF; ++j) X[c * kNF + j] = feat[j];
+    ABP[c * 3] = A;
+    ABP[c * 3 + 1] = B;
+    ABP[c * 3 + 2] = P;
+}
+
+}  // namespace wfs

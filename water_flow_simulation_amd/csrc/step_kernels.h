@@ -38,6 +38,7 @@
 #include <hip/hip_bf16.h>
 
 #include "lattice.h"
+#include "closure.h"
 
 namespace wfs {
 
@@ -202,6 +203,12 @@ struct StepArgs {
     Center ctr;                 // per-direction centre of the 16-bit storage formats
     int bc_out_fused;           // outlet row inside the step ((nx-1) % 64 != 0)
     int write_all;
+    // learned closure (k_step_learned only)
+    const uint16_t* __restrict__ geo;   // per cell: 255/d | 255 z_rel << 8
+    const uint8_t* __restrict__ hcol;   // per column (x + px*y): fluid cells in the column
+    const MlpPack* __restrict__ mlp;    // network weights in the wavefront layout
+    float* __restrict__ c2out;          // C^2 per cell on write_all steps, when not null
+    float inv_u0;                       // 1 / mean inlet speed (lattice units)
 };
 
 // Work-group -> (row, x-tile) map. Work-groups are dispatched round-robin over
@@ -258,9 +265,16 @@ __device__ __forceinline__ void boundary_row(R* f, const R* v, int side, R uin, 
 // second moment of the equilibrium, so no per-direction equilibrium is formed
 // for the stress. Same mathematics as the per-direction form; different
 // rounding (FMAs, one reciprocal).
+//
+// pair_moments: density, velocity (with the Guo half-force shift), the
+// non-equilibrium stress and its norm Q; pair_relax: the post-collision state.
 template <typename R>
-__device__ __forceinline__ void collide_pairs(R* f, bool forced, R fx, R fy, R fz, R tau0, R tau0sq, R ksmag,
-                                              R& r, R& ux, R& uy, R& uz) {
+struct PairMoments {
+    R r, ri, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz, qn;
+};
+
+template <typename R>
+__device__ __forceinline__ void pair_moments(const R* f, bool forced, R fx, R fy, R fz, PairMoments<R>& m) {
     // pair sums and differences (i, i+1) for i = 1, 3, ..., 17
     R sp[9], dp[9];
 #pragma unroll
@@ -269,25 +283,36 @@ __device__ __forceinline__ void collide_pairs(R* f, bool forced, R fx, R fy, R f
         dp[k] = f[2 * k + 1] - f[2 * k + 2];
     }
     // pairs: 0:x 1:y 2:z 3:(1,1,0) 4:(1,-1,0) 5:(1,0,1) 6:(1,0,-1) 7:(0,1,1) 8:(0,1,-1)
-    r = f[0] + ((sp[0] + sp[1] + sp[2]) + (sp[3] + sp[4] + sp[5]) + (sp[6] + sp[7] + sp[8]));
+    const R r = f[0] + ((sp[0] + sp[1] + sp[2]) + (sp[3] + sp[4] + sp[5]) + (sp[6] + sp[7] + sp[8]));
     const R mx = dp[0] + dp[3] + dp[4] + dp[5] + dp[6];
     const R my = dp[1] + dp[3] - dp[4] + dp[7] + dp[8];
     const R mz = dp[2] + dp[5] - dp[6] + dp[7] - dp[8];
     const R ri = R(1) / r;
+    R ux, uy, uz;
     if (forced) {
         ux = fma(R(0.5), fx, mx) * ri; uy = fma(R(0.5), fy, my) * ri; uz = fma(R(0.5), fz, mz) * ri;
     } else {
         ux = mx * ri; uy = my * ri; uz = mz * ri;
     }
     const R rux = r * ux, ruy = r * uy, ruz = r * uz, r3 = r * (R(1) / R(3));
-    const R pxx = (sp[0] + sp[3] + sp[4] + sp[5] + sp[6]) - fma(rux, ux, r3);
-    const R pyy = (sp[1] + sp[3] + sp[4] + sp[7] + sp[8]) - fma(ruy, uy, r3);
-    const R pzz = (sp[2] + sp[5] + sp[6] + sp[7] + sp[8]) - fma(ruz, uz, r3);
-    const R pxy = (sp[3] - sp[4]) - rux * uy;
-    const R pxz = (sp[5] - sp[6]) - rux * uz;
-    const R pyz = (sp[7] - sp[8]) - ruy * uz;
-    const R qn = sqrt(fma(pxx, pxx, fma(pyy, pyy, pzz * pzz)) + R(2) * fma(pxy, pxy, fma(pxz, pxz, pyz * pyz)));
-    const R tau = R(0.5) * (tau0 + sqrt(fma(ksmag * qn, ri, tau0sq)));
+    m.pxx = (sp[0] + sp[3] + sp[4] + sp[5] + sp[6]) - fma(rux, ux, r3);
+    m.pyy = (sp[1] + sp[3] + sp[4] + sp[7] + sp[8]) - fma(ruy, uy, r3);
+    m.pzz = (sp[2] + sp[5] + sp[6] + sp[7] + sp[8]) - fma(ruz, uz, r3);
+    m.pxy = (sp[3] - sp[4]) - rux * uy;
+    m.pxz = (sp[5] - sp[6]) - rux * uz;
+    m.pyz = (sp[7] - sp[8]) - ruy * uz;
+    m.qn = sqrt(fma(m.pxx, m.pxx, fma(m.pyy, m.pyy, m.pzz * m.pzz)) +
+                R(2) * fma(m.pxy, m.pxy, fma(m.pxz, m.pxz, m.pyz * m.pyz)));
+    m.r = r; m.ri = ri; m.ux = ux; m.uy = uy; m.uz = uz;
+}
+
+// Relaxation of the pair form with eddy-viscosity constant ksmag = 18 sqrt(2) C^2.
+template <typename R>
+__device__ __forceinline__ void pair_relax(R* f, bool forced, R fx, R fy, R fz, const PairMoments<R>& m, R tau0,
+                                           R tau0sq, R ksmag) {
+    const R r = m.r, ux = m.ux, uy = m.uy, uz = m.uz;
+    const R pxx = m.pxx, pyy = m.pyy, pzz = m.pzz, pxy = m.pxy, pxz = m.pxz, pyz = m.pyz;
+    const R tau = R(0.5) * (tau0 + sqrt(fma(ksmag * m.qn, m.ri, tau0sq)));
     const R om = R(1) / tau;
     const R om1 = R(1) - om;
     const R t3 = (pxx + pyy + pzz) * (R(1) / R(3));
@@ -333,9 +358,158 @@ __device__ __forceinline__ void collide_pairs(R* f, bool forced, R fx, R fy, R f
     pair(8, uy - uz, fma(R(-2), pyz, syz), fy - fz, wr2, g2, R(1) / R(36), h2);
 }
 
+template <typename R>
+__device__ __forceinline__ void collide_pairs(R* f, bool forced, R fx, R fy, R fz, R tau0, R tau0sq, R ksmag,
+                                              R& r, R& ux, R& uy, R& uz) {
+    PairMoments<R> m;
+    pair_moments<R>(f, forced, fx, fy, fz, m);
+    pair_relax<R>(f, forced, fx, fy, fz, m, tau0, tau0sq, ksmag);
+    r = m.r; ux = m.ux; uy = m.uy; uz = m.uz;
+}
+
 #ifndef WFS_STEP_WAVES_PER_EU
 #define WFS_STEP_WAVES_PER_EU 1   // minimum resident waves per SIMD requested from the register allocator
 #endif
+
+// Pull streaming (x-edge own copy, rigid-lid mirror, bounce-back on wall
+// links) and the wall function for one active cell: f gets the populations
+// entering the collision.
+template <class Codec>
+__device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs, const StepArgs<typename Codec::R>& a,
+                                       int c, int x, int z, uint8_t fl, uint32_t wm, typename Codec::R* f) {
+    using R = typename Codec::R;
+    const int px = a.px;
+    const int plane = px * a.ny;
+    const long long N = a.N;
+    const bool xlo = x == 0, xhi = x == a.nx - 1, top = z == a.nz - 1;
+    // 1. pull streaming
+    if (!(fl & FLAG_NEARWALL) && !xlo && !xhi && !top) {
+        // interior: every source is the plain upstream neighbour
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            const int s = c - kCX[i] - px * kCY[i] - plane * kCZ[i];
+            f[i] = Codec::template ld<i>(fs + (long long)i * N + s, a.ctr.c[center_class(i)]);
+        });
+    } else {
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+            const bool xout = (cx == 1 && xlo) || (cx == -1 && xhi);
+            const bool wall = (wm >> i) & 1u;
+            int dir = i, s = c;
+            if (xout) {
+                dir = i;
+                s = c;
+            } else if (wall) {
+                dir = kOPP[i];
+                s = c;
+            } else if (cz == -1 && top) {
+                dir = kMIRZ[i];
+                s = c - cx - px * cy;
+            } else {
+                s = c - cx - px * cy - plane * cz;
+            }
+                // decode with the centre of the direction actually read
+            // (the mirror keeps c_x, so only the opposite direction has another centre)
+            constexpr int ki = center_class(i), ko = center_class(kOPP[i]);
+            const float e_i = a.ctr.c[ki];
+            const float e = (ki != ko && dir == kOPP[i]) ? a.ctr.c[ko] : e_i;
+            f[i] = Codec::template ld<i>(fs + (long long)dir * N + s, e);
+        });
+    }
+
+    // 2. wall function
+    if (wm != 0u && (fl & FLAG_CF)) {
+        const R cf = a.cfw[c];
+        int inx = 0, iny = 0, inz = 0;
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            if ((wm >> i) & 1u) { inx -= kCX[i]; iny -= kCY[i]; inz -= kCZ[i]; }
+        });
+        const R nwx = R(inx), nwy = R(iny), nwz = R(inz);
+        const R nn = sqrt(nwx * nwx + nwy * nwy + nwz * nwz);
+        const R afx = R(((wm >> 1) & 1u) + ((wm >> 2) & 1u));
+        const R afy = R(((wm >> 3) & 1u) + ((wm >> 4) & 1u));
+        const R afz = R(((wm >> 5) & 1u) + ((wm >> 6) & 1u));
+        const R area = sqrt(afx * afx + afy * afy + afz * afz);
+        if (nn > R(0) && area > R(0)) {
+            const R ex = nwx / nn, ey = nwy / nn, ez = nwz / nn;
+            const R ox = a.ux[c], oy = a.uy[c], oz = a.uz[c];   // previous step, own cell
+            const R un = ox * ex + oy * ey + oz * ez;
+            R tx = ox - un * ex, ty = oy - un * ey, tz = oz - un * ez;
+            const R ut = sqrt(tx * tx + ty * ty + tz * tz);
+            if (ut > R(1e-7)) {
+                tx /= ut; ty /= ut; tz /= ut;
+                const R rw = a.rho[c];
+                R D = R(0), K = R(0);
+                static_for<Q>([&](auto I) {
+                    constexpr int i = I;
+                    if ((wm >> i) & 1u) {
+                        const R ct = dot3<kCX[i], kCY[i], kCZ[i]>(tx, ty, tz);
+                        D += ct * f[i];
+                        K += weight<R>(i) * ct * ct;
+                    }
+                });
+                if (K > R(0)) {
+                    const R ur = a.uref[c];
+                    const R U = ur > R(0) ? ur : ut;
+                    const R us0 = (R(-2) * D - rw * cf * U * U * area) / (R(6) * rw * K);
+                    const R us = fmin(fmax(us0, -ut), ut);
+                    static_for<Q>([&](auto I) {
+                        constexpr int i = I;
+                        if ((wm >> i) & 1u) {
+                            constexpr R c6w = R(6) * weight<R>(i);
+                            f[i] = f[i] + c6w * rw * dot3<kCX[i], kCY[i], kCZ[i]>(tx, ty, tz) * us;
+                        }
+                    });
+                }
+            }
+        }
+    }
+}
+
+// Fused inlet/outlet rows and the stores of one tile (every lane of the
+// wavefront calls this; inactive lanes only take part in the shifts).
+template <class Codec>
+__device__ __forceinline__ void finish(typename Codec::T* __restrict__ fd, const StepArgs<typename Codec::R>& a,
+                                       typename Codec::R* f, int c, int x, int row, int tile, bool active) {
+    using R = typename Codec::R;
+    const long long N = a.N;
+    // Inlet and outlet rows, fused: the boundary lane takes its neighbour's
+    // post-collision populations (rounded through the storage format, as a
+    // separate pass would read them) by a cross-lane shift. The outlet is done
+    // here when x = nx-2 and nx-1 share a wavefront; otherwise by k_bc.
+    const int xw = tile * int(blockDim.x) + (int(threadIdx.x) & ~63);   // first x of this wavefront
+    const bool in_wave = xw == 0;
+    const bool out_wave = a.bc_out_fused && xw <= a.nx - 1 && a.nx - 1 < xw + 64;
+    if (in_wave || out_wave) {
+        R v[Q];
+        if (in_wave) {
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                v[i] = __shfl_down(Codec::template rt<i>(f[i], a.ctr.c[center_class(i)], a.ctr.n2[center_class(i)]), 1);
+            });
+            const bool nb = __shfl_down(int(active), 1) != 0;
+            if (x == 0 && active && nb) boundary_row<R>(f, v, 0, a.uin[row], R(1));
+        }
+        if (out_wave) {
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                v[i] = __shfl_up(Codec::template rt<i>(f[i], a.ctr.c[center_class(i)], a.ctr.n2[center_class(i)]), 1);
+            });
+            const bool nb = __shfl_up(int(active), 1) != 0;
+            if (x == a.nx - 1 && active && nb) boundary_row<R>(f, v, 1, R(0), R(1));
+        }
+    }
+
+    if (active) {
+        typename Codec::T* const out = fd + c;
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            Codec::template st<i>(out + (long long)i * N, f[i], store_key<Codec>(a.ctr, i));
+        });
+    }
+}
 
 // One x-tile of one row: the whole step for its cells.
 template <class Codec>
@@ -345,103 +519,16 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
     const int x = tile * int(blockDim.x) + int(threadIdx.x);
     const bool inrow = x < a.nx;
     const int z = row / a.ny;
-    const int px = a.px;
-    const int plane = px * a.ny;
-    const int c = x + px * row;
+    const int c = x + a.px * row;
     // Lanes outside the row or on solid cells do no work but stay in the wave
-    // for the boundary-row shuffles below.
+    // for the boundary-row shuffles.
     const uint8_t fl = inrow ? a.flag[c] : uint8_t(FLAG_SOLID);
     const bool active = !(fl & FLAG_SOLID);
-    const long long N = a.N;
     R f[Q];
 
     if (active) {
         const uint32_t wm = (fl & FLAG_NEARWALL) ? a.wmask[c] : 0u;
-        const bool xlo = x == 0, xhi = x == a.nx - 1, top = z == a.nz - 1;
-        // 1. pull streaming
-        if (!(fl & FLAG_NEARWALL) && !xlo && !xhi && !top) {
-            // interior: every source is the plain upstream neighbour
-            static_for<Q>([&](auto I) {
-                constexpr int i = I;
-                const int s = c - kCX[i] - px * kCY[i] - plane * kCZ[i];
-                f[i] = Codec::template ld<i>(fs + (long long)i * N + s, a.ctr.c[center_class(i)]);
-            });
-        } else {
-            static_for<Q>([&](auto I) {
-                constexpr int i = I;
-                constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
-                const bool xout = (cx == 1 && xlo) || (cx == -1 && xhi);
-                const bool wall = (wm >> i) & 1u;
-                int dir = i, s = c;
-                if (xout) {
-                    dir = i;
-                    s = c;
-                } else if (wall) {
-                    dir = kOPP[i];
-                    s = c;
-                } else if (cz == -1 && top) {
-                    dir = kMIRZ[i];
-                    s = c - cx - px * cy;
-                } else {
-                    s = c - cx - px * cy - plane * cz;
-                }
-                    // decode with the centre of the direction actually read
-                // (the mirror keeps c_x, so only the opposite direction has another centre)
-                constexpr int ki = center_class(i), ko = center_class(kOPP[i]);
-                const float e_i = a.ctr.c[ki];
-                const float e = (ki != ko && dir == kOPP[i]) ? a.ctr.c[ko] : e_i;
-                f[i] = Codec::template ld<i>(fs + (long long)dir * N + s, e);
-            });
-        }
-
-        // 2. wall function
-        if (wm != 0u && (fl & FLAG_CF)) {
-            const R cf = a.cfw[c];
-            int inx = 0, iny = 0, inz = 0;
-            static_for<Q>([&](auto I) {
-                constexpr int i = I;
-                if ((wm >> i) & 1u) { inx -= kCX[i]; iny -= kCY[i]; inz -= kCZ[i]; }
-            });
-            const R nwx = R(inx), nwy = R(iny), nwz = R(inz);
-            const R nn = sqrt(nwx * nwx + nwy * nwy + nwz * nwz);
-            const R afx = R(((wm >> 1) & 1u) + ((wm >> 2) & 1u));
-            const R afy = R(((wm >> 3) & 1u) + ((wm >> 4) & 1u));
-            const R afz = R(((wm >> 5) & 1u) + ((wm >> 6) & 1u));
-            const R area = sqrt(afx * afx + afy * afy + afz * afz);
-            if (nn > R(0) && area > R(0)) {
-                const R ex = nwx / nn, ey = nwy / nn, ez = nwz / nn;
-                const R ox = a.ux[c], oy = a.uy[c], oz = a.uz[c];   // previous step, own cell
-                const R un = ox * ex + oy * ey + oz * ez;
-                R tx = ox - un * ex, ty = oy - un * ey, tz = oz - un * ez;
-                const R ut = sqrt(tx * tx + ty * ty + tz * tz);
-                if (ut > R(1e-7)) {
-                    tx /= ut; ty /= ut; tz /= ut;
-                    const R rw = a.rho[c];
-                    R D = R(0), K = R(0);
-                    static_for<Q>([&](auto I) {
-                        constexpr int i = I;
-                        if ((wm >> i) & 1u) {
-                            const R ct = dot3<kCX[i], kCY[i], kCZ[i]>(tx, ty, tz);
-                            D += ct * f[i];
-                            K += weight<R>(i) * ct * ct;
-                        }
-                    });
-                    if (K > R(0)) {
-                        const R ur = a.uref[c];
-                        const R U = ur > R(0) ? ur : ut;
-                        const R us0 = (R(-2) * D - rw * cf * U * U * area) / (R(6) * rw * K);
-                        const R us = fmin(fmax(us0, -ut), ut);
-                        static_for<Q>([&](auto I) {
-                            constexpr int i = I;
-                            if ((wm >> i) & 1u) {
-                                constexpr R c6w = R(6) * weight<R>(i);
-                                f[i] = f[i] + c6w * rw * dot3<kCX[i], kCY[i], kCZ[i]>(tx, ty, tz) * us;
-                            }
-                        });
-                    }
-                }
-            }
-        }
+        gather<Codec>(fs, a, c, x, z, fl, wm, f);
 
         R r, ux, uy, uz;
         if constexpr (Codec::kExact) {
@@ -538,40 +625,51 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
         }
     }
 
-    // Inlet and outlet rows, fused: the boundary lane takes its neighbour's
-    // post-collision populations (rounded through the storage format, as a
-    // separate pass would read them) by a cross-lane shift. The outlet is done
-    // here when x = nx-2 and nx-1 share a wavefront; otherwise by k_bc.
-    const int xw = tile * int(blockDim.x) + (int(threadIdx.x) & ~63);   // first x of this wavefront
-    const bool in_wave = xw == 0;
-    const bool out_wave = a.bc_out_fused && xw <= a.nx - 1 && a.nx - 1 < xw + 64;
-    if (in_wave || out_wave) {
-        R v[Q];
-        if (in_wave) {
-            static_for<Q>([&](auto I) {
-                constexpr int i = I;
-                v[i] = __shfl_down(Codec::template rt<i>(f[i], a.ctr.c[center_class(i)], a.ctr.n2[center_class(i)]), 1);
-            });
-            const bool nb = __shfl_down(int(active), 1) != 0;
-            if (x == 0 && active && nb) boundary_row<R>(f, v, 0, a.uin[row], R(1));
-        }
-        if (out_wave) {
-            static_for<Q>([&](auto I) {
-                constexpr int i = I;
-                v[i] = __shfl_up(Codec::template rt<i>(f[i], a.ctr.c[center_class(i)], a.ctr.n2[center_class(i)]), 1);
-            });
-            const bool nb = __shfl_up(int(active), 1) != 0;
-            if (x == a.nx - 1 && active && nb) boundary_row<R>(f, v, 1, R(0), R(1));
-        }
-    }
+    finish<Codec>(fd, a, f, c, x, row, tile, active);
+}
 
+// The step with the learned closure (float arithmetic, pair-form collision).
+// The network runs on the whole wavefront between the moments and the
+// relaxation, so every lane reaches it; inactive lanes feed zero features.
+template <class Codec>
+__device__ __forceinline__ void step_tile_learned(const typename Codec::T* __restrict__ fs,
+                                                  typename Codec::T* __restrict__ fd, const StepArgs<float>& a,
+                                                  int row, int tile, const MlpPack* P) {
+    using R = float;
+    const int x = tile * int(blockDim.x) + int(threadIdx.x);
+    const bool inrow = x < a.nx;
+    const int z = row / a.ny;
+    const int c = x + a.px * row;
+    const uint8_t fl = inrow ? a.flag[c] : uint8_t(FLAG_SOLID);
+    const bool active = !(fl & FLAG_SOLID);
+    const bool forced = (fl & FLAG_FORCED) != 0;
+    R f[Q];
+    PairMoments<R> m;
+    R fx = R(0), fy = R(0), fz = R(0);
+    uint32_t wm = 0u;
+    float feat[kNF] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (active) {
-        typename Codec::T* const out = fd + c;
-        static_for<Q>([&](auto I) {
-            constexpr int i = I;
-            Codec::template st<i>(out + (long long)i * N, f[i], store_key<Codec>(a.ctr, i));
-        });
+        wm = (fl & FLAG_NEARWALL) ? a.wmask[c] : 0u;
+        gather<Codec>(fs, a, c, x, z, fl, wm, f);
+        if (forced) { fx = a.Fx[c] * a.fscale; fy = a.Fy[c] * a.fscale; fz = a.Fz[c] * a.fscale; }
+        pair_moments<R>(f, forced, fx, fy, fz, m);
+        // resolved strain rate S = -3 Pi_dev / (2 rho tau), tau from the constant-Cs model
+        const R tau_s = R(0.5) * (a.tau0 + sqrtf(fmaf(a.ksmag * m.qn, m.ri, a.tau0sq)));
+        const R k = R(-1.5) * m.ri / tau_s;
+        const R t3 = (m.pxx + m.pyy + m.pzz) * (R(1) / R(3));
+        closure_features(k * (m.pxx - t3), k * (m.pyy - t3), k * (m.pzz - t3), k * m.pxy, k * m.pxz, k * m.pyz,
+                         m.ux, m.uy, m.uz, a.geo[c], a.hcol[c - a.px * a.ny * z], a.inv_u0, feat);
     }
+    const float y = mlp_wave(feat, P);
+    if (active) {
+        const float c2 = kC2Min + kC2Scale * softplus_f(y);
+        pair_relax<R>(f, forced, fx, fy, fz, m, a.tau0, a.tau0sq, R(18) * R(1.41421356) * c2);
+        if (a.write_all || (wm != 0u && (fl & FLAG_CF))) {
+            a.rho[c] = m.r; a.ux[c] = m.ux; a.uy[c] = m.uy; a.uz[c] = m.uz;
+        }
+        if (a.write_all && a.c2out) a.c2out[c] = c2;
+    }
+    finish<Codec>(fd, a, f, c, x, row, tile, active);
 }
 
 template <class Codec>
@@ -580,6 +678,16 @@ k_step(const typename Codec::T* __restrict__ fs, typename Codec::T* __restrict__
     int row, tile;
     if (!tile_of(blockIdx.x, a.tiles, a.rows, row, tile)) return;   // whole work-group
     step_tile<Codec>(fs, fd, a, row, tile);
+}
+
+template <class Codec>
+__global__ void __launch_bounds__(256) k_step_learned(const typename Codec::T* __restrict__ fs,
+                                                      typename Codec::T* __restrict__ fd, StepArgs<float> a) {
+    __shared__ MlpPack P;
+    int row, tile;
+    if (!tile_of(blockIdx.x, a.tiles, a.rows, row, tile)) return;   // whole work-group
+    stage_pack(a.mlp, &P);
+    step_tile_learned<Codec>(fs, fd, a, row, tile, &P);
 }
 
 // ---------------------------------------------------------------- many reaches in one launch

@@ -13,6 +13,7 @@
 #include "wfs.h"
 #include "aux_kernels.h"
 #include "step_kernels.h"
+#include "closure.h"
 
 using namespace wfs;
 
@@ -132,6 +133,11 @@ struct wfs_solver {
     virtual int profile_x(int field, int z0, int z1, double* out) = 0;
     virtual int total(int field, double* out) = 0;
     virtual int check_finite() = 0;
+    virtual int set_closure(int kind, const float* params, const uint16_t* geo, const uint8_t* hcol, double u0) = 0;
+    virtual int get_closure_field(float* c2) = 0;
+    virtual int closure_samples(const uint16_t* geo, const uint8_t* hcol, int ncx, int ncy, int ncz, double kv,
+                                double u0, float* X, float* ABP, uint8_t* valid) = 0;
+    int closure = WFS_CLOSURE_SMAGORINSKY;
 
     int alloc_buf(DevBuf& b, size_t bytes_) {
         bytes_ = (bytes_ + 255) & ~size_t(255);  // whole 256-byte blocks: fills and vector access stay in bounds
@@ -154,6 +160,8 @@ struct Solver final : wfs_solver {
     using R = typename Codec::R;
     DevBuf f[2], flag, wmask, Fx, Fy, Fz, rho, ux, uy, uz, cfw, uref, uin, bedwall;
     DevBuf sums[6], partial, out_x, counter, stage;
+    DevBuf geo, hcol, mlp_params, mlp_pack, c2buf;   // learned closure (allocated when first set)
+    float inv_u0 = 0.f;
     int cur = 0;
     R fscale = R(1);
     int bx = 128;
@@ -361,6 +369,22 @@ struct Solver final : wfs_solver {
         a.bc_out_fused = (d.nx - 1) % 64 != 0 ? 1 : 0;
         a.write_all = 0;
         a.ctr = ctr;
+        a.geo = static_cast<const uint16_t*>(geo.p);
+        a.hcol = static_cast<const uint8_t*>(hcol.p);
+        a.mlp = static_cast<const MlpPack*>(mlp_pack.p);
+        a.c2out = static_cast<float*>(c2buf.p);
+        a.inv_u0 = inv_u0;
+    }
+
+    // One step launch: the constant-Cs kernel or, with the learned closure, k_step_learned.
+    void launch_step(const dim3& grid, const StepArgs<R>& a) {
+        if constexpr (std::is_same<R, float>::value) {
+            if (closure == WFS_CLOSURE_MLP) {
+                hipLaunchKernelGGL(k_step_learned<Codec>, grid, dim3(bx), 0, stream, fp(cur), fp(cur ^ 1), a);
+                return;
+            }
+        }
+        hipLaunchKernelGGL(k_step<Codec>, grid, dim3(bx), 0, stream, fp(cur), fp(cur ^ 1), a);
     }
 
     int step(int n) override {
@@ -375,7 +399,7 @@ struct Solver final : wfs_solver {
                 int rc = mark();
                 if (rc) return rc;
             }
-            hipLaunchKernelGGL(k_step<Codec>, grid, dim3(bx), 0, stream, fp(cur), fp(cur ^ 1), a);
+            launch_step(grid, a);
             WFS_LAUNCH_CHECK();
             if (timing) {
                 int rc = mark();
@@ -523,6 +547,87 @@ struct Solver final : wfs_solver {
         return WFS_OK;
     }
 
+    int set_closure(int kind, const float* params, const uint16_t* geo_h, const uint8_t* hcol_h,
+                    double u0) override {
+        if (kind == WFS_CLOSURE_SMAGORINSKY) {
+            closure = kind;
+            return WFS_OK;
+        }
+        if (kind != WFS_CLOSURE_MLP) return fail(WFS_EINVAL, "wfs_set_closure: unknown closure kind");
+        if constexpr (!std::is_same<R, float>::value) {
+            return fail(WFS_EINVAL, "wfs_set_closure: the learned closure runs in float arithmetic (not FP64 storage)");
+        } else {
+            if (!params || !geo_h || !hcol_h) return fail(WFS_EINVAL, "wfs_set_closure: null argument");
+            if (!(u0 > 0.0) || !std::isfinite(u0)) return fail(WFS_EINVAL, "wfs_set_closure: u0 must be positive");
+            for (int k = 0; k < kNParams; ++k)
+                if (!std::isfinite(params[k])) return fail(WFS_EINVAL, "wfs_set_closure: non-finite weight");
+            int rc;
+            if (!geo.p) {
+                if ((rc = alloc_buf(geo, (size_t)Np * sizeof(uint16_t))) ||
+                    (rc = alloc_buf(hcol, (size_t)px * d.ny)) || (rc = alloc_buf(mlp_params, kNParams * sizeof(float))) ||
+                    (rc = alloc_buf(mlp_pack, sizeof(MlpPack))) || (rc = alloc_buf(c2buf, (size_t)Np * sizeof(float))))
+                    return rc;
+            }
+            WFS_HIP(hipMemcpyAsync(stage.p, geo_h, (size_t)N * sizeof(uint16_t), hipMemcpyHostToDevice, stream));
+            hipLaunchKernelGGL((k_pad<uint16_t, uint16_t>), dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
+                               static_cast<const uint16_t*>(stage.p), static_cast<uint16_t*>(geo.p), d.nx, px, Np,
+                               uint16_t(0));
+            WFS_LAUNCH_CHECK();
+            WFS_HIP(hipStreamSynchronize(stream));
+            const long long ncol = (long long)px * d.ny;
+            WFS_HIP(hipMemcpyAsync(stage.p, hcol_h, (size_t)d.nx * d.ny, hipMemcpyHostToDevice, stream));
+            hipLaunchKernelGGL((k_pad<uint8_t, uint8_t>), dim3(blocks_for(ncol, 256)), dim3(256), 0, stream,
+                               static_cast<const uint8_t*>(stage.p), static_cast<uint8_t*>(hcol.p), d.nx, px, ncol,
+                               uint8_t(0));
+            WFS_LAUNCH_CHECK();
+            WFS_HIP(hipMemcpyAsync(mlp_params.p, params, kNParams * sizeof(float), hipMemcpyHostToDevice, stream));
+            hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(256), 0, stream, static_cast<const float*>(mlp_params.p),
+                               static_cast<MlpPack*>(mlp_pack.p));
+            WFS_LAUNCH_CHECK();
+            if ((rc = zero(c2buf))) return rc;
+            WFS_HIP(hipStreamSynchronize(stream));
+            inv_u0 = (float)(1.0 / u0);
+            closure = kind;
+            return WFS_OK;
+        }
+    }
+
+    int get_closure_field(float* out) override {
+        if (!c2buf.p) return fail(WFS_EINVAL, "wfs_get_closure_field: no learned closure set");
+        return get_cells(c2buf, out, false);
+    }
+
+    // Training samples from this (fine) run, box-filtered 2x2x2 onto the
+    // coarse grid ncx x ncy x ncz; outputs are device arrays (dense coarse order).
+    int closure_samples(const uint16_t* geo_h, const uint8_t* hcol_h, int ncx, int ncy, int ncz, double kv, double u0,
+                        float* X, float* ABP, uint8_t* valid) override {
+        if (ncx < 3 || ncy < 3 || ncz < 2 || 2 * ncx > d.nx || 2 * ncy > d.ny || 2 * ncz > d.nz)
+            return fail(WFS_EINVAL, "wfs_closure_samples: coarse grid must be at most half the fine grid");
+        if (!(u0 > 0.0) || !(kv > 0.0)) return fail(WFS_EINVAL, "wfs_closure_samples: u0 and kv must be positive");
+        if (!geo_h || !hcol_h || !X || !ABP || !valid) return fail(WFS_EINVAL, "wfs_closure_samples: null argument");
+        const long long nc = (long long)ncx * ncy * ncz;
+        DevBuf ub, ts, cf, g, hc;
+        int rc;
+        if ((rc = alloc_buf(ub, (size_t)nc * 3 * sizeof(float))) || (rc = alloc_buf(ts, (size_t)nc * 6 * sizeof(float))) ||
+            (rc = alloc_buf(cf, (size_t)nc)) || (rc = alloc_buf(g, (size_t)nc * sizeof(uint16_t))) ||
+            (rc = alloc_buf(hc, (size_t)ncx * ncy)))
+            return rc;
+        WFS_HIP(hipMemcpyAsync(g.p, geo_h, (size_t)nc * sizeof(uint16_t), hipMemcpyHostToDevice, stream));
+        WFS_HIP(hipMemcpyAsync(hc.p, hcol_h, (size_t)ncx * ncy, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_coarsen<R>, dim3(blocks_for(nc, 256)), dim3(256), 0, stream, rp(ux), rp(uy), rp(uz),
+                           flags(), px, d.ny, ncx, ncy, ncz, (float)kv, static_cast<float*>(ub.p),
+                           static_cast<float*>(ts.p), static_cast<uint8_t*>(cf.p));
+        WFS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_samples, dim3(blocks_for(nc, 256)), dim3(256), 0, stream, static_cast<const float*>(ub.p),
+                           static_cast<const float*>(ts.p), static_cast<const uint8_t*>(cf.p),
+                           static_cast<const uint16_t*>(g.p), static_cast<const uint8_t*>(hc.p), ncx, ncy, ncz,
+                           (float)(1.0 / u0), X, ABP, valid);
+        WFS_LAUNCH_CHECK();
+        WFS_HIP(hipStreamSynchronize(stream));
+        bytes -= ub.bytes + ts.bytes + cf.bytes + g.bytes + hc.bytes;
+        return WFS_OK;
+    }
+
     int check_finite() override {
         int rc = zero(counter);
         if (rc) return rc;
@@ -585,6 +690,8 @@ struct Batch final : BatchBase {
             if (c->device != device) return fail(WFS_EINVAL, "wfs_batch_create: cases must be on one device");
             if (bx0 < 0) bx0 = c->bx;
             if (c->bx != bx0) return fail(WFS_EINVAL, "wfs_batch_create: cases must share block_x");
+            if (c->closure != WFS_CLOSURE_SMAGORINSKY)
+                return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the constant-Cs closure");
             WFS_HIP(hipStreamSynchronize(c->stream));
             cases.push_back(c);
             CaseArgs<Codec> ca{};
@@ -1164,5 +1271,97 @@ int wfs_bandwidth_probe(int32_t device, int64_t bytes, int32_t iters, double* gb
 }
 
 int64_t wfs_device_bytes(const wfs_solver* h) { return h ? (int64_t)h->bytes : -1; }
+
+// ---------------------------------------------------------------- learned closure
+
+int wfs_set_closure(wfs_solver* h, int32_t kind, const float* params, const uint16_t* geo, const uint8_t* hcol,
+                    double u0) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    return h->set_closure(kind, params, geo, hcol, u0);
+}
+
+int wfs_get_closure_field(wfs_solver* h, float* c2) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!c2) return fail(WFS_EINVAL, "wfs_get_closure_field: null output");
+    DeviceGuard g(h->device);
+    return h->get_closure_field(c2);
+}
+
+int wfs_closure_samples(wfs_solver* fine, const uint16_t* geo, const uint8_t* hcol, int32_t ncx, int32_t ncy,
+                        int32_t ncz, double kv, double u0, float* X, float* ABP, uint8_t* valid) {
+    int rc = check_handle(fine);
+    if (rc) return rc;
+    DeviceGuard g(fine->device);
+    WFS_HIP(hipStreamSynchronize(fine->stream));
+    return fine->closure_samples(geo, hcol, ncx, ncy, ncz, kv, u0, X, ABP, valid);
+}
+
+int64_t wfs_mlp_work_floats(int64_t n) {
+    if (n < 0) return -1;
+    const int64_t blocks = (n + kGradBlock - 1) / kGradBlock;
+    const int64_t grad = blocks * kPartStride;
+    const int64_t pack = kPackWords;
+    return grad > pack ? grad : pack;
+}
+
+static int mlp_device(int32_t device) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        (void)hipGetLastError();
+        return fail(WFS_ENODEV, "no such device");
+    }
+    return WFS_OK;
+}
+
+int wfs_mlp_eval(int32_t device, const float* X, int64_t n, const float* params, float* c2, float* work,
+                 int64_t work_floats, void* stream) {
+    int rc = mlp_device(device);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!X || !c2)) || !params || !work) return fail(WFS_EINVAL, "wfs_mlp_eval: bad arguments");
+    if (work_floats < kPackWords) return fail(WFS_EINVAL, "wfs_mlp_eval: work buffer too small");
+    DeviceGuard g(device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(256), 0, st, params, reinterpret_cast<MlpPack*>(work));
+    WFS_LAUNCH_CHECK();
+    if (n == 0) return WFS_OK;
+    hipLaunchKernelGGL(k_mlp_eval, dim3(blocks_for(n, 256)), dim3(256), 0, st, X, (long long)n,
+                       reinterpret_cast<const MlpPack*>(work), c2);
+    WFS_LAUNCH_CHECK();
+    return WFS_OK;
+}
+
+int wfs_mlp_grad(int32_t device, const float* X, const float* ABP, int64_t n, const float* params, float* grad,
+                 double* sums, float* work, int64_t work_floats, void* stream) {
+    int rc = mlp_device(device);
+    if (rc) return rc;
+    if (n < 1 || !X || !ABP || !params || !grad || !sums || !work) return fail(WFS_EINVAL, "wfs_mlp_grad: bad arguments");
+    if (work_floats < wfs_mlp_work_floats(n)) return fail(WFS_EINVAL, "wfs_mlp_grad: work buffer too small");
+    const long long blocks = (n + kGradBlock - 1) / kGradBlock;
+    if (blocks > (1ll << 30)) return fail(WFS_EINVAL, "wfs_mlp_grad: batch too large");
+    DeviceGuard g(device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)blocks), dim3(kGradBlock), 0, st, X, ABP, (long long)n, params, work);
+    WFS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(1), dim3(256), 0, st, work, (int)blocks, (long long)n, grad, sums);
+    WFS_LAUNCH_CHECK();
+    return WFS_OK;
+}
+
+int wfs_mlp_adam(int32_t device, float* params, const float* grad, float* m, float* v, int64_t t, double lr,
+                 double beta1, double beta2, double eps, void* stream) {
+    int rc = mlp_device(device);
+    if (rc) return rc;
+    if (!params || !grad || !m || !v || t < 1 || !(lr > 0.0)) return fail(WFS_EINVAL, "wfs_mlp_adam: bad arguments");
+    DeviceGuard g(device);
+    const float c1 = (float)(1.0 / (1.0 - std::pow(beta1, (double)t)));
+    const float c2 = (float)(1.0 / (1.0 - std::pow(beta2, (double)t)));
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(kNParams, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), params,
+                       grad, m, v, kNParams, (float)lr, (float)beta1, (float)beta2, (float)eps, c1, c2);
+    WFS_LAUNCH_CHECK();
+    return WFS_OK;
+}
 
 }  // extern "C"

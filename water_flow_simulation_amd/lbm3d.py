@@ -54,6 +54,7 @@ class LBM3D:
         self.csmag2 = float(cs * cs)
         self.storage = storage
         self.nstep = 0
+        self.closure = "smagorinsky"
         L = nat.lib()
         desc = nat.wfs_desc(self.nx, self.ny, self.nz, self.tau0, self.cs, nat.STORAGE[storage], int(block_x),
                             int(device))
@@ -249,6 +250,32 @@ class LBM3D:
         ms, n = ctypes.c_double(0.0), ctypes.c_int64(0)
         nat.check(nat.lib().wfs_timing_read(self.handle, ctypes.byref(ms), ctypes.byref(n)))
         return float(ms.value), int(n.value)
+
+    # ------------------------------------------------------------ sub-grid closure
+    def set_closure(self, kind="smagorinsky", params=None, geo=None, hcol=None, u0=None):
+        """Select the sub-grid closure: "smagorinsky" (the reference's constant
+        Cs, floodsim/lbm3d.py:120-123) or "mlp", the learned per-cell C^2
+        (``params``: 1377 float weights; ``geo``/``hcol``: the geometry codes of
+        closure.geometry(solid); ``u0``: mean inlet speed, lattice units)."""
+        if kind not in nat.CLOSURE:
+            raise ValueError(f"unknown closure {kind!r}")
+        if kind == "smagorinsky":
+            nat.check(nat.lib().wfs_set_closure(self.handle, 0, None, None, None, 0.0))
+            self.closure = kind
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        g = np.ascontiguousarray(geo, dtype=np.uint16)
+        h = np.ascontiguousarray(hcol, dtype=np.uint8)
+        if p.size != nat.MLP_PARAMS or g.size != self.N or h.size != self.nx * self.ny:
+            raise TypeError("set_closure: params (1377), geo (nz, ny, nx) and hcol (ny, nx) sizes")
+        nat.check(nat.lib().wfs_set_closure(self.handle, 1, nat.ptr(p), nat.ptr(g), nat.ptr(h), float(u0)))
+        self.closure = kind
+
+    def closure_field(self):
+        """C^2 of every cell in the last step of the previous ``step`` call (learned closure)."""
+        out = np.empty(self.N, np.float32)
+        nat.check(nat.lib().wfs_get_closure_field(self.handle, nat.ptr(out)))
+        return out
 
     # ------------------------------------------------------------ reference helpers
     def shape3(self, a):

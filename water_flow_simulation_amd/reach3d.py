@@ -450,12 +450,18 @@ class Run3D:
 
     cadence = 20
 
-    def __init__(self, reach: Reach, be=None, *, storage="f32", block_x=0, device=0):
+    def __init__(self, reach: Reach, be=None, *, storage="f32", block_x=0, device=0, closure=None):
+        """closure: None for the constant-Cs Smagorinsky model, or the weights
+        of the learned closure (closure.py), evaluated inside the step."""
         self.r, self.be = reach, be
         nu_lb = max(1e-6 * reach.dt / reach.dx ** 2, 2e-4)
         uin_lb = (reach.uin_phys / reach.vel).astype(np.float32)
         self.lbm = LBM3D(be, reach.solid, reach.Fx, reach.Fy, reach.Fz, uin_lb, tau0=0.5 + 3 * nu_lb,
                          cs=reach.cfg.cs, cf_wall=reach.cf, storage=storage, block_x=block_x, device=device)
+        if closure is not None:
+            from .closure import geometry
+            geo, hcol = geometry(reach.solid)
+            self.lbm.set_closure("mlp", closure, geo, hcol, float(uin_lb[uin_lb > 0].mean()))
         if reach.bedwall is not None:
             self.lbm.set_bedwall(reach.bedwall)
             self.lbm.update_uref()
