@@ -63,10 +63,21 @@ def chunks(lbm, nsteps, cadence=20):
         k += m
 
 
-def time_gpu(reach, storage, steps, warmup, block_x):
+def closure_weights():
+    """The committed trained closure, or an untrained network (timing does not depend on the weights)."""
+    from water_flow_simulation_amd import closure as CL
+    return CL.load_params()[0] if os.path.exists(CL.WEIGHTS) else CL.init_params(0)
+
+
+def time_gpu(reach, storage, steps, warmup, block_x, learned=False):
     from water_flow_simulation_amd.lbm3d import LBM3D
     lbm = LBM3D(None, *solver_args(reach), storage=storage, block_x=block_x)
     lbm.set_bedwall(reach.bedwall)
+    if learned:
+        from water_flow_simulation_amd import closure as CL
+        geo, hcol = CL.geometry(reach.solid)
+        uin = solver_args(reach)[4]
+        lbm.set_closure("mlp", closure_weights(), geo, hcol, float(uin[uin > 0].mean()))
     chunks(lbm, warmup)
     lbm.synchronize()
     lbm.check_finite()
@@ -102,6 +113,27 @@ def cpu_baseline(reach, target_s=15.0):
                 sample=f"{n} steps of the full {reach.nx}x{reach.ny}x{reach.nz} reach (all cells counted)")
 
 
+def time_batched(cases, steps, seed=0, scale=0.5):
+    """The boat-test sweep: the same seeded ensemble run one case after
+    another and as one batch (one step launch over all cases, device-side
+    cadence, HIP-graph replay)."""
+    from water_flow_simulation_amd import sweep
+    reaches, _ = sweep.ensemble(cases, seed, scale=scale)
+    cells = sum(r.cells for r in reaches)
+    runs, _ = sweep.run_sequential(reaches, 40)          # warm-up (allocation, first launches)
+    for run in runs:
+        run.close()
+    runs, seq_s = sweep.run_sequential(reaches, steps)
+    for run in runs:
+        run.close()
+    b, _, bat_s = sweep.run_batched(reaches, steps)
+    b.close()
+    return {"cases": cases, "seed": seed, "cells_total": cells, "steps": steps,
+            "sequential_mlups": round(steps * cells / seq_s / 1e6, 1),
+            "batched_mlups": round(steps * cells / bat_s / 1e6, 1), "speedup": round(seq_s / bat_s, 3),
+            "ensemble": f"seeded boat-test reaches at dx 0.2 m, lengths and widths x{scale}"}
+
+
 def traffic_per_launch():
     """HBM bytes per step-kernel launch from the committed rocprofv3 counter summary, if any."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -118,7 +150,10 @@ def main():
     ap.add_argument("--workload", default="trapezoid_tug")
     ap.add_argument("--storage", default="f32", choices=sorted(B_ALG))
     ap.add_argument("--block-x", type=int, default=0)
-    ap.add_argument("--variants", default="f32fast,f16dev,bf16dev", help="extra storage formats timed (comma list)")
+    ap.add_argument("--variants", default="f32fast,f16dev,bf16dev,f32+mlp,f16dev+mlp",
+                    help="extra variants timed (comma list; storage format, '+mlp' for the learned closure)")
+    ap.add_argument("--batch-cases", type=int, default=8, help="cases of the batched-sweep line (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -156,16 +191,21 @@ def main():
         "mass_fp64": r["mass"],
     }
     variants = {}
-    for st in [v for v in a.variants.split(",") if v and v != a.storage]:
-        log(f"timing variant {st}")
-        rv = time_gpu(reach, st, a.steps, a.warmup, a.block_x)
+    for name in [v for v in a.variants.split(",") if v and v != a.storage]:
+        st, learned = name.split("+")[0], name.endswith("+mlp")
+        log(f"timing variant {name}")
+        rv = time_gpu(reach, st, a.steps, a.warmup, a.block_x, learned)
         kv = rv["kernel_ms"] / max(rv["launches"], 1) / 1e3
-        variants[st] = {"mlups": round(a.steps * reach.cells / rv["wall"] / 1e6, 1),
-                        "kernel_ms_avg": round(kv * 1e3, 5),
-                        "roofline_frac": round(B_ALG[st] * n_fluid / kv / 1e9 / PEAK_HBM_GBS, 4),
-                        "speedup_vs_f32": round(r["wall"] / rv["wall"], 3)}
+        b_alg = B_ALG[st] + (2 if learned else 0)       # + the 2-byte geometry code of the closure
+        variants[name] = {"mlups": round(a.steps * reach.cells / rv["wall"] / 1e6, 1),
+                          "kernel_ms_avg": round(kv * 1e3, 5),
+                          "roofline_frac": round(b_alg * n_fluid / kv / 1e9 / PEAK_HBM_GBS, 4),
+                          "speedup_vs_f32": round(r["wall"] / rv["wall"], 3)}
     if variants:
         out["variants"] = variants
+    if a.batch_cases > 0:
+        log(f"timing the batched sweep ({a.batch_cases} cases)")
+        out["batched"] = time_batched(a.batch_cases, a.batch_steps)
     if not a.no_cpu_baseline:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(reach)

@@ -209,7 +209,7 @@ int64_t wfs_device_bytes(const wfs_solver* h);
 #define WFS_MLP_FEATURES 8
 #define WFS_MLP_HIDDEN 32
 #define WFS_MLP_PARAMS 1377
-#define WFS_C2_MIN 0.002
+#define WFS_C2_MIN 0.01
 #define WFS_C2_SCALE 0.0289
 enum wfs_closure { WFS_CLOSURE_SMAGORINSKY = 0, WFS_CLOSURE_MLP = 1 };
 

@@ -30,7 +30,7 @@ OFF_B2 = OFF_W2 + NH * NH
 OFF_W3 = OFF_B2 + NH
 OFF_B3 = OFF_W3 + NH
 NPARAMS = OFF_B3 + 1
-C2_MIN = float(np.float32(0.002))
+C2_MIN = float(np.float32(0.01))
 C2_SCALE = float(np.float32(0.0289))
 
 

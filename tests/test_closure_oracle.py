@@ -35,7 +35,7 @@ def test_constants_agree_with_library_header():
     assert C.NF == nat.MLP_FEATURES
     assert abs(C.C2_MIN - nat.C2_MIN) < 1e-9 and abs(C.C2_SCALE - nat.C2_SCALE) < 1e-9
     text = open(nat.HEADER).read()
-    assert "#define WFS_MLP_PARAMS 1377" in text and "#define WFS_C2_MIN 0.002" in text
+    assert "#define WFS_MLP_PARAMS 1377" in text and "#define WFS_C2_MIN 0.01" in text
 
 
 def test_bf16_rounding_matches_torch():

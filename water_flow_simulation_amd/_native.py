@@ -96,7 +96,7 @@ _SIGS = {
 CLOSURE = {"smagorinsky": 0, "mlp": 1}
 MLP_PARAMS = 1377
 MLP_FEATURES = 8
-C2_MIN = 0.002
+C2_MIN = 0.01
 C2_SCALE = 0.0289
 
 _lib = None

@@ -5,7 +5,7 @@ judgement of the closure against fine-grid runs.
 Model (csrc/closure.h). An 8-32-32-1 network (ReLU hidden layers, softplus
 output) gives every cell a Smagorinsky coefficient
 
-    C^2 = 0.002 + 0.0289 * softplus(y),   nu_t = C^2 |S|,
+    C^2 = 0.01 + 0.0289 * softplus(y),   nu_t = C^2 |S|,
 
 from eight features: strain-rate magnitude and shape from the
 non-equilibrium moments, the share of vertical shear, speed and vertical
@@ -19,12 +19,14 @@ Training data. A reach is run at a fine spacing dx_f with the constant-Cs
 model; snapshots are box-filtered 2x2x2 onto the grid of dx_c = 2 dx_f. For a
 coarse cell the filter gives the resolved strain S (central differences of the
 filtered velocity) and the subfilter stress T = mean(u u) - mean(u) mean(u);
-the network is fitted in stress space,
+the network is fitted to the relative error of the eddy-viscosity stress,
 
-    loss = sum |T_dev + 2 C^2 |S| S|^2 / sum |T_dev|^2
+    sum |T_dev + 2 C^2 |S| S|^2 / sum |T_dev|^2,
 
-(the relative error of the eddy-viscosity stress), by Adam on the GPU with the
-library's forward/backward kernels (wfs_mlp_grad, wfs_mlp_adam). PyTorch holds
+plus (objective "mixed", the default) the relative error of the subfilter
+energy transfer, sum (C^2 |S|^3 - Pi)^2 / sum Pi^2 with Pi = -T:S, by Adam on
+the GPU with the library's forward/backward kernels (wfs_mlp_grad,
+wfs_mlp_adam). PyTorch holds
 the device buffers and shuffles indices; it computes nothing of the model.
 
 Judgement. On held-out cases: the same relative stress error for the
@@ -134,7 +136,9 @@ def case_pair(seed, width=(20.0, 40.0), depth=3.0, length=60.0, dx_c=0.1, run_fl
     dx_c / 2 with half the time step (the fine run), so that lattice
     velocities agree. Bed width drawn from ``width`` (a multiple of 2 dx_c),
     trapezoid with 1:2 banks, Manning 0.03, Q from the Manning rule on the E3
-    slope; one or two propeller boats of a random preset at a random place."""
+    slope; one or two propeller boats of a random preset at a random place,
+    with the preset's power scaled by the flow area over 100 m^2 (capped at
+    1) so that small test channels carry proportionate thrust."""
     rng = np.random.default_rng(seed)
     B = float(rng.uniform(*width))
     B = round(B / (2 * dx_c)) * 2 * dx_c
@@ -143,8 +147,13 @@ def case_pair(seed, width=(20.0, 40.0), depth=3.0, length=60.0, dx_c=0.1, run_fl
     key = ("longtail", "outboard", "tug", "pontoon")[int(rng.integers(0, 4))]
     nb = int(rng.integers(1, 3))
     boat = replace(R.boat_type(key), tilt_deg=float(rng.uniform(0, 15)))
-    boat = replace(boat, prop_depth=min(boat.prop_depth, 0.6 * depth), draft=min(boat.draft, 0.4 * depth),
-                   hull_L=min(boat.hull_L, 0.2 * length), hull_B=min(boat.hull_B, 0.3 * B))
+    area = (B + sec.z * depth) * depth
+    draft = min(boat.draft, 0.4 * depth)
+    D = min(boat.prop_D, 0.4 * depth)
+    # propeller disc below the keel and clear of the bed
+    zp = min(max(boat.prop_depth, draft + 0.6 * D), depth - 0.6 * D)
+    boat = replace(boat, prop_D=D, prop_depth=zp, draft=draft, hull_L=min(boat.hull_L, 0.2 * length),
+                   hull_B=min(boat.hull_B, 0.3 * B), power_kW=boat.power_kW * min(1.0, area / 100.0))
     fleet = R.Fleet(boat, rows=1, per_row=nb, x_first=float(rng.uniform(0.2, 0.35)) * length, enabled=with_boat)
     cfg_c = R.ReachConfig(length=length, dx=dx_c, run_flows=run_flows, avg_frac=avg_frac)
     coarse = R.build_reach(sec, depth, Q, E3_SLOPE, fleet, cfg_c)
@@ -188,7 +197,8 @@ def collect(run, coarse_solid, u0c, kv=1.0):
 def fine_run(fine, coarse, snapshots=8, storage="f32", log=None):
     """Runs the fine case to its end with the constant-Cs model and collects
     samples at ``snapshots`` evenly spaced times of the averaging window.
-    Returns (X, ABP, run) with the run kept open for its time means."""
+    Returns (X, ABP, run) with the run kept open for its time means, or None
+    when the fine run does not stay finite (the case is then left out)."""
     torch = _torch()
     run = R.Run3D(fine, storage=storage)
     ncx, ncy, ncz = fine.nx // 2, fine.ny // 2, fine.nz // 2
@@ -204,6 +214,13 @@ def fine_run(fine, coarse, snapshots=8, storage="f32", log=None):
         mk = max(mk - mk % R.Run3D.cadence, done + R.Run3D.cadence)
         run.advance(mk - done)
         done = mk
+        try:
+            run.lbm.check_finite()
+        except nat.NonFiniteState as e:
+            if log:
+                log(f"  fine run left out: {e}")
+            run.close()
+            return None
         X, ABP = collect(run, csol, u0c, kv)
         Xs.append(X)
         Ps.append(ABP)
@@ -211,6 +228,13 @@ def fine_run(fine, coarse, snapshots=8, storage="f32", log=None):
             log(f"  fine step {done}/{steps_total}: {X.shape[0]} samples")
     while not run.done:
         run.advance(R.Run3D.cadence * 20)
+    try:
+        run.lbm.check_finite()
+    except nat.NonFiniteState as e:
+        if log:
+            log(f"  fine run left out: {e}")
+        run.close()
+        return None
     return torch.cat(Xs), torch.cat(Ps), run
 
 
@@ -232,6 +256,32 @@ def evaluate(X, ABP, p, chunk=1 << 20):
                                          grad.data_ptr(), sums.data_ptr(), work.data_ptr(), work.numel(), st))
         tot += sums.cpu().numpy()
     return float(tot[0] / tot[1])
+
+
+OBJECTIVES = ("stress", "dissipation", "mixed")
+
+
+def objective_coeffs(ABP, objective="stress"):
+    """Per-sample coefficients (A, B, P) of loss = P + 4 C^2 B + 4 C^4 A for a
+    training objective:
+      stress       |T + 2 C^2 |S| S|^2 (the samples as collected);
+      dissipation  (C^2 |S|^3 - Pi)^2, Pi = -T:S the subfilter energy transfer,
+                   i.e. A = |S|^6 / 4, B = |S|^2 B_stress / 2, P = Pi^2;
+      mixed        the two, each divided by its sum of P over the samples."""
+    if objective == "stress":
+        return ABP
+    if objective not in OBJECTIVES:
+        raise ValueError(f"unknown objective {objective!r}")
+    torch = _torch()
+    A, B, P = ABP.double().unbind(1)
+    s2 = torch.sqrt(2 * A)                  # |S|^2
+    smag = torch.sqrt(s2)
+    pi = torch.where(smag > 0, -B / torch.clamp(smag, min=1e-30), torch.zeros_like(B))
+    diss = torch.stack([s2 ** 3 / 4, s2 * B / 2, pi * pi], 1)
+    if objective == "dissipation":
+        return diss.float().contiguous()
+    st = ABP.double() / ABP[:, 2].double().sum()
+    return (st + diss / diss[:, 2].sum()).float().contiguous()
 
 
 def constant_errors(ABP, cs=0.17):
@@ -305,28 +355,63 @@ def filtered_means(run, coarse):
     return out, cover
 
 
-def aposteriori(coarse, fine_means, cover, params, storage="f32"):
-    """Coarse runs with the constant-Cs model and with the network, each
-    against the filtered fine time-mean velocity: relative L2 over cells fluid
-    in both grids, away from the inlet and outlet rows."""
+def _pairs(a):
+    a = np.asarray(a, np.float64)
+    return a[:2 * (len(a) // 2)].reshape(-1, 2).mean(1)
+
+
+def profile_errors(pc, pf):
+    """Coarse-run profiles pc against the fine run's pf (two fine x-cells per
+    coarse one): level max |d eta| (mm, both relative to the outlet), surface
+    velocity relative L2, discharge max relative error; inlet/outlet cells
+    excluded."""
+    sl = slice(2, -2)
+    eta_f = _pairs(pf["eta"])
+    eta_f = eta_f - eta_f[-1]
+    eta_c = np.asarray(pc["eta"]) - pc["eta"][-1]
+    us_f, q_f = _pairs(pf["us"]), _pairs(pf["q"])
+    return {"eta_max_mm": float(np.abs(eta_c - eta_f)[sl].max() * 1e3),
+            "us_rel_l2": float(np.linalg.norm((pc["us"] - us_f)[sl]) / np.linalg.norm(us_f[sl])),
+            "q_max_rel": float(np.abs(pc["q"] - q_f)[sl].max() / np.abs(q_f[sl]).max())}
+
+
+def aposteriori(coarse, fine_means, cover, closures, fine_profiles=None, storage="f32"):
+    """Coarse runs, one per entry of ``closures`` (name -> weights, None for
+    the constant-Cs model), against the fine run: relative L2 of the time-mean
+    velocity over cells fluid in both grids away from the inlet and outlet
+    rows, and with ``fine_profiles`` the errors of the reported profiles."""
     res = {}
     mask = cover & ~coarse.solid
     mask[:, :, :2] = False
     mask[:, :, -2:] = False
     ref = fine_means[:, mask]
-    for name, cl in (("smagorinsky", None), ("learned", params)):
+    for name, cl in closures.items():
         run = R.Run3D(coarse, storage=storage, closure=cl)
-        run.run_to_end()
-        run.lbm.check_finite()
+        unstable = None
+        while not run.done:
+            run.advance(R.Run3D.cadence * 500)
+            try:
+                run.lbm.check_finite()
+            except nat.NonFiniteState:
+                unstable = run.lbm.nstep
+                break
+        if unstable is not None:        # the run blew up: record when
+            res[name] = {"unstable_at_step": int(unstable)}
+            run.close()
+            continue
         u = np.stack([run.field(k, mean=True) for k in ("ux", "uy", "uz")])[:, mask]
-        res[name] = float(np.linalg.norm(u - ref) / np.linalg.norm(ref))
+        row = {"u_rel_l2": float(np.linalg.norm(u - ref) / np.linalg.norm(ref))}
+        if fine_profiles is not None:
+            row.update(profile_errors(R.profiles(run), fine_profiles))
+        res[name] = row
         run.close()
     return res
 
 
 # ---------------------------------------------------------------- study
 def study(cases=10, held_out=4, seed=0, width=(20.0, 40.0), depth=3.0, length=60.0, dx_c=0.1, snapshots=8,
-          epochs=6, batch=65536, lr=3e-3, run_flows=1.0, avg_frac=0.5, posteriori=True, log=print):
+          epochs=6, batch=65536, lr=3e-3, run_flows=1.0, avg_frac=0.5, posteriori=True, log=print,
+          objective="mixed"):
     """Fine runs of ``cases`` seeded reaches (the last ``held_out`` kept for
     evaluation), training, and the a-priori and a-posteriori judgement."""
     torch = _torch()
@@ -338,41 +423,58 @@ def study(cases=10, held_out=4, seed=0, width=(20.0, 40.0), depth=3.0, length=60
         cells += fine.cells
         if log:
             log(f"case {k}: {label}; fine {fine.nx}x{fine.ny}x{fine.nz}, coarse {coarse.nx}x{coarse.ny}x{coarse.nz}")
-        X, ABP, run = fine_run(fine, coarse, snapshots, log=log)
+        got = fine_run(fine, coarse, snapshots, log=log)
+        if got is None:
+            continue
+        X, ABP, run = got
         if k < cases - held_out:
             train_X.append(X)
             train_P.append(ABP)
             run.close()
         else:
             means, cover = filtered_means(run, coarse) if posteriori else (None, None)
+            prof = R.profiles(run) if posteriori else None
             run.close()
-            test.append((X, ABP, coarse, means, cover, label))
+            test.append((X, ABP, coarse, means, cover, prof, label))
     X = torch.cat(train_X)
     ABP = torch.cat(train_P)
     del train_X, train_P
     t_data = time.time() - t0
-    consts = constant_errors(ABP)
+    Y = objective_coeffs(ABP, objective)
+    consts = constant_errors(Y)
     if log:
-        log(f"{X.shape[0]} training samples; constant-Cs error {consts['smagorinsky']:.4f}, "
+        log(f"{X.shape[0]} training samples ({objective}); constant-Cs error {consts['smagorinsky']:.4f}, "
             f"best constant C^2 = {consts['best_c2']:.4f} error {consts['best_constant']:.4f}")
     t1 = time.time()
-    p = train(X, ABP, init_params(seed), epochs, batch, lr, seed, log)
+    p = train(X, Y, init_params(seed), epochs, batch, lr, seed, log)
     t_train = time.time() - t1
-    out = {"samples": int(X.shape[0]), "fine_cells_total": int(cells), "data_seconds": round(t_data, 1),
-           "train_seconds": round(t_train, 1), "train": {**consts, "learned": evaluate(X, ABP, p)}, "held_out": []}
-    best_c2 = consts["best_c2"]
-    for Xh, Ph, coarse, means, cover, label in test:
-        c = constant_errors(Ph)
-        s = Ph.double().sum(0).cpu().numpy()
-        e_best = float((s[2] + 4 * best_c2 * s[1] + 4 * best_c2 ** 2 * s[0]) / s[2])
-        row = {"case": label, "samples": int(Xh.shape[0]), "smagorinsky": c["smagorinsky"],
-               "train_best_constant": e_best, "learned": evaluate(Xh, Ph, p)}
-        if posteriori:
-            row["aposteriori_rel_l2"] = aposteriori(coarse, means, cover, p)
-        out["held_out"].append(row)
+    out = {"objective": objective, "samples": int(X.shape[0]), "fine_cells_total": int(cells),
+           "data_seconds": round(t_data, 1), "train_seconds": round(t_train, 1),
+           "train": {**consts, "learned": evaluate(X, Y, p)}, "held_out": []}
+    del Y
+    for Xh, Ph, coarse, means, cover, prof, label in test:
+        out["held_out"].append(judge(Xh, Ph, p, consts["best_c2"], label, coarse if posteriori else None, means,
+                                     cover, prof, objective))
         if log:
-            log(f"held out {label}: {row}")
+            log(f"held out {label}: {out['held_out'][-1]}")
     return p, out
+
+
+def judge(Xh, Ph, p, best_c2, label, coarse=None, means=None, cover=None, prof=None, objective="stress"):
+    """A-priori errors of the network, Cs = 0.17 and the training set's best
+    constant on held-out samples (stress and dissipation metrics), and with a
+    coarse case the a-posteriori comparison."""
+    row = {"case": label, "samples": int(Xh.shape[0])}
+    for obj in ("stress", "dissipation"):
+        Y = objective_coeffs(Ph, obj)
+        c = constant_errors(Y)
+        s = Y.double().sum(0).cpu().numpy()
+        row[obj] = {"smagorinsky": c["smagorinsky"], "learned": evaluate(Xh, Y, p)}
+        if obj == objective or objective == "mixed":
+            row[obj]["train_best_constant"] = float((s[2] + 4 * best_c2 * s[1] + 4 * best_c2 ** 2 * s[0]) / s[2])
+    if coarse is not None:
+        row["aposteriori"] = aposteriori(coarse, means, cover, {"smagorinsky": None, "learned": p}, prof)
+    return row
 
 
 def main():
@@ -390,11 +492,12 @@ def main():
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--run-flows", type=float, default=1.0)
     ap.add_argument("--no-posteriori", action="store_true")
+    ap.add_argument("--objective", default="mixed", choices=OBJECTIVES)
     ap.add_argument("--out", default=None, help="write the trained weights (JSON) here")
     a = ap.parse_args()
     p, res = study(a.cases, a.held_out, a.seed, tuple(a.width), a.depth, a.length, 2 * a.dx, a.snapshots, a.epochs,
                    a.batch, a.lr, a.run_flows, posteriori=not a.no_posteriori,
-                   log=lambda m: print(m, flush=True))
+                   log=lambda m: print(m, flush=True), objective=a.objective)
     print(json.dumps(res), flush=True)
     if a.out:
         save_params(a.out, p, {"study": res, "args": vars(a)})

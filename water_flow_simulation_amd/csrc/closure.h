@@ -46,7 +46,7 @@ constexpr int kOffW3 = kOffB2 + kNH;
 constexpr int kOffB3 = kOffW3 + kNH;
 constexpr int kNParams = kOffB3 + 1;   // 1377
 
-constexpr float kC2Min = 0.002f;    // floor of C^2 (Cs ~ 0.045)
+constexpr float kC2Min = 0.01f;     // floor of C^2 (Cs = 0.1): the near-inviscid lattice (tau0 ~ 0.5006) needs it
 constexpr float kC2Scale = 0.0289f; // C^2 per unit softplus (0.17^2)
 
 typedef short s4v __attribute__((ext_vector_type(4)));
