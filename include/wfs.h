@@ -72,6 +72,14 @@ const char* wfs_last_error(void);
 /* Creates a solver (replaces LBM3D.__init__ allocation, lbm3d.py:342-366). */
 int wfs_create(const wfs_desc* desc, wfs_solver** out);
 
+/* Work-group width of the step kernel (64, 128, 192 or 256 cells along x;
+ * the x-tile of one work-group). Takes effect at the next wfs_step; batches
+ * keep the width their cases had when the batch was made. No counterpart in
+ * the reference (its launch shape is fixed, lbm3d.py:401-404); used by the
+ * launch-shape autotuner. */
+int wfs_set_block_x(wfs_solver* h, int32_t block_x);
+int32_t wfs_block_x(const wfs_solver* h);
+
 /* Frees all device memory of the solver. NULL is accepted. */
 int wfs_destroy(wfs_solver* h);
 
@@ -152,6 +160,18 @@ int wfs_profile_x(wfs_solver* h, int32_t field, int32_t z0, int32_t z1, double* 
 
 /* FP64 total over fluid cells of a field (e.g. mass = total of WFS_F_RHO). */
 int wfs_total(wfs_solver* h, int32_t field, double* out);
+
+/* FP64 momentum budget of the next step, lattice units (momentum per step):
+ * out[0..2] the applied body force (thrust, F * scale summed over fluid
+ * cells), out[3..5] the force of the fluid on the cells marked by
+ * wfs_set_body_mask (hulls), out[6..8] the force on all other walls (bed,
+ * banks, domain edges). Wall forces come from momentum exchange on the
+ * bounce-back links, including the wall function's slip correction. Together
+ * with the inlet/outlet fluxes these close the reach's momentum balance, the
+ * budget that compare() (floodsim/reach3d.py:571-618) estimates from profiles.
+ * No reference counterpart (the reference reduces nothing on the device). */
+int wfs_set_body_mask(wfs_solver* h, const uint8_t* body);
+int wfs_forces(wfs_solver* h, double* out);
 
 /* Watchdog: WFS_ENONFINITE if any fluid cell's rho or u is NaN/Inf. */
 int wfs_check_finite(wfs_solver* h);

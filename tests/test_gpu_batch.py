@@ -23,8 +23,9 @@ def _state(run):
     return sums, navg, run.lbm.macro(), run.lbm.populations()
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_batch_matches_sequential_bitwise(use_graph):
+@pytest.mark.parametrize("use_graph,group", [(True, 0), (False, 0), (True, 2)])
+def test_batch_matches_sequential_bitwise(use_graph, group):
+    print(f"case: batch of 5 reaches, graph={use_graph}, cases per launch={group or 'all'}", flush=True)
     reaches = _cases()
     pattern = (100, 35, 40)              # graph chunks, an odd remainder (parity flip), a re-captured graph
     seq = []
@@ -33,7 +34,8 @@ def test_batch_matches_sequential_bitwise(use_graph):
         for m in pattern:
             run.advance(m)
         seq.append(run)
-    b = BatchRun(reaches)
+    b = BatchRun(reaches, group=group)
+    assert len(b._hs) == (1 if group == 0 else 3)
     for m in pattern:
         b.advance(m, use_graph=use_graph)
         # eager readout between graph replays
@@ -53,3 +55,39 @@ def test_batch_matches_sequential_bitwise(use_graph):
     for run in seq:
         run.close()
     b.close()
+
+
+def test_block_width_switch_mid_run_and_autotuner(tmp_path, monkeypatch):
+    """Changing the work-group width between steps leaves the result bit for
+    bit unchanged, and the autotuner times every width and caches its pick."""
+    print("case: channel128, width switch and autotune", flush=True)
+    monkeypatch.setenv("WFS_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    from water_flow_simulation_amd import autotune
+    monkeypatch.setattr(autotune, "_cache", None)
+    r = channel128()
+    a, b = R.Run3D(r, block_x=128), R.Run3D(r, block_x=128)
+    a.advance(60)
+    b.advance(20)
+    b.lbm.block_x = 64
+    b.advance(20)
+    b.lbm.block_x = 256
+    b.advance(20)
+    assert b.lbm.block_x == 256
+    fl = np.tile(~r.solid.ravel(), 19)
+    assert np.array_equal(a.lbm.populations()[fl], b.lbm.populations()[fl])
+    with pytest.raises(ValueError):
+        b.lbm.block_x = 96
+    a.close()
+    b.close()
+    t = autotune.time_widths(r, "f32", steps=10, warmup=2)
+    assert sorted(t) == [64, 128, 192, 256] and all(v > 0 for v in t.values())
+    bx = autotune.block_x_for(r, "f32")
+    assert bx in (64, 128, 192, 256)
+    assert (tmp_path / "tune.json").exists()
+    run = R.Run3D(r, block_x="auto")
+    assert run.lbm.block_x == bx
+    run.close()
+    reaches, _ = ensemble(3, seed=1, scale=0.2)
+    batch = BatchRun(reaches, group="auto")
+    assert batch.group in (1, 2, 3) and len(batch.tuning) >= 2
+    batch.close()

@@ -82,6 +82,10 @@ _SIGS = {
     "wfs_batch_time": ([_P, _I32, _P, _P], ctypes.c_int),
     "wfs_bandwidth_probe": ([_I32, ctypes.c_int64, _I32, _P], ctypes.c_int),
     "wfs_device_bytes": ([_P], ctypes.c_int64),
+    "wfs_set_body_mask": ([_P, _P], ctypes.c_int),
+    "wfs_forces": ([_P, _P], ctypes.c_int),
+    "wfs_set_block_x": ([_P, _I32], ctypes.c_int),
+    "wfs_block_x": ([_P], _I32),
     "wfs_set_closure": ([_P, _I32, _P, _P, _P, ctypes.c_double], ctypes.c_int),
     "wfs_get_closure_field": ([_P, _P], ctypes.c_int),
     "wfs_closure_samples": ([_P, _P, _P, _I32, _I32, _I32, ctypes.c_double, ctypes.c_double, _P, _P, _P],

@@ -1,14 +1,16 @@
 """Many reaches stepped together on one GPU (the boat-test sweep).
 
 ``BatchRun`` takes built reaches, sets each up as a normal ``Run3D`` (solver
-upload, bed/bank mask, initial uref, ramp state), then hands all of them to
-one libwfs batch: every step is a single launch over all cases, and the run
+upload, bed/bank mask, initial uref, ramp state), then hands them to libwfs
+batches: every step is a single launch over all cases of a batch, and the run
 driver's 20-step cadence (ramp, uref, time-mean sums; floodsim/reach3d.py
 :424-453) runs on the device from per-case schedules, replayed from a HIP
-graph. Results are read per case with the ``Run3D`` readouts (``field``,
-``profiles``). This replaces the one-case-after-another loop of
-``run_test`` (floodsim/boat_tests.py:79-110) for cases that run the same
-number of steps.
+graph. With ``group`` cases per batch the cases are split into several
+batches, each on its own stream, which the GPU runs side by side; the
+autotuner picks ``group`` and the work-group width. Results are read per case
+with the ``Run3D`` readouts (``field``, ``profiles``). This replaces the
+one-case-after-another loop of ``run_test`` (floodsim/boat_tests.py:79-110)
+for cases that run the same number of steps.
 """
 from __future__ import annotations
 
@@ -20,45 +22,80 @@ from .reach3d import Run3D
 
 
 class BatchRun:
-    def __init__(self, reaches, storage="f32", block_x=0, device=0, be=None):
-        self.runs = [Run3D(r, be, storage=storage, block_x=block_x, device=device) for r in reaches]
-        n = len(self.runs)
-        handles = (ctypes.c_void_p * n)(*[run.lbm.handle.value for run in self.runs])
-        self._h = ctypes.c_void_p()
-        nat.check(nat.lib().wfs_batch_create(handles, n, ctypes.byref(self._h)), "wfs_batch_create")
-        for k, run in enumerate(self.runs):
-            nat.check(nat.lib().wfs_batch_set_schedule(self._h, k, run.r.dt, run.t_ramp, run.t_avg, run.t))
+    def __init__(self, reaches, storage="f32", block_x=0, device=0, be=None, group=0):
+        """block_x: work-group width (0: library default, "auto": autotuned on
+        the largest case); group: cases per batched launch (0: all, "auto":
+        autotuned)."""
+        bx = block_x
+        if block_x == "auto":
+            from .autotune import block_x_for
+            bx = block_x_for(max(reaches, key=lambda r: r.cells), storage)
+        self.runs = [Run3D(r, be, storage=storage, block_x=bx, device=device) for r in reaches]
+        self._hs = []
+        self.group = 0
         self.steps_wall = 0.0
         self.nstep = 0
+        if group == "auto":
+            from .autotune import tune_group
+            tune_group(self)
+        else:
+            self.set_group(group or len(self.runs))
+
+    def set_group(self, k):
+        """Regroup the cases into batches of k (the last may be smaller)."""
+        if self._hs:
+            self.synchronize()
+        self._destroy()
+        n = len(self.runs)
+        k = max(1, min(int(k), n))
+        L = nat.lib()
+        for start in range(0, n, k):
+            sub = self.runs[start:start + k]
+            handles = (ctypes.c_void_p * len(sub))(*[run.lbm.handle.value for run in sub])
+            h = ctypes.c_void_p()
+            nat.check(L.wfs_batch_create(handles, len(sub), ctypes.byref(h)), "wfs_batch_create")
+            self._hs.append((h, start, len(sub)))
+            for j, run in enumerate(sub):
+                nat.check(L.wfs_batch_set_schedule(h, j, run.r.dt, run.t_ramp, run.t_avg, run.t))
+        self.group = k
 
     def advance(self, nsteps, use_graph=True):
-        """Advance every case by nsteps (asynchronous until synchronize())."""
+        """Advance every case by nsteps; returns when all batches are done."""
         t0 = time.time()
-        nat.check(nat.lib().wfs_batch_run(self._h, int(nsteps), 1 if use_graph else 0), "wfs_batch_run")
+        for h, _, _ in self._hs:          # queued on each batch's own stream: the batches overlap
+            nat.check(nat.lib().wfs_batch_run(h, int(nsteps), 1 if use_graph else 0), "wfs_batch_run")
         self.nstep += int(nsteps)
         self.synchronize()
         self.steps_wall += time.time() - t0
 
     def synchronize(self):
-        nat.check(nat.lib().wfs_batch_synchronize(self._h))
+        L = nat.lib()
         t, navg = ctypes.c_double(0.0), ctypes.c_int32(0)
-        for k, run in enumerate(self.runs):
-            nat.check(nat.lib().wfs_batch_time(self._h, k, ctypes.byref(t), ctypes.byref(navg)))
-            run.t = t.value
-            run.navg = navg.value
-            run.lbm.nstep = self.nstep
-            run._ramp_done = run.t_ramp <= 0 or run.t >= run.t_ramp
+        for h, start, n in self._hs:
+            nat.check(L.wfs_batch_synchronize(h))
+            for j in range(n):
+                run = self.runs[start + j]
+                nat.check(L.wfs_batch_time(h, j, ctypes.byref(t), ctypes.byref(navg)))
+                run.t = t.value
+                run.navg = navg.value
+                run.lbm.nstep = nat.lib().wfs_nstep(run.lbm.handle)
+                run._ramp_done = run.t_ramp <= 0 or run.t >= run.t_ramp
 
     def mlups(self):
         cells = sum(run.r.cells for run in self.runs)
         return self.nstep * cells / max(self.steps_wall, 1e-9) / 1e6
 
+    def _destroy(self):
+        for h, _, _ in getattr(self, "_hs", []):
+            if h.value:
+                nat.lib().wfs_batch_destroy(h)
+        self._hs = []
+
     def close(self):
-        if getattr(self, "_h", None) is not None and self._h.value:
-            nat.lib().wfs_batch_destroy(self._h)
-            self._h = None
-        for run in self.runs:
+        self._destroy()
+        for run in getattr(self, "runs", []):
             run.close()
+        self.runs = []
 
     def __del__(self):
         try:

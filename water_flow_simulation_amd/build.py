@@ -24,10 +24,13 @@ HEADERS = ["lattice.h", "step_kernels.h", "aux_kernels.h"]
 
 # -ffp-contract=off: no FMA contraction, so the FP32 kernels round like the
 # scalar IEEE formulas they implement. Packed FP32 instructions are disabled.
+# MFMA accumulators in ordinary VGPRs (the learned closure's network reads
+# every accumulator back for bias, ReLU and the output layer).
 HIPFLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form=true",
     "-Wall", "-Wno-unused-result",
 ]
 

@@ -285,3 +285,80 @@ __global__ void k_nonfinite(const R* __restrict__ rho, const R* __restrict__ ux,
 }
 
 }  // namespace wfs
+
+namespace wfs {
+
+// ---------------------------------------------------------------- FP64 momentum budget
+// Forces of one step, in lattice units, summed in double:
+//   out[0..2] applied body force (thrust): F * fscale over forced fluid cells;
+//   out[3..5] force of the fluid on the cells marked in `body` (hulls);
+//   out[6..8] force on every other wall (bed, banks, domain edges).
+// For a wall link i of a fluid cell (its pull source x - c_i is a wall), the
+// next step returns the population that leaves toward the wall, g =
+// f_opp(i)(x) as stored, as f_i = g + delta (bounce-back plus the wall
+// function's slip correction, evaluated here by the step's own gather), so
+// the wall receives the momentum c_opp(i) (g + f_i). Each block writes its
+// nine partial sums; k_sum_parts adds the blocks in a fixed order.
+template <class Codec>
+__global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __restrict__ fs,
+                                                    StepArgs<typename Codec::R> a,
+                                                    const uint8_t* __restrict__ body, double* __restrict__ part) {
+    using R = typename Codec::R;
+    __shared__ double red[9][256];
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < a.N) {
+        const int row = int(c / a.px);
+        const int x = int(c - (long long)row * a.px);
+        const int z = row / a.ny, y = row - z * a.ny;
+        const uint8_t fl = x < a.nx ? a.flag[c] : uint8_t(FLAG_SOLID);
+        if (!(fl & FLAG_SOLID)) {
+            if (fl & FLAG_FORCED) {
+                acc[0] = double(a.Fx[c] * a.fscale);
+                acc[1] = double(a.Fy[c] * a.fscale);
+                acc[2] = double(a.Fz[c] * a.fscale);
+            }
+            if (fl & FLAG_NEARWALL) {
+                const uint32_t wm = a.wmask[c];
+                R f[Q];
+                gather<Codec>(fs, a, int(c), x, z, fl, wm, f);
+                static_for<Q>([&](auto I) {
+                    constexpr int i = I;
+                    if ((wm >> i) & 1u) {
+                        constexpr int o = kOPP[i];
+                        const R g = Codec::template ld<o>(fs + (long long)o * a.N + c, a.ctr.c[center_class(o)]);
+                        const double m = double(g) + double(f[i]);
+                        const int xs = x - kCX[i], ys = y - kCY[i], zs = z - kCZ[i];
+                        const bool in = xs >= 0 && xs < a.nx && ys >= 0 && ys < a.ny && zs >= 0 && zs < a.nz;
+                        const bool hb = in && body && body[xs + (long long)a.px * (ys + (long long)a.ny * zs)];
+                        const int k = hb ? 3 : 6;
+                        acc[k] -= kCX[i] * m;
+                        acc[k + 1] -= kCY[i] * m;
+                        acc[k + 2] -= kCZ[i] * m;
+                    }
+                });
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int s = int(blockDim.x) / 2; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s)
+#pragma unroll
+            for (int k = 0; k < 9; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 9) part[(long long)blockIdx.x * 9 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// out[k] = sum over blocks b of part[b * nv + k], in block order.
+__global__ void k_sum_parts(const double* __restrict__ part, long long blocks, int nv, double* __restrict__ out) {
+    const int k = int(threadIdx.x);
+    if (k >= nv) return;
+    double s = 0.0;
+    for (long long b = 0; b < blocks; ++b) s += part[b * nv + k];
+    out[k] = s;
+}
+
+}  // namespace wfs

@@ -91,20 +91,23 @@ __device__ __forceinline__ float softplus_f(float y) {
     return y > 15.f ? y : __logf(1.f + __expf(y));
 }
 
-// The eight features of one cell (formulas in the header comment).
+// The eight features of one cell (formulas in the header comment). The
+// network sees them rounded to bfloat16, so the hardware square root,
+// reciprocal square root and reciprocal (about 1 ulp) serve here.
 __device__ __forceinline__ void closure_features(float sxx, float syy, float szz, float sxy, float sxz, float syz,
                                                  float ux, float uy, float uz, uint16_t geo, uint8_t hcol,
                                                  float inv_u0, float* x) {
     const float od = sxy * sxy + sxz * sxz + syz * syz;
     const float ss = sxx * sxx + syy * syy + szz * szz + 2.f * od;
-    const float smag = sqrtf(2.f * ss);
+    const float smag = __builtin_amdgcn_sqrtf(2.f * ss);
     x[0] = 0.25f * __logf(smag + 1e-7f) + 1.5f;
     const float t3 = sxx * sxx * sxx + syy * syy * syy + szz * szz * szz + 3.f * sxy * sxy * (sxx + syy) +
                      3.f * sxz * sxz * (sxx + szz) + 3.f * syz * syz * (syy + szz) + 6.f * sxy * sxz * syz;
     const bool live = ss > 1e-20f;
-    x[1] = live ? fminf(fmaxf(2.4494897f * t3 / (ss * sqrtf(ss)), -1.f), 1.f) : 0.f;
-    x[2] = sqrtf(ux * ux + uy * uy + uz * uz) * inv_u0 - 1.f;
-    x[3] = live ? 2.f * (sxz * sxz + syz * syz) / ss : 0.f;
+    const float rs = __builtin_amdgcn_rsqf(ss);
+    x[1] = live ? fminf(fmaxf(2.4494897f * t3 * (rs * rs * rs), -1.f), 1.f) : 0.f;
+    x[2] = __builtin_amdgcn_sqrtf(ux * ux + uy * uy + uz * uz) * inv_u0 - 1.f;
+    x[3] = live ? 2.f * (sxz * sxz + syz * syz) * (rs * rs) : 0.f;
     x[4] = uz * inv_u0;
     x[5] = float(geo & 255u) * (1.f / 255.f);
     x[6] = float(geo >> 8) * (1.f / 255.f);
@@ -129,7 +132,7 @@ __device__ __forceinline__ float mlp_wave(const float* x, const MlpPack* P) {
     const s4v a1 = __builtin_bit_cast(s4v, u2v{P->w1[2 * lane], P->w1[2 * lane + 1]});
     const bf8v a2s0 = __builtin_bit_cast(bf8v, *reinterpret_cast<const u4v*>(&P->w2[(0 * 64 + lane) * 4]));
     const bf8v a2s1 = __builtin_bit_cast(bf8v, *reinterpret_cast<const u4v*>(&P->w2[(1 * 64 + lane) * 4]));
-    float yb[2];
+    float y = 0.f;
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
         const u2v xb = (b == h) ? own : other;
@@ -151,9 +154,10 @@ __device__ __forceinline__ float mlp_wave(const float* x, const MlpPack* P) {
         float part = 0.f;
 #pragma unroll
         for (int r = 0; r < 16; ++r) part = fmaf(P->w3[h * 16 + r], fmaxf(acc2[r], 0.f), part);
-        yb[b] = part + __shfl_xor(part, 32) + P->b3;
+        const float yb = part + __shfl_xor(part, 32) + P->b3;
+        y = (b == h) ? yb : y;
     }
-    return h ? yb[1] : yb[0];
+    return y;
 }
 
 // Copies the pack (global) into LDS; every thread of the block calls it.

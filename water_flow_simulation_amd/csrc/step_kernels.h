@@ -655,7 +655,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
         pair_moments<R>(f, forced, fx, fy, fz, m);
         // resolved strain rate S = -3 Pi_dev / (2 rho tau), tau from the constant-Cs model
         const R tau_s = R(0.5) * (a.tau0 + sqrtf(fmaf(a.ksmag * m.qn, m.ri, a.tau0sq)));
-        const R k = R(-1.5) * m.ri / tau_s;
+        const R k = R(-1.5) * m.ri * __builtin_amdgcn_rcpf(tau_s);
         const R t3 = (m.pxx + m.pyy + m.pzz) * (R(1) / R(3));
         closure_features(k * (m.pxx - t3), k * (m.pyy - t3), k * (m.pzz - t3), k * m.pxy, k * m.pxz, k * m.pyz,
                          m.ux, m.uy, m.uz, a.geo[c], a.hcol[c - a.px * a.ny * z], a.inv_u0, feat);

@@ -134,10 +134,13 @@ struct wfs_solver {
     virtual int total(int field, double* out) = 0;
     virtual int check_finite() = 0;
     virtual int set_closure(int kind, const float* params, const uint16_t* geo, const uint8_t* hcol, double u0) = 0;
+    virtual int set_body_mask(const uint8_t* body) = 0;
+    virtual int forces(double* out) = 0;
     virtual int get_closure_field(float* c2) = 0;
     virtual int closure_samples(const uint16_t* geo, const uint8_t* hcol, int ncx, int ncy, int ncz, double kv,
                                 double u0, float* X, float* ABP, uint8_t* valid) = 0;
     int closure = WFS_CLOSURE_SMAGORINSKY;
+    int bx = 128;   // step-kernel work-group width (64..256)
 
     int alloc_buf(DevBuf& b, size_t bytes_) {
         bytes_ = (bytes_ + 255) & ~size_t(255);  // whole 256-byte blocks: fills and vector access stay in bounds
@@ -161,10 +164,10 @@ struct Solver final : wfs_solver {
     DevBuf f[2], flag, wmask, Fx, Fy, Fz, rho, ux, uy, uz, cfw, uref, uin, bedwall;
     DevBuf sums[6], partial, out_x, counter, stage;
     DevBuf geo, hcol, mlp_params, mlp_pack, c2buf;   // learned closure (allocated when first set)
+    DevBuf body;                                      // hull cells for the force budget (when set)
     float inv_u0 = 0.f;
     int cur = 0;
     R fscale = R(1);
-    int bx = 128;
     Center ctr{};   // storage centres (16-bit formats), fixed at upload
 
     T* fp(int k) { return static_cast<T*>(f[k].p); }
@@ -592,6 +595,38 @@ struct Solver final : wfs_solver {
         }
     }
 
+    int set_body_mask(const uint8_t* m) override {
+        if (!m) {
+            body = DevBuf();
+            return WFS_OK;
+        }
+        if (!body.p) {
+            int rc = alloc_buf(body, (size_t)Np);
+            if (rc) return rc;
+        }
+        return put_bytes(body, m);
+    }
+
+    int forces(double* out) override {
+        StepArgs<R> a;
+        fill_args(a);
+        const long long blocks = (Np + 255) / 256;
+        DevBuf part, sum;
+        int rc;
+        if ((rc = alloc_buf(part, (size_t)blocks * 9 * sizeof(double))) || (rc = alloc_buf(sum, 9 * sizeof(double))))
+            return rc;
+        hipLaunchKernelGGL(k_wall_force<Codec>, dim3((unsigned)blocks), dim3(256), 0, stream, fp(cur), a,
+                           static_cast<const uint8_t*>(body.p), static_cast<double*>(part.p));
+        WFS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, stream, static_cast<const double*>(part.p), blocks, 9,
+                           static_cast<double*>(sum.p));
+        WFS_LAUNCH_CHECK();
+        WFS_HIP(hipMemcpyAsync(out, sum.p, 9 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        WFS_HIP(hipStreamSynchronize(stream));
+        bytes -= part.bytes + sum.bytes;
+        return WFS_OK;
+    }
+
     int get_closure_field(float* out) override {
         if (!c2buf.p) return fail(WFS_EINVAL, "wfs_get_closure_field: no learned closure set");
         return get_cells(c2buf, out, false);
@@ -674,6 +709,7 @@ struct Batch final : BatchBase {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool dirty_sched = true;
+    int bx = 128;                        // work-group width fixed when the batch is made
 
     ~Batch() override {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -716,6 +752,7 @@ struct Batch final : BatchBase {
             sched.push_back(sc);
         }
         groups = (unsigned)gcase.size();
+        bx = bx0;
         int rc;
         if ((rc = cases[0]->alloc_buf(d_group_case, gcase.size() * sizeof(unsigned short)))) return rc;
         if ((rc = cases[0]->alloc_buf(d_sched, sched.size() * sizeof(CaseSched<R>)))) return rc;
@@ -760,7 +797,7 @@ struct Batch final : BatchBase {
         const CaseArgs<Codec>* da = static_cast<const CaseArgs<Codec>*>(d_args.p);
         int par = parity0;
         for (int s = 0; s < m; ++s) {
-            hipLaunchKernelGGL(k_step_batch<Codec>, dim3(groups), dim3(cases[0]->bx), 0, stream, da,
+            hipLaunchKernelGGL(k_step_batch<Codec>, dim3(groups), dim3(bx), 0, stream, da,
                                static_cast<const unsigned short*>(d_group_case.p), par, s == m - 1 ? 1 : 0);
             WFS_LAUNCH_CHECK();
             for (size_t k = 0; k < cases.size(); ++k) {
@@ -928,14 +965,21 @@ int wfs_create(const wfs_desc* desc, wfs_solver** out) {
     WFS_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     int rc = h->alloc();
     if (rc) return rc;
-    if (auto* s = dynamic_cast<Solver<wfs::CodecF32>*>(h.get())) s->bx = bx;
-    if (auto* s = dynamic_cast<Solver<wfs::CodecF16Dev>*>(h.get())) s->bx = bx;
-    if (auto* s = dynamic_cast<Solver<wfs::CodecBF16Dev>*>(h.get())) s->bx = bx;
-    if (auto* s = dynamic_cast<Solver<wfs::CodecF64>*>(h.get())) s->bx = bx;
-    if (auto* s = dynamic_cast<Solver<wfs::CodecF32Fast>*>(h.get())) s->bx = bx;
+    h->bx = bx;
     *out = h.release();
     return WFS_OK;
 }
+
+int wfs_set_block_x(wfs_solver* h, int32_t block_x) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    if (block_x < 64 || block_x > 256 || block_x % 64)
+        return fail(WFS_EINVAL, "wfs_set_block_x: block_x must be 64, 128, 192 or 256");
+    h->bx = block_x;
+    return WFS_OK;
+}
+
+int32_t wfs_block_x(const wfs_solver* h) { return h ? h->bx : -1; }
 
 int wfs_destroy(wfs_solver* h) {
     if (!h) return WFS_OK;
@@ -1280,6 +1324,21 @@ int wfs_set_closure(wfs_solver* h, int32_t kind, const float* params, const uint
     if (rc) return rc;
     DeviceGuard g(h->device);
     return h->set_closure(kind, params, geo, hcol, u0);
+}
+
+int wfs_set_body_mask(wfs_solver* h, const uint8_t* body) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    return h->set_body_mask(body);
+}
+
+int wfs_forces(wfs_solver* h, double* out) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!out) return fail(WFS_EINVAL, "wfs_forces: null output");
+    DeviceGuard g(h->device);
+    return h->forces(out);
 }
 
 int wfs_get_closure_field(wfs_solver* h, float* c2) {

@@ -251,6 +251,32 @@ class LBM3D:
         nat.check(nat.lib().wfs_timing_read(self.handle, ctypes.byref(ms), ctypes.byref(n)))
         return float(ms.value), int(n.value)
 
+    # ------------------------------------------------------------ momentum budget
+    def set_body_mask(self, body):
+        """Mark the solid cells (e.g. hulls) whose force forces() reports apart."""
+        b = np.ascontiguousarray(np.asarray(body).ravel(), dtype=np.uint8)
+        if b.size != self.N:
+            raise TypeError("set_body_mask: mask must have the shape of solid")
+        nat.check(nat.lib().wfs_set_body_mask(self.handle, nat.ptr(b)))
+
+    def forces(self):
+        """FP64 momentum budget of the next step in lattice units: applied
+        thrust, force on the marked body cells, force on the other walls
+        (each a length-3 array)."""
+        out = np.zeros(9)
+        nat.check(nat.lib().wfs_forces(self.handle, nat.ptr(out)))
+        return {"thrust": out[0:3].copy(), "body": out[3:6].copy(), "walls": out[6:9].copy()}
+
+    # ------------------------------------------------------------ launch shape
+    @property
+    def block_x(self):
+        """Work-group width of the step kernel (cells along x per work-group)."""
+        return int(nat.lib().wfs_block_x(self.handle))
+
+    @block_x.setter
+    def block_x(self, v):
+        nat.check(nat.lib().wfs_set_block_x(self.handle, int(v)))
+
     # ------------------------------------------------------------ sub-grid closure
     def set_closure(self, kind="smagorinsky", params=None, geo=None, hcol=None, u0=None):
         """Select the sub-grid closure: "smagorinsky" (the reference's constant

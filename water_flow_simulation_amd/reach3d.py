@@ -452,8 +452,12 @@ class Run3D:
 
     def __init__(self, reach: Reach, be=None, *, storage="f32", block_x=0, device=0, closure=None):
         """closure: None for the constant-Cs Smagorinsky model, or the weights
-        of the learned closure (closure.py), evaluated inside the step."""
+        of the learned closure (closure.py), evaluated inside the step.
+        block_x: work-group width, 0 for the default, "auto" for the tuned one."""
         self.r, self.be = reach, be
+        if block_x == "auto":
+            from .autotune import block_x_for
+            block_x = block_x_for(reach, storage, closure)
         nu_lb = max(1e-6 * reach.dt / reach.dx ** 2, 2e-4)
         uin_lb = (reach.uin_phys / reach.vel).astype(np.float32)
         self.lbm = LBM3D(be, reach.solid, reach.Fx, reach.Fy, reach.Fz, uin_lb, tau0=0.5 + 3 * nu_lb,
@@ -550,6 +554,22 @@ class Run3D:
         """One plane of a field (axis 0 = z, 1 = y, 2 = x) in physical units."""
         a = self.field(name, mean)
         return a[idx] if axis == 0 else (a[:, idx, :] if axis == 1 else a[:, :, idx])
+
+    def hull_mask(self):
+        """Solid cells that are not part of the bed/bank cross-section (the hulls)."""
+        s = self.r.solid
+        return s & ~np.broadcast_to(s[:, :, :1], s.shape)
+
+    def forces(self):
+        """FP64 momentum budget of the current state in newtons: applied
+        thrust, force of the water on the hulls, and on the bed and banks
+        (x, y, z each; lattice momentum per step times rho dx^4 / dt^2)."""
+        if not getattr(self, "_body_set", False):
+            self.lbm.set_body_mask(self.hull_mask())
+            self._body_set = True
+        f = self.lbm.forces()
+        k = RHO * self.r.dx ** 4 / self.r.dt ** 2
+        return {name: v * k for name, v in f.items()}
 
     def close(self):
         self.lbm.close()
