@@ -37,6 +37,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
+#include <type_traits>
+
 #include "lattice.h"
 #include "closure.h"
 
@@ -175,6 +177,20 @@ __device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int
 template <int N, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
     static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Element idx of an array smaller than 4 GiB: a uniform base plus a 32-bit
+// byte offset, the global-memory form with the base in scalar registers (no
+// per-lane 64-bit address arithmetic). wfs_create keeps every per-direction
+// population array and every cell array below 4 GiB.
+template <typename T>
+__device__ __forceinline__ T* elem(T* base, uint32_t idx) {
+    using Byte = std::conditional_t<std::is_const<T>::value, const char, char>;
+    return reinterpret_cast<T*>(reinterpret_cast<Byte*>(base) + (uint32_t)(idx * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ T& at(T* base, int idx) {
+    return *elem(base, (uint32_t)idx);
 }
 
 template <typename R>
@@ -388,7 +404,7 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
         static_for<Q>([&](auto I) {
             constexpr int i = I;
             const int s = c - kCX[i] - px * kCY[i] - plane * kCZ[i];
-            f[i] = Codec::template ld<i>(fs + (long long)i * N + s, a.ctr.c[center_class(i)]);
+            f[i] = Codec::template ld<i>(elem(fs + (long long)i * N, (uint32_t)s), a.ctr.c[center_class(i)]);
         });
     } else {
         static_for<Q>([&](auto I) {
@@ -414,13 +430,13 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
             constexpr int ki = center_class(i), ko = center_class(kOPP[i]);
             const float e_i = a.ctr.c[ki];
             const float e = (ki != ko && dir == kOPP[i]) ? a.ctr.c[ko] : e_i;
-            f[i] = Codec::template ld<i>(fs + (long long)dir * N + s, e);
+            f[i] = Codec::template ld<i>(elem(fs + (long long)dir * N, (uint32_t)s), e);
         });
     }
 
     // 2. wall function
     if (wm != 0u && (fl & FLAG_CF)) {
-        const R cf = a.cfw[c];
+        const R cf = at(a.cfw, c);
         int inx = 0, iny = 0, inz = 0;
         static_for<Q>([&](auto I) {
             constexpr int i = I;
@@ -434,13 +450,13 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
         const R area = sqrt(afx * afx + afy * afy + afz * afz);
         if (nn > R(0) && area > R(0)) {
             const R ex = nwx / nn, ey = nwy / nn, ez = nwz / nn;
-            const R ox = a.ux[c], oy = a.uy[c], oz = a.uz[c];   // previous step, own cell
+            const R ox = at(a.ux, c), oy = at(a.uy, c), oz = at(a.uz, c);   // previous step, own cell
             const R un = ox * ex + oy * ey + oz * ez;
             R tx = ox - un * ex, ty = oy - un * ey, tz = oz - un * ez;
             const R ut = sqrt(tx * tx + ty * ty + tz * tz);
             if (ut > R(1e-7)) {
                 tx /= ut; ty /= ut; tz /= ut;
-                const R rw = a.rho[c];
+                const R rw = at(a.rho, c);
                 R D = R(0), K = R(0);
                 static_for<Q>([&](auto I) {
                     constexpr int i = I;
@@ -451,7 +467,7 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
                     }
                 });
                 if (K > R(0)) {
-                    const R ur = a.uref[c];
+                    const R ur = at(a.uref, c);
                     const R U = ur > R(0) ? ur : ut;
                     const R us0 = (R(-2) * D - rw * cf * U * U * area) / (R(6) * rw * K);
                     const R us = fmin(fmax(us0, -ut), ut);
@@ -503,10 +519,9 @@ __device__ __forceinline__ void finish(typename Codec::T* __restrict__ fd, const
     }
 
     if (active) {
-        typename Codec::T* const out = fd + c;
         static_for<Q>([&](auto I) {
             constexpr int i = I;
-            Codec::template st<i>(out + (long long)i * N, f[i], store_key<Codec>(a.ctr, i));
+            Codec::template st<i>(elem(fd + (long long)i * N, (uint32_t)c), f[i], store_key<Codec>(a.ctr, i));
         });
     }
 }
@@ -522,12 +537,12 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
     const int c = x + a.px * row;
     // Lanes outside the row or on solid cells do no work but stay in the wave
     // for the boundary-row shuffles.
-    const uint8_t fl = inrow ? a.flag[c] : uint8_t(FLAG_SOLID);
+    const uint8_t fl = inrow ? at(a.flag, c) : uint8_t(FLAG_SOLID);
     const bool active = !(fl & FLAG_SOLID);
     R f[Q];
 
     if (active) {
-        const uint32_t wm = (fl & FLAG_NEARWALL) ? a.wmask[c] : 0u;
+        const uint32_t wm = (fl & FLAG_NEARWALL) ? at(a.wmask, c) : 0u;
         gather<Codec>(fs, a, c, x, z, fl, wm, f);
 
         R r, ux, uy, uz;
@@ -547,7 +562,7 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
             const bool forced = (fl & FLAG_FORCED) != 0;
             R fx = R(0), fy = R(0), fz = R(0);
             if (forced) {
-                fx = a.Fx[c] * a.fscale; fy = a.Fy[c] * a.fscale; fz = a.Fz[c] * a.fscale;
+                fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale;
                 ux = (mx + R(0.5) * fx) / r; uy = (my + R(0.5) * fy) / r; uz = (mz + R(0.5) * fz) / r;
             } else {
                 ux = mx / r; uy = my / r; uz = mz / r;
@@ -617,11 +632,11 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
         } else {
             const bool forced = (fl & FLAG_FORCED) != 0;
             R fx = R(0), fy = R(0), fz = R(0);
-            if (forced) { fx = a.Fx[c] * a.fscale; fy = a.Fy[c] * a.fscale; fz = a.Fz[c] * a.fscale; }
+            if (forced) { fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale; }
             collide_pairs<R>(f, forced, fx, fy, fz, a.tau0, a.tau0sq, a.ksmag, r, ux, uy, uz);
         }
         if (a.write_all || (wm != 0u && (fl & FLAG_CF))) {
-            a.rho[c] = r; a.ux[c] = ux; a.uy[c] = uy; a.uz[c] = uz;
+            at(a.rho, c) = r; at(a.ux, c) = ux; at(a.uy, c) = uy; at(a.uz, c) = uz;
         }
     }
 
@@ -640,7 +655,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     const bool inrow = x < a.nx;
     const int z = row / a.ny;
     const int c = x + a.px * row;
-    const uint8_t fl = inrow ? a.flag[c] : uint8_t(FLAG_SOLID);
+    const uint8_t fl = inrow ? at(a.flag, c) : uint8_t(FLAG_SOLID);
     const bool active = !(fl & FLAG_SOLID);
     const bool forced = (fl & FLAG_FORCED) != 0;
     R f[Q];
@@ -649,25 +664,25 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     uint32_t wm = 0u;
     float feat[kNF] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (active) {
-        wm = (fl & FLAG_NEARWALL) ? a.wmask[c] : 0u;
+        wm = (fl & FLAG_NEARWALL) ? at(a.wmask, c) : 0u;
         gather<Codec>(fs, a, c, x, z, fl, wm, f);
-        if (forced) { fx = a.Fx[c] * a.fscale; fy = a.Fy[c] * a.fscale; fz = a.Fz[c] * a.fscale; }
+        if (forced) { fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale; }
         pair_moments<R>(f, forced, fx, fy, fz, m);
         // resolved strain rate S = -3 Pi_dev / (2 rho tau), tau from the constant-Cs model
         const R tau_s = R(0.5) * (a.tau0 + sqrtf(fmaf(a.ksmag * m.qn, m.ri, a.tau0sq)));
         const R k = R(-1.5) * m.ri * __builtin_amdgcn_rcpf(tau_s);
         const R t3 = (m.pxx + m.pyy + m.pzz) * (R(1) / R(3));
         closure_features(k * (m.pxx - t3), k * (m.pyy - t3), k * (m.pzz - t3), k * m.pxy, k * m.pxz, k * m.pyz,
-                         m.ux, m.uy, m.uz, a.geo[c], a.hcol[c - a.px * a.ny * z], a.inv_u0, feat);
+                         m.ux, m.uy, m.uz, at(a.geo, c), at(a.hcol, c - a.px * a.ny * z), a.inv_u0, feat);
     }
     const float y = mlp_wave(feat, P);
     if (active) {
         const float c2 = kC2Min + kC2Scale * softplus_f(y);
         pair_relax<R>(f, forced, fx, fy, fz, m, a.tau0, a.tau0sq, R(18) * R(1.41421356) * c2);
         if (a.write_all || (wm != 0u && (fl & FLAG_CF))) {
-            a.rho[c] = m.r; a.ux[c] = m.ux; a.uy[c] = m.uy; a.uz[c] = m.uz;
+            at(a.rho, c) = m.r; at(a.ux, c) = m.ux; at(a.uy, c) = m.uy; at(a.uz, c) = m.uz;
         }
-        if (a.write_all && a.c2out) a.c2out[c] = c2;
+        if (a.write_all && a.c2out) at(a.c2out, c) = c2;
     }
     finish<Codec>(fd, a, f, c, x, row, tile, active);
 }

@@ -960,6 +960,11 @@ int wfs_create(const wfs_desc* desc, wfs_solver** out) {
     h->px = (d.nx + 63) / 64 * 64;
     h->Np = (long long)h->px * d.ny * d.nz;
     if (h->Np >= (1ll << 31)) return fail(WFS_EINVAL, "wfs_create: padded lattice exceeds 2^31 cells");
+    // the step kernel addresses each field and each population direction with 32-bit byte offsets
+    const unsigned long long rsz = d.storage == WFS_STORE_F64 ? 8ull : 4ull;
+    if ((unsigned long long)h->Np * rsz >= (1ull << 32))
+        return fail(WFS_EINVAL, "wfs_create: one field or population direction would exceed 4 GiB "
+                                "(at most 2^30 padded cells with float fields, 2^29 with FP64 storage)");
     h->device = d.device;
     DeviceGuard g(d.device);
     WFS_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
