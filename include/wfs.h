@@ -80,6 +80,25 @@ int wfs_create(const wfs_desc* desc, wfs_solver** out);
 int wfs_set_block_x(wfs_solver* h, int32_t block_x);
 int32_t wfs_block_x(const wfs_solver* h);
 
+/* Creates a solver on the tile-sparse layout: a tile is one 64-cell x-run of
+ * a row (y, z); only tiles holding a fluid cell of the dense solid mask
+ * (u8, nx*ny*nz) are stored, so a meandering reach in a wide bounding box
+ * keeps only its channel. Same physics and the same results as wfs_create
+ * (bit for bit); wfs_upload must then be given the same solid mask. The work
+ * of one step is one wavefront per stored tile. Requires (nx - 1) % 64 != 0.
+ * Not available on this layout: the learned closure, wfs_forces, training
+ * samples, batches. No reference counterpart (the reference's reaches are
+ * straight prisms, reach3d.py:213-216). */
+int wfs_create_tiled(const wfs_desc* desc, const uint8_t* solid, wfs_solver** out);
+/* Cells held on the device (padded rows, or stored tiles x 64). */
+int64_t wfs_stored_cells(const wfs_solver* h);
+
+/* Body forces on a list of n dense cell indices (lattice units); every other
+ * cell gets zero. Replaces wfs_set_force's dense arrays when only actuator
+ * disks carry force. */
+int wfs_set_force_cells(wfs_solver* h, int64_t n, const int64_t* cells, const float* fx, const float* fy,
+                        const float* fz);
+
 /* Frees all device memory of the solver. NULL is accepted. */
 int wfs_destroy(wfs_solver* h);
 

@@ -1,0 +1,63 @@
+"""The tile-sparse layout (only the 64-cell x-runs of a row that hold water
+are stored) against the dense layout on the same geometry: the step, the
+inlet/outlet rows, the run cadence (uref, ramp, time-mean sums) and the FP64
+readouts must agree bit for bit on every fluid cell."""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from tests.helpers import channel128
+from water_flow_simulation_amd import reach3d as R
+from water_flow_simulation_amd.meander import meander_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(run):
+    sums, navg = run.lbm.sums()
+    return sums, navg, run.lbm.macro(), run.lbm.populations()
+
+
+def _compare(reach, storage, pattern):
+    dense = R.Run3D(replace(reach, layout="dense"), storage=storage)
+    tiles = R.Run3D(replace(reach, layout="tiles"), storage=storage)
+    assert tiles.lbm.stored_cells < dense.lbm.stored_cells
+    for m in pattern:
+        dense.advance(m)
+        tiles.advance(m)
+    fl = ~reach.solid.ravel()
+    s_d, n_d, m_d, p_d = _state(dense)
+    s_t, n_t, m_t, p_t = _state(tiles)
+    assert n_d == n_t
+    for k in s_d:
+        assert np.array_equal(s_d[k][fl], s_t[k][fl]), k
+    for a, b in zip(m_d, m_t):
+        assert np.array_equal(a[fl], b[fl])
+    m19 = np.tile(fl, 19)
+    assert np.array_equal(p_d[m19], p_t[m19])
+    for field in ("rho", "ux", "sum_ux"):
+        assert np.array_equal(dense.lbm.profile_x(field), tiles.lbm.profile_x(field)), field
+    assert dense.lbm.total("rho") == tiles.lbm.total("rho")
+    tiles.lbm.check_finite()
+    frac = tiles.lbm.stored_cells / dense.lbm.stored_cells
+    dense.close()
+    tiles.close()
+    return frac
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16dev"])
+def test_tiles_match_dense_on_the_parity_channel(storage):
+    print(f"case: channel128 dense vs tiles, {storage}", flush=True)
+    frac = _compare(channel128(), storage, (100, 35, 40))
+    print("stored fraction", frac, flush=True)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16dev"])
+def test_tiles_match_dense_on_a_meander(storage):
+    print(f"case: meander at scale 0.05 (625x317x50 box) dense vs tiles, {storage}", flush=True)
+    r = meander_case(scale=0.05)
+    assert r.props and r.force_cells[0].size > 0
+    frac = _compare(r, storage, (60, 40))
+    print("stored fraction", frac, flush=True)
+    assert frac < 0.75

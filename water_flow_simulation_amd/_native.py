@@ -82,6 +82,9 @@ _SIGS = {
     "wfs_batch_time": ([_P, _I32, _P, _P], ctypes.c_int),
     "wfs_bandwidth_probe": ([_I32, ctypes.c_int64, _I32, _P], ctypes.c_int),
     "wfs_device_bytes": ([_P], ctypes.c_int64),
+    "wfs_create_tiled": ([_P, _P, _P], ctypes.c_int),
+    "wfs_stored_cells": ([_P], ctypes.c_int64),
+    "wfs_set_force_cells": ([_P, ctypes.c_int64, _P, _P, _P, _P], ctypes.c_int),
     "wfs_set_body_mask": ([_P, _P], ctypes.c_int),
     "wfs_forces": ([_P, _P], ctypes.c_int),
     "wfs_set_block_x": ([_P, _I32], ctypes.c_int),

@@ -17,24 +17,55 @@
 
 namespace wfs {
 
-// dense (pitch nx) -> padded (pitch px); padding cells get `fill`.
-template <typename S, typename D>
-__global__ void k_pad(const S* __restrict__ dense, D* __restrict__ padded, int nx, int px, long long Npad, D fill) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= Npad) return;
-    const long long j = c / px;
-    const int x = int(c - j * px);
-    padded[c] = x < nx ? (D)dense[x + (long long)nx * j] : fill;
+// Where the host's dense cells (pitch nx, rows j = y + ny*z) live on the
+// device. Dense layout (tmap = tcoord = null): cell = x + px*j, rows padded to
+// px. Tile-sparse layout: tmap[(x / 64) + ntx*j] is the compact index of the
+// 64-cell tile holding x (-1: an all-solid tile that is not stored) and
+// tcoord[3t..3t+2] = (x-tile, y, z) of tile t; cell = t*64 + x % 64.
+struct CellMap {
+    int nx, px, ny, ntx;
+    const int* tmap;
+    const int* tcoord;
+};
+
+// Device cell of dense (x, row j), or -1 when its tile is not stored.
+__device__ __forceinline__ long long cell_of(const CellMap& m, int x, long long j) {
+    if (!m.tmap) return x + (long long)m.px * j;
+    const int t = m.tmap[(x >> 6) + (long long)m.ntx * j];
+    return t < 0 ? -1 : (long long)t * 64 + (x & 63);
 }
 
-// padded -> dense
+// Dense (x, row j) of device cell c; x >= nx for padding cells.
+__device__ __forceinline__ void dense_of(const CellMap& m, long long c, int& x, long long& j) {
+    if (!m.tcoord) {
+        j = c / m.px;
+        x = int(c - j * m.px);
+        return;
+    }
+    const long long t = c >> 6;
+    x = m.tcoord[3 * t] * 64 + int(c & 63);
+    j = m.tcoord[3 * t + 1] + (long long)m.ny * m.tcoord[3 * t + 2];
+}
+
+// dense -> device cells; padding cells get `fill`.
 template <typename S, typename D>
-__global__ void k_unpad(const S* __restrict__ padded, D* __restrict__ dense, int nx, int px, long long Ndense) {
+__global__ void k_pad(const S* __restrict__ dense, D* __restrict__ dev, CellMap m, long long Ndev, D fill) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Ndev) return;
+    int x;
+    long long j;
+    dense_of(m, c, x, j);
+    dev[c] = x < m.nx ? (D)dense[x + (long long)m.nx * j] : fill;
+}
+
+// device cells -> dense; cells of tiles that are not stored get `fill`.
+template <typename S, typename D>
+__global__ void k_unpad(const S* __restrict__ dev, D* __restrict__ dense, CellMap m, long long Ndense, D fill) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= Ndense) return;
-    const long long j = k / nx;
-    const int x = int(k - j * nx);
-    dense[k] = (D)padded[x + (long long)px * j];
+    const long long j = k / m.nx;
+    const long long c = cell_of(m, int(k - j * m.nx), j);
+    dense[k] = c < 0 ? fill : (D)dev[c];
 }
 
 // Equilibrium at rho = 1, u = (u0, 0, 0), evaluated in double and stored once
@@ -42,10 +73,12 @@ __global__ void k_unpad(const S* __restrict__ padded, D* __restrict__ dense, int
 // (floodsim/lbm3d.py:362-364, 377-384).
 template <class Codec>
 __global__ void k_init(typename Codec::T* __restrict__ f, const uint8_t* __restrict__ flag,
-                       const typename Codec::R* __restrict__ uin, int px, long long N, Center ctr) {
+                       const typename Codec::R* __restrict__ uin, CellMap m, long long N, Center ctr) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
-    const long long j = c / px;  // row index y + ny*z
+    int x;
+    long long j;  // row index y + ny*z
+    dense_of(m, c, x, j);
     const double u = (flag[c] & FLAG_SOLID) ? 0.0 : (double)uin[j];
     const double W[3] = {1.0 / 3, 1.0 / 18, 1.0 / 36};
     static_for<Q>([&](auto I) {
@@ -65,14 +98,17 @@ __global__ void k_init(typename Codec::T* __restrict__ f, const uint8_t* __restr
 // (x >= nx) are flagged solid.
 template <typename R>
 __global__ void k_flags(const uint8_t* __restrict__ solid, const R* __restrict__ cfw, uint8_t* __restrict__ flag,
-                        uint32_t* __restrict__ wmask, int nx, int px, int ny, int nz) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= px) return;
-    const int y = blockIdx.y, z = blockIdx.z;
-    const long long c = x + (long long)px * (y + (long long)ny * z);
+                        uint32_t* __restrict__ wmask, CellMap m, int nz, long long Ndev) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= Ndev) return;
+    int x;
+    long long j;
+    dense_of(m, c, x, j);
+    const int nx = m.nx, ny = m.ny;
+    const int y = int(j % ny), z = int(j / ny);
     uint8_t fl = 0;
     uint32_t wm = 0;
-    if (x >= nx || solid[x + (long long)nx * (y + (long long)ny * z)]) {
+    if (x >= nx || solid[x + (long long)nx * j]) {
         fl = FLAG_SOLID;
     } else {
         for (int i = 0; i < Q; ++i) {
@@ -91,6 +127,21 @@ __global__ void k_flags(const uint8_t* __restrict__ solid, const R* __restrict__
     wmask[c] = wm;
 }
 
+// Three values per listed dense cell written into device fields (forces given
+// as a cell list). Cells whose tile is not stored are skipped.
+template <typename R>
+__global__ void k_scatter3(const int64_t* __restrict__ idx, const float* __restrict__ v, long long n, CellMap m,
+                           R* __restrict__ a, R* __restrict__ b, R* __restrict__ c3) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const long long j = idx[k] / m.nx;
+    const long long c = cell_of(m, int(idx[k] - j * m.nx), j);
+    if (c < 0) return;
+    a[c] = R(v[k]);
+    b[c] = R(v[n + k]);
+    c3[c] = R(v[2 * n + k]);
+}
+
 // FLAG_FORCED where any force component is nonzero.
 template <typename R>
 __global__ void k_forced(uint8_t* __restrict__ flag, const R* __restrict__ Fx, const R* __restrict__ Fy,
@@ -107,15 +158,14 @@ __global__ void k_forced(uint8_t* __restrict__ flag, const R* __restrict__ Fx, c
 // z = 0..nz-1 in order.
 template <typename R>
 __global__ void k_uref(const R* __restrict__ ux, const R* __restrict__ uy, const uint8_t* __restrict__ flag,
-                       const uint8_t* __restrict__ bedwall, R* __restrict__ uref, int nx, int px, int ny, int nz) {
+                       const uint8_t* __restrict__ bedwall, R* __restrict__ uref, CellMap m, int nz) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
-    if (x >= nx) return;
-    const long long col = x + (long long)px * y;
-    const long long plane = (long long)px * ny;
+    if (x >= m.nx) return;
     R sx = R(0), sy = R(0), nfl = R(0);
     for (int z = 0; z < nz; ++z) {
-        const long long c = col + plane * z;
+        const long long c = cell_of(m, x, y + (long long)m.ny * z);
+        if (c < 0) continue;   // tile not stored: solid
         const R fl = (flag[c] & FLAG_SOLID) ? R(0) : R(1);
         sx += ux[c] * fl;
         sy += uy[c] * fl;
@@ -124,8 +174,8 @@ __global__ void k_uref(const R* __restrict__ ux, const R* __restrict__ uy, const
     const R ncol = nfl > R(1) ? nfl : R(1);
     const R U = sqrt(sx * sx + sy * sy) / ncol;
     for (int z = 0; z < nz; ++z) {
-        const long long c = col + plane * z;
-        uref[c] = bedwall[c] ? U : R(0);
+        const long long c = cell_of(m, x, y + (long long)m.ny * z);
+        if (c >= 0) uref[c] = bedwall[c] ? U : R(0);
     }
 }
 
@@ -219,25 +269,27 @@ __global__ void k_fill_macro(R* __restrict__ rho, R* __restrict__ ux, R* __restr
 
 // Populations: padded storage -> dense real array f[i*Nd + k], and back.
 template <class Codec, typename O>
-__global__ void k_decode(const typename Codec::T* __restrict__ f, O* __restrict__ out, int nx, int px,
+__global__ void k_decode(const typename Codec::T* __restrict__ f, O* __restrict__ out, CellMap m,
                          long long Npad, long long Nd, Center ctr) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= Nd) return;
-    const long long j = k / nx;
-    const long long c = (k - j * nx) + (long long)px * j;
+    const long long j = k / m.nx;
+    const long long c = cell_of(m, int(k - j * m.nx), j);
     static_for<Q>([&](auto I) {
         constexpr int i = I;
-        out[(long long)i * Nd + k] = (O)Codec::template ld<i>(f + (long long)i * Npad + c, ctr.c[center_class(i)]);
+        out[(long long)i * Nd + k] =
+            c < 0 ? O(0) : (O)Codec::template ld<i>(f + (long long)i * Npad + c, ctr.c[center_class(i)]);
     });
 }
 
 template <class Codec>
-__global__ void k_encode(typename Codec::T* __restrict__ f, const float* __restrict__ in, int nx, int px,
+__global__ void k_encode(typename Codec::T* __restrict__ f, const float* __restrict__ in, CellMap m,
                          long long Npad, long long Nd, Center ctr) {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= Nd) return;
-    const long long j = k / nx;
-    const long long c = (k - j * nx) + (long long)px * j;
+    const long long j = k / m.nx;
+    const long long c = cell_of(m, int(k - j * m.nx), j);
+    if (c < 0) return;
     static_for<Q>([&](auto I) {
         constexpr int i = I;
         Codec::template st<i>(f + (long long)i * Npad + c, (typename Codec::R)in[(long long)i * Nd + k],
@@ -249,19 +301,19 @@ __global__ void k_encode(typename Codec::T* __restrict__ f, const float* __restr
 // pass 1 writes one partial per (x, chunk of rows), pass 2 adds the chunks
 // in order. Deterministic (no atomics).
 template <typename R>
-__global__ void k_profile_partial(const R* __restrict__ v, const uint8_t* __restrict__ flag, int nx, int px, int j0,
+__global__ void k_profile_partial(const R* __restrict__ v, const uint8_t* __restrict__ flag, CellMap m, int j0,
                                   int j1, int rows_per_chunk, double* __restrict__ partial) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= nx) return;
+    if (x >= m.nx) return;
     const int chunk = blockIdx.y;
     const int a = j0 + chunk * rows_per_chunk;
     const int b = min(a + rows_per_chunk, j1);
     double s = 0.0;
     for (int j = a; j < b; ++j) {
-        const long long c = x + (long long)px * j;
-        if (!(flag[c] & FLAG_SOLID)) s += v ? (double)v[c] : 1.0;
+        const long long c = cell_of(m, x, j);
+        if (c >= 0 && !(flag[c] & FLAG_SOLID)) s += v ? (double)v[c] : 1.0;
     }
-    partial[(long long)chunk * nx + x] = s;
+    partial[(long long)chunk * m.nx + x] = s;
 }
 
 __global__ void k_profile_final(const double* __restrict__ partial, int nx, int nchunks, double* __restrict__ out) {

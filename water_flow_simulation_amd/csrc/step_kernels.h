@@ -225,6 +225,11 @@ struct StepArgs {
     const MlpPack* __restrict__ mlp;    // network weights in the wavefront layout
     float* __restrict__ c2out;          // C^2 per cell on write_all steps, when not null
     float inv_u0;                       // 1 / mean inlet speed (lattice units)
+    // tile-sparse layout (k_step_sparse only): a tile is one 64-cell x-run of a
+    // row; cell c = tile * 64 + lane
+    const int* __restrict__ nbr;        // per tile: 27 neighbour tiles (dx+1) + 3 ((dy+1) + 3 (dz+1)), -1 none
+    const int* __restrict__ tcoord;     // per tile: x-tile, y, z
+    const int* __restrict__ blk2tile;   // work-group -> tile (-1: idle), rows kept on one XCD
 };
 
 // Work-group -> (row, x-tile) map. Work-groups are dispatched round-robin over
@@ -284,6 +289,14 @@ __device__ __forceinline__ void boundary_row(R* f, const R* v, int side, R uin, 
 //
 // pair_moments: density, velocity (with the Guo half-force shift), the
 // non-equilibrium stress and its norm Q; pair_relax: the post-collision state.
+// Reciprocal and square root of the pair form: the hardware instructions
+// (about 1 ulp) in float, IEEE in double. The pair form is judged against
+// FP64 runs, not bit for bit, and is VALU-bound in the 16-bit formats.
+__device__ __forceinline__ float pair_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double pair_rcp(double v) { return 1.0 / v; }
+__device__ __forceinline__ float pair_sqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
+__device__ __forceinline__ double pair_sqrt(double v) { return sqrt(v); }
+
 template <typename R>
 struct PairMoments {
     R r, ri, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz, qn;
@@ -303,7 +316,7 @@ __device__ __forceinline__ void pair_moments(const R* f, bool forced, R fx, R fy
     const R mx = dp[0] + dp[3] + dp[4] + dp[5] + dp[6];
     const R my = dp[1] + dp[3] - dp[4] + dp[7] + dp[8];
     const R mz = dp[2] + dp[5] - dp[6] + dp[7] - dp[8];
-    const R ri = R(1) / r;
+    const R ri = pair_rcp(r);
     R ux, uy, uz;
     if (forced) {
         ux = fma(R(0.5), fx, mx) * ri; uy = fma(R(0.5), fy, my) * ri; uz = fma(R(0.5), fz, mz) * ri;
@@ -317,8 +330,8 @@ __device__ __forceinline__ void pair_moments(const R* f, bool forced, R fx, R fy
     m.pxy = (sp[3] - sp[4]) - rux * uy;
     m.pxz = (sp[5] - sp[6]) - rux * uz;
     m.pyz = (sp[7] - sp[8]) - ruy * uz;
-    m.qn = sqrt(fma(m.pxx, m.pxx, fma(m.pyy, m.pyy, m.pzz * m.pzz)) +
-                R(2) * fma(m.pxy, m.pxy, fma(m.pxz, m.pxz, m.pyz * m.pyz)));
+    m.qn = pair_sqrt(fma(m.pxx, m.pxx, fma(m.pyy, m.pyy, m.pzz * m.pzz)) +
+                     R(2) * fma(m.pxy, m.pxy, fma(m.pxz, m.pxz, m.pyz * m.pyz)));
     m.r = r; m.ri = ri; m.ux = ux; m.uy = uy; m.uz = uz;
 }
 
@@ -328,8 +341,8 @@ __device__ __forceinline__ void pair_relax(R* f, bool forced, R fx, R fy, R fz, 
                                            R tau0sq, R ksmag) {
     const R r = m.r, ux = m.ux, uy = m.uy, uz = m.uz;
     const R pxx = m.pxx, pyy = m.pyy, pzz = m.pzz, pxy = m.pxy, pxz = m.pxz, pyz = m.pyz;
-    const R tau = R(0.5) * (tau0 + sqrt(fma(ksmag * m.qn, m.ri, tau0sq)));
-    const R om = R(1) / tau;
+    const R tau = R(0.5) * (tau0 + pair_sqrt(fma(ksmag * m.qn, m.ri, tau0sq)));
+    const R om = pair_rcp(tau);
     const R om1 = R(1) - om;
     const R t3 = (pxx + pyy + pzz) * (R(1) / R(3));
     const R usq = fma(ux, ux, fma(uy, uy, uz * uz));
@@ -390,9 +403,32 @@ __device__ __forceinline__ void collide_pairs(R* f, bool forced, R fx, R fy, R f
 // Pull streaming (x-edge own copy, rigid-lid mirror, bounce-back on wall
 // links) and the wall function for one active cell: f gets the populations
 // entering the collision.
-template <class Codec>
+//
+// Pull source of direction (cx, cy, cz) for the cell: in the dense layout a
+// fixed offset; in the tile-sparse layout the source tile comes from the
+// tile's neighbour table (nb, wavefront-uniform) and only the first/last
+// lane crosses into the x-neighbour tile.
+template <bool kSparse, int cx, int cy, int cz>
+__device__ __forceinline__ int pull_source(int c, int px, int plane, const int* nb, int lane) {
+    if constexpr (!kSparse) {
+        return c - cx - px * cy - plane * cz;
+    } else {
+        constexpr int kyz = 3 * ((1 - cy) + 3 * (1 - cz));
+        const int ts = nb[1 + kyz];
+        if constexpr (cx == 0) {
+            return ts * 64 + lane;
+        } else {
+            const int td = nb[(cx > 0 ? 0 : 2) + kyz];
+            const bool cross = cx > 0 ? lane == 0 : lane == 63;
+            return cross ? td * 64 + ((lane - cx) & 63) : ts * 64 + lane - cx;
+        }
+    }
+}
+
+template <class Codec, bool kSparse = false>
 __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs, const StepArgs<typename Codec::R>& a,
-                                       int c, int x, int z, uint8_t fl, uint32_t wm, typename Codec::R* f) {
+                                       int c, int x, int z, uint8_t fl, uint32_t wm, typename Codec::R* f,
+                                       const int* nb = nullptr, int lane = 0) {
     using R = typename Codec::R;
     const int px = a.px;
     const int plane = px * a.ny;
@@ -403,7 +439,7 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
         // interior: every source is the plain upstream neighbour
         static_for<Q>([&](auto I) {
             constexpr int i = I;
-            const int s = c - kCX[i] - px * kCY[i] - plane * kCZ[i];
+            const int s = pull_source<kSparse, kCX[i], kCY[i], kCZ[i]>(c, px, plane, nb, lane);
             f[i] = Codec::template ld<i>(elem(fs + (long long)i * N, (uint32_t)s), a.ctr.c[center_class(i)]);
         });
     } else {
@@ -421,9 +457,9 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
                 s = c;
             } else if (cz == -1 && top) {
                 dir = kMIRZ[i];
-                s = c - cx - px * cy;
+                s = pull_source<kSparse, cx, cy, 0>(c, px, plane, nb, lane);
             } else {
-                s = c - cx - px * cy - plane * cz;
+                s = pull_source<kSparse, cx, cy, cz>(c, px, plane, nb, lane);
             }
                 // decode with the centre of the direction actually read
             // (the mirror keeps c_x, so only the opposite direction has another centre)
@@ -488,14 +524,14 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
 // wavefront calls this; inactive lanes only take part in the shifts).
 template <class Codec>
 __device__ __forceinline__ void finish(typename Codec::T* __restrict__ fd, const StepArgs<typename Codec::R>& a,
-                                       typename Codec::R* f, int c, int x, int row, int tile, bool active) {
+                                       typename Codec::R* f, int c, int x, int row, int xw, bool active) {
     using R = typename Codec::R;
     const long long N = a.N;
     // Inlet and outlet rows, fused: the boundary lane takes its neighbour's
     // post-collision populations (rounded through the storage format, as a
     // separate pass would read them) by a cross-lane shift. The outlet is done
     // here when x = nx-2 and nx-1 share a wavefront; otherwise by k_bc.
-    const int xw = tile * int(blockDim.x) + (int(threadIdx.x) & ~63);   // first x of this wavefront
+    // xw: first x of this wavefront.
     const bool in_wave = xw == 0;
     const bool out_wave = a.bc_out_fused && xw <= a.nx - 1 && a.nx - 1 < xw + 64;
     if (in_wave || out_wave) {
@@ -527,14 +563,20 @@ __device__ __forceinline__ void finish(typename Codec::T* __restrict__ fd, const
 }
 
 // One x-tile of one row: the whole step for its cells.
-template <class Codec>
+// Dense layout: tile = x-tile of blockDim.x cells of the row, t unused.
+// Tile-sparse layout: a work-group is one wavefront, tile = its x-tile
+// (64 cells) and t its index in the compact storage.
+template <class Codec, bool kSparse = false>
 __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ fs, typename Codec::T* __restrict__ fd,
-                                          const StepArgs<typename Codec::R>& a, int row, int tile) {
+                                          const StepArgs<typename Codec::R>& a, int row, int tile, int t = 0) {
     using R = typename Codec::R;
-    const int x = tile * int(blockDim.x) + int(threadIdx.x);
+    const int lane = int(threadIdx.x) & 63;
+    const int x = kSparse ? tile * 64 + lane : tile * int(blockDim.x) + int(threadIdx.x);
+    const int xw = kSparse ? tile * 64 : tile * int(blockDim.x) + (int(threadIdx.x) & ~63);
     const bool inrow = x < a.nx;
     const int z = row / a.ny;
-    const int c = x + a.px * row;
+    const int c = kSparse ? t * 64 + lane : x + a.px * row;
+    const int* nb = kSparse ? a.nbr + 27 * t : nullptr;
     // Lanes outside the row or on solid cells do no work but stay in the wave
     // for the boundary-row shuffles.
     const uint8_t fl = inrow ? at(a.flag, c) : uint8_t(FLAG_SOLID);
@@ -543,7 +585,7 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
 
     if (active) {
         const uint32_t wm = (fl & FLAG_NEARWALL) ? at(a.wmask, c) : 0u;
-        gather<Codec>(fs, a, c, x, z, fl, wm, f);
+        gather<Codec, kSparse>(fs, a, c, x, z, fl, wm, f, nb, lane);
 
         R r, ux, uy, uz;
         if constexpr (Codec::kExact) {
@@ -640,7 +682,7 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
         }
     }
 
-    finish<Codec>(fd, a, f, c, x, row, tile, active);
+    finish<Codec>(fd, a, f, c, x, row, xw, active);
 }
 
 // The step with the learned closure (float arithmetic, pair-form collision).
@@ -652,6 +694,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
                                                   int row, int tile, const MlpPack* P) {
     using R = float;
     const int x = tile * int(blockDim.x) + int(threadIdx.x);
+    const int xw = tile * int(blockDim.x) + (int(threadIdx.x) & ~63);
     const bool inrow = x < a.nx;
     const int z = row / a.ny;
     const int c = x + a.px * row;
@@ -669,7 +712,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
         if (forced) { fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale; }
         pair_moments<R>(f, forced, fx, fy, fz, m);
         // resolved strain rate S = -3 Pi_dev / (2 rho tau), tau from the constant-Cs model
-        const R tau_s = R(0.5) * (a.tau0 + sqrtf(fmaf(a.ksmag * m.qn, m.ri, a.tau0sq)));
+        const R tau_s = R(0.5) * (a.tau0 + pair_sqrt(fmaf(a.ksmag * m.qn, m.ri, a.tau0sq)));
         const R k = R(-1.5) * m.ri * __builtin_amdgcn_rcpf(tau_s);
         const R t3 = (m.pxx + m.pyy + m.pzz) * (R(1) / R(3));
         closure_features(k * (m.pxx - t3), k * (m.pyy - t3), k * (m.pzz - t3), k * m.pxy, k * m.pxz, k * m.pyz,
@@ -684,7 +727,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
         }
         if (a.write_all && a.c2out) at(a.c2out, c) = c2;
     }
-    finish<Codec>(fd, a, f, c, x, row, tile, active);
+    finish<Codec>(fd, a, f, c, x, row, xw, active);
 }
 
 template <class Codec>
@@ -703,6 +746,17 @@ __global__ void __launch_bounds__(256) k_step_learned(const typename Codec::T* _
     if (!tile_of(blockIdx.x, a.tiles, a.rows, row, tile)) return;   // whole work-group
     stage_pack(a.mlp, &P);
     step_tile_learned<Codec>(fs, fd, a, row, tile, &P);
+}
+
+// The step on the tile-sparse layout: one wavefront per work-group, one
+// active tile per wavefront (blk2tile keeps every tile of a row on one XCD).
+template <class Codec>
+__global__ void __launch_bounds__(64) k_step_sparse(const typename Codec::T* __restrict__ fs,
+                                                    typename Codec::T* __restrict__ fd, StepArgs<typename Codec::R> a) {
+    const int t = a.blk2tile[blockIdx.x];
+    if (t < 0) return;   // whole work-group
+    const int tx = a.tcoord[3 * t], y = a.tcoord[3 * t + 1], z = a.tcoord[3 * t + 2];
+    step_tile<Codec, true>(fs, fd, a, y + a.ny * z, tx, t);
 }
 
 // ---------------------------------------------------------------- many reaches in one launch

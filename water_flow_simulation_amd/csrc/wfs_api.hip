@@ -3,6 +3,7 @@
 // include/wfs.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -141,6 +142,13 @@ struct wfs_solver {
                                 double u0, float* X, float* ABP, uint8_t* valid) = 0;
     int closure = WFS_CLOSURE_SMAGORINSKY;
     int bx = 128;   // step-kernel work-group width (64..256)
+    // tile-sparse layout: only the 64-cell x-runs of a row that hold fluid are stored
+    bool tiled = false;
+    int ntx = 0;                                         // x-tiles per row (px / 64)
+    long long T = 0;                                     // stored tiles
+    std::vector<int> h_tmap, h_tcoord, h_nbr, h_blk2tile;
+    virtual int set_force_cells(long long n, const int64_t* idx, const float* fx, const float* fy,
+                                const float* fz) = 0;
 
     int alloc_buf(DevBuf& b, size_t bytes_) {
         bytes_ = (bytes_ + 255) & ~size_t(255);  // whole 256-byte blocks: fills and vector access stay in bounds
@@ -165,6 +173,18 @@ struct Solver final : wfs_solver {
     DevBuf sums[6], partial, out_x, counter, stage;
     DevBuf geo, hcol, mlp_params, mlp_pack, c2buf;   // learned closure (allocated when first set)
     DevBuf body;                                      // hull cells for the force budget (when set)
+    DevBuf tmap_d, tcoord_d, nbr_d, blk2tile_d;       // tile-sparse tables (when tiled)
+
+    CellMap cmap() const {
+        CellMap m;
+        m.nx = d.nx;
+        m.px = px;
+        m.ny = d.ny;
+        m.ntx = ntx;
+        m.tmap = tiled ? static_cast<const int*>(tmap_d.p) : nullptr;
+        m.tcoord = tiled ? static_cast<const int*>(tcoord_d.p) : nullptr;
+        return m;
+    }
     float inv_u0 = 0.f;
     int cur = 0;
     R fscale = R(1);
@@ -191,6 +211,14 @@ struct Solver final : wfs_solver {
         if ((rc = alloc_buf(partial, (size_t)d.nx * 256 * sizeof(double)))) return rc;
         if ((rc = alloc_buf(counter, 256))) return rc;
         if ((rc = alloc_buf(stage, (size_t)N * sizeof(double)))) return rc;  // dense host <-> device staging
+        if (tiled) {
+            for (auto pr : {std::make_pair(&tmap_d, &h_tmap), std::make_pair(&tcoord_d, &h_tcoord),
+                            std::make_pair(&nbr_d, &h_nbr), std::make_pair(&blk2tile_d, &h_blk2tile)}) {
+                if ((rc = alloc_buf(*pr.first, pr.second->size() * sizeof(int)))) return rc;
+                WFS_HIP(hipMemcpy(pr.first->p, pr.second->data(), pr.second->size() * sizeof(int),
+                                  hipMemcpyHostToDevice));
+            }
+        }
         return WFS_OK;
     }
 
@@ -200,7 +228,7 @@ struct Solver final : wfs_solver {
     int put_cells(DevBuf& b, const float* src) {
         WFS_HIP(hipMemcpyAsync(stage.p, src, (size_t)N * sizeof(float), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL((k_pad<float, R>), dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
-                           static_cast<const float*>(stage.p), rp(b), d.nx, px, Np, R(0));
+                           static_cast<const float*>(stage.p), rp(b), cmap(), Np, R(0));
         WFS_LAUNCH_CHECK();
         WFS_HIP(hipStreamSynchronize(stream));
         return WFS_OK;
@@ -210,8 +238,7 @@ struct Solver final : wfs_solver {
     int put_bytes(DevBuf& b, const uint8_t* src) {
         WFS_HIP(hipMemcpyAsync(stage.p, src, (size_t)N, hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL((k_pad<uint8_t, uint8_t>), dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
-                           static_cast<const uint8_t*>(stage.p), static_cast<uint8_t*>(b.p), d.nx, px, Np,
-                           uint8_t(0));
+                           static_cast<const uint8_t*>(stage.p), static_cast<uint8_t*>(b.p), cmap(), Np, uint8_t(0));
         WFS_LAUNCH_CHECK();
         WFS_HIP(hipStreamSynchronize(stream));
         return WFS_OK;
@@ -226,15 +253,15 @@ struct Solver final : wfs_solver {
         return WFS_OK;
     }
 
-    // device padded R cells -> host dense float or double cells
-    int get_cells(DevBuf& b, void* dst, bool f64) {
+    // device R cells -> host dense float or double cells (cells of tiles that are not stored: fill)
+    int get_cells(DevBuf& b, void* dst, bool f64, double fill = 0.0) {
         if (!dst) return WFS_OK;
         if (f64)
             hipLaunchKernelGGL((k_unpad<R, double>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(b),
-                               static_cast<double*>(stage.p), d.nx, px, N);
+                               static_cast<double*>(stage.p), cmap(), N, fill);
         else
             hipLaunchKernelGGL((k_unpad<R, float>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(b),
-                               static_cast<float*>(stage.p), d.nx, px, N);
+                               static_cast<float*>(stage.p), cmap(), N, (float)fill);
         WFS_LAUNCH_CHECK();
         WFS_HIP(hipMemcpyAsync(dst, stage.p, (size_t)N * (f64 ? sizeof(double) : sizeof(float)),
                                hipMemcpyDeviceToHost, stream));
@@ -262,13 +289,13 @@ struct Solver final : wfs_solver {
         set_center(uin_h);
         // the dense solid mask stays in the staging buffer while the flags are built
         WFS_HIP(hipMemcpyAsync(stage.p, solid, (size_t)N, hipMemcpyHostToDevice, stream));
-        const dim3 gx(blocks_for(px, 64), d.ny, d.nz);
-        hipLaunchKernelGGL(k_flags<R>, gx, dim3(64), 0, stream, static_cast<const uint8_t*>(stage.p), rp(cfw),
-                           static_cast<uint8_t*>(flag.p), static_cast<uint32_t*>(wmask.p), d.nx, px, d.ny, d.nz);
+        hipLaunchKernelGGL(k_flags<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
+                           static_cast<const uint8_t*>(stage.p), rp(cfw), static_cast<uint8_t*>(flag.p),
+                           static_cast<uint32_t*>(wmask.p), cmap(), d.nz, Np);
         WFS_LAUNCH_CHECK();
         for (int k = 0; k < 2; ++k) {
             hipLaunchKernelGGL(k_init<Codec>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, fp(k), flags(),
-                               rp(uin), px, Np, ctr);
+                               rp(uin), cmap(), Np, ctr);
             WFS_LAUNCH_CHECK();
         }
         hipLaunchKernelGGL(k_fill_macro<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, rp(rho), rp(ux),
@@ -319,6 +346,32 @@ struct Solver final : wfs_solver {
         return WFS_OK;
     }
 
+    int set_force_cells(long long n, const int64_t* idx, const float* fx, const float* fy,
+                        const float* fz) override {
+        int rc;
+        for (DevBuf* b : {&Fx, &Fy, &Fz})
+            if ((rc = zero(*b))) return rc;
+        if (n > 0) {
+            DevBuf di, dv;
+            if ((rc = alloc_buf(di, (size_t)n * sizeof(int64_t))) || (rc = alloc_buf(dv, (size_t)n * 3 * sizeof(float))))
+                return rc;
+            WFS_HIP(hipMemcpyAsync(di.p, idx, (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+            float* v = static_cast<float*>(dv.p);
+            WFS_HIP(hipMemcpyAsync(v, fx, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
+            WFS_HIP(hipMemcpyAsync(v + n, fy, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
+            WFS_HIP(hipMemcpyAsync(v + 2 * n, fz, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
+            hipLaunchKernelGGL(k_scatter3<R>, dim3(blocks_for(n, 256)), dim3(256), 0, stream,
+                               static_cast<const int64_t*>(di.p), v, n, cmap(), rp(Fx), rp(Fy), rp(Fz));
+            WFS_LAUNCH_CHECK();
+            WFS_HIP(hipStreamSynchronize(stream));
+            bytes -= di.bytes + dv.bytes;
+        }
+        hipLaunchKernelGGL(k_forced<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
+                           static_cast<uint8_t*>(flag.p), rp(Fx), rp(Fy), rp(Fz), Np);
+        WFS_LAUNCH_CHECK();
+        return WFS_OK;
+    }
+
     int set_force_scale(float s) override {
         fscale = R(s);
         return WFS_OK;
@@ -337,7 +390,7 @@ struct Solver final : wfs_solver {
     int update_uref() override {
         if (!have_bedwall) return fail(WFS_EINVAL, "wfs_update_uref: no bed/bank mask set (wfs_set_bedwall)");
         hipLaunchKernelGGL(k_uref<R>, dim3(blocks_for(d.nx, 64), d.ny), dim3(64), 0, stream, rp(ux), rp(uy), flags(),
-                           static_cast<const uint8_t*>(bedwall.p), rp(uref), d.nx, px, d.ny, d.nz);
+                           static_cast<const uint8_t*>(bedwall.p), rp(uref), cmap(), d.nz);
         WFS_LAUNCH_CHECK();
         return WFS_OK;
     }
@@ -377,10 +430,18 @@ struct Solver final : wfs_solver {
         a.mlp = static_cast<const MlpPack*>(mlp_pack.p);
         a.c2out = static_cast<float*>(c2buf.p);
         a.inv_u0 = inv_u0;
+        a.nbr = static_cast<const int*>(nbr_d.p);
+        a.tcoord = static_cast<const int*>(tcoord_d.p);
+        a.blk2tile = static_cast<const int*>(blk2tile_d.p);
     }
 
     // One step launch: the constant-Cs kernel or, with the learned closure, k_step_learned.
     void launch_step(const dim3& grid, const StepArgs<R>& a) {
+        if (tiled) {
+            hipLaunchKernelGGL(k_step_sparse<Codec>, dim3((unsigned)h_blk2tile.size()), dim3(64), 0, stream,
+                               fp(cur), fp(cur ^ 1), a);
+            return;
+        }
         if constexpr (std::is_same<R, float>::value) {
             if (closure == WFS_CLOSURE_MLP) {
                 hipLaunchKernelGGL(k_step_learned<Codec>, grid, dim3(bx), 0, stream, fp(cur), fp(cur ^ 1), a);
@@ -421,7 +482,7 @@ struct Solver final : wfs_solver {
 
     int get_macro(void* r, void* a, void* b, void* c, bool f64) override {
         int rc;
-        if ((rc = get_cells(rho, r, f64)) || (rc = get_cells(ux, a, f64)) || (rc = get_cells(uy, b, f64)) ||
+        if ((rc = get_cells(rho, r, f64, 1.0)) || (rc = get_cells(ux, a, f64)) || (rc = get_cells(uy, b, f64)) ||
             (rc = get_cells(uz, c, f64)))
             return rc;
         return WFS_OK;
@@ -447,10 +508,10 @@ struct Solver final : wfs_solver {
         }
         if (f64)
             hipLaunchKernelGGL((k_decode<Codec, double>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
-                               static_cast<double*>(tmp.p), d.nx, px, Np, N, ctr);
+                               static_cast<double*>(tmp.p), cmap(), Np, N, ctr);
         else
             hipLaunchKernelGGL((k_decode<Codec, float>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
-                               static_cast<float*>(tmp.p), d.nx, px, Np, N, ctr);
+                               static_cast<float*>(tmp.p), cmap(), Np, N, ctr);
         WFS_LAUNCH_CHECK();
         WFS_HIP(hipMemcpyAsync(out, tmp.p, n * esz, hipMemcpyDeviceToHost, stream));
         WFS_HIP(hipStreamSynchronize(stream));
@@ -467,7 +528,7 @@ struct Solver final : wfs_solver {
         }
         WFS_HIP(hipMemcpyAsync(tmp.p, in, n * sizeof(float), hipMemcpyHostToDevice, stream));
         hipLaunchKernelGGL(k_encode<Codec>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
-                           static_cast<const float*>(tmp.p), d.nx, px, Np, N, ctr);
+                           static_cast<const float*>(tmp.p), cmap(), Np, N, ctr);
         WFS_LAUNCH_CHECK();
         WFS_HIP(hipStreamSynchronize(stream));
         return WFS_OK;
@@ -524,7 +585,7 @@ struct Solver final : wfs_solver {
         const int per = (nrows + nchunks - 1) / nchunks;
         const int used = (nrows + per - 1) / per;
         hipLaunchKernelGGL(k_profile_partial<R>, dim3(blocks_for(d.nx, 64), used), dim3(64), 0, stream, v, flags(),
-                           d.nx, px, j0, j1, per, static_cast<double*>(partial.p));
+                           cmap(), j0, j1, per, static_cast<double*>(partial.p));
         WFS_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_profile_final, dim3(blocks_for(d.nx, 64)), dim3(64), 0, stream,
                            static_cast<const double*>(partial.p), d.nx, used, static_cast<double*>(out_x.p));
@@ -557,6 +618,7 @@ struct Solver final : wfs_solver {
             return WFS_OK;
         }
         if (kind != WFS_CLOSURE_MLP) return fail(WFS_EINVAL, "wfs_set_closure: unknown closure kind");
+        if (tiled) return fail(WFS_EINVAL, "wfs_set_closure: the learned closure runs on the dense layout");
         if constexpr (!std::is_same<R, float>::value) {
             return fail(WFS_EINVAL, "wfs_set_closure: the learned closure runs in float arithmetic (not FP64 storage)");
         } else {
@@ -573,14 +635,15 @@ struct Solver final : wfs_solver {
             }
             WFS_HIP(hipMemcpyAsync(stage.p, geo_h, (size_t)N * sizeof(uint16_t), hipMemcpyHostToDevice, stream));
             hipLaunchKernelGGL((k_pad<uint16_t, uint16_t>), dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
-                               static_cast<const uint16_t*>(stage.p), static_cast<uint16_t*>(geo.p), d.nx, px, Np,
+                               static_cast<const uint16_t*>(stage.p), static_cast<uint16_t*>(geo.p), cmap(), Np,
                                uint16_t(0));
             WFS_LAUNCH_CHECK();
             WFS_HIP(hipStreamSynchronize(stream));
             const long long ncol = (long long)px * d.ny;
             WFS_HIP(hipMemcpyAsync(stage.p, hcol_h, (size_t)d.nx * d.ny, hipMemcpyHostToDevice, stream));
+            const CellMap cm{d.nx, px, d.ny, 0, nullptr, nullptr};   // per column: rows of the (x, y) plane
             hipLaunchKernelGGL((k_pad<uint8_t, uint8_t>), dim3(blocks_for(ncol, 256)), dim3(256), 0, stream,
-                               static_cast<const uint8_t*>(stage.p), static_cast<uint8_t*>(hcol.p), d.nx, px, ncol,
+                               static_cast<const uint8_t*>(stage.p), static_cast<uint8_t*>(hcol.p), cm, ncol,
                                uint8_t(0));
             WFS_LAUNCH_CHECK();
             WFS_HIP(hipMemcpyAsync(mlp_params.p, params, kNParams * sizeof(float), hipMemcpyHostToDevice, stream));
@@ -608,6 +671,7 @@ struct Solver final : wfs_solver {
     }
 
     int forces(double* out) override {
+        if (tiled) return fail(WFS_EINVAL, "wfs_forces: runs on the dense layout");
         StepArgs<R> a;
         fill_args(a);
         const long long blocks = (Np + 255) / 256;
@@ -636,6 +700,7 @@ struct Solver final : wfs_solver {
     // coarse grid ncx x ncy x ncz; outputs are device arrays (dense coarse order).
     int closure_samples(const uint16_t* geo_h, const uint8_t* hcol_h, int ncx, int ncy, int ncz, double kv, double u0,
                         float* X, float* ABP, uint8_t* valid) override {
+        if (tiled) return fail(WFS_EINVAL, "wfs_closure_samples: runs on the dense layout");
         if (ncx < 3 || ncy < 3 || ncz < 2 || 2 * ncx > d.nx || 2 * ncy > d.ny || 2 * ncz > d.nz)
             return fail(WFS_EINVAL, "wfs_closure_samples: coarse grid must be at most half the fine grid");
         if (!(u0 > 0.0) || !(kv > 0.0)) return fail(WFS_EINVAL, "wfs_closure_samples: u0 and kv must be positive");
@@ -726,6 +791,7 @@ struct Batch final : BatchBase {
             if (c->device != device) return fail(WFS_EINVAL, "wfs_batch_create: cases must be on one device");
             if (bx0 < 0) bx0 = c->bx;
             if (c->bx != bx0) return fail(WFS_EINVAL, "wfs_batch_create: cases must share block_x");
+            if (c->tiled) return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the dense layout");
             if (c->closure != WFS_CLOSURE_SMAGORINSKY)
                 return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the constant-Cs closure");
             WFS_HIP(hipStreamSynchronize(c->stream));
@@ -791,7 +857,7 @@ struct Batch final : BatchBase {
             if (!c->have_bedwall) continue;
             hipLaunchKernelGGL(k_uref<R>, dim3(blocks_for(c->d.nx, 64), c->d.ny), dim3(64), 0, stream,
                                c->rp(c->ux), c->rp(c->uy), c->flags(), static_cast<const uint8_t*>(c->bedwall.p),
-                               c->rp(c->uref), c->d.nx, c->px, c->d.ny, c->d.nz);
+                               c->rp(c->uref), c->cmap(), c->d.nz);
             WFS_LAUNCH_CHECK();
         }
         const CaseArgs<Codec>* da = static_cast<const CaseArgs<Codec>*>(d_args.p);
@@ -975,11 +1041,117 @@ int wfs_create(const wfs_desc* desc, wfs_solver** out) {
     return WFS_OK;
 }
 
+// Tile tables of a solid mask: tiles are the 64-cell x-runs of each row that
+// hold a fluid cell, numbered in row order (z, y, x-tile).
+static void build_tiles(wfs_solver* h, const uint8_t* solid) {
+    const wfs_desc& d = h->d;
+    const int ntx = h->px / 64;
+    h->ntx = ntx;
+    const long long rows = (long long)d.ny * d.nz;
+    h->h_tmap.assign((size_t)(ntx * rows), -1);
+    h->h_tcoord.clear();
+    std::vector<std::vector<int>> row_tiles;   // per non-empty row, its tiles
+    long long T = 0;
+    for (long long j = 0; j < rows; ++j) {
+        std::vector<int> rt;
+        const uint8_t* r = solid + (size_t)j * d.nx;
+        for (int tx = 0; tx < ntx; ++tx) {
+            const int x0 = tx * 64, x1 = std::min(d.nx, x0 + 64);
+            bool fluid = false;
+            for (int x = x0; x < x1 && !fluid; ++x) fluid = r[x] == 0;
+            if (!fluid) continue;
+            h->h_tmap[(size_t)(tx + ntx * j)] = (int)T;
+            h->h_tcoord.push_back(tx);
+            h->h_tcoord.push_back((int)(j % d.ny));
+            h->h_tcoord.push_back((int)(j / d.ny));
+            rt.push_back((int)T);
+            ++T;
+        }
+        if (!rt.empty()) row_tiles.push_back(std::move(rt));
+    }
+    h->T = T;
+    h->h_nbr.assign((size_t)T * 27, -1);
+    for (long long t = 0; t < T; ++t) {
+        const int tx = h->h_tcoord[3 * t], y = h->h_tcoord[3 * t + 1], z = h->h_tcoord[3 * t + 2];
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int a = tx + dx, b = y + dy, c = z + dz;
+                    if (a < 0 || a >= ntx || b < 0 || b >= d.ny || c < 0 || c >= d.nz) continue;
+                    h->h_nbr[(size_t)(t * 27 + (dx + 1) + 3 * ((dy + 1) + 3 * (dz + 1)))] =
+                        h->h_tmap[(size_t)(a + ntx * (b + (long long)d.ny * c))];
+                }
+    }
+    // work-group -> tile: rows dealt round-robin to the 8 XCD slots (work-group
+    // L runs on XCD L % 8), so every tile of a row shares one L2
+    std::vector<std::vector<int>> slot(8);
+    for (size_t r = 0; r < row_tiles.size(); ++r)
+        slot[r % 8].insert(slot[r % 8].end(), row_tiles[r].begin(), row_tiles[r].end());
+    size_t len = 0;
+    for (auto& s8 : slot) len = std::max(len, s8.size());
+    h->h_blk2tile.assign(8 * len, -1);
+    for (int s8 = 0; s8 < 8; ++s8)
+        for (size_t k = 0; k < slot[s8].size(); ++k) h->h_blk2tile[8 * k + s8] = slot[s8][k];
+}
+
+int wfs_create_tiled(const wfs_desc* desc, const uint8_t* solid, wfs_solver** out) {
+    if (!desc || !out || !solid) return fail(WFS_EINVAL, "wfs_create_tiled: null argument");
+    *out = nullptr;
+    const wfs_desc& d = *desc;
+    if (d.nx < 3 || (d.nx - 1) % 64 == 0)
+        return fail(WFS_EINVAL, "wfs_create_tiled: need nx >= 3 with (nx - 1) % 64 != 0 (outlet row fused)");
+    const long long N = (long long)d.nx * d.ny * d.nz;
+    if (N >= (1ll << 40)) return fail(WFS_EINVAL, "wfs_create_tiled: lattice too large");
+    if (!(d.tau0 > 0.5) || !std::isfinite(d.tau0)) return fail(WFS_EINVAL, "wfs_create_tiled: tau0 must exceed 0.5");
+    if (!(d.cs >= 0.0) || !std::isfinite(d.cs)) return fail(WFS_EINVAL, "wfs_create_tiled: cs must be >= 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        return fail(WFS_ENODEV, "wfs_create_tiled: no HIP device visible");
+    }
+    if (d.device < 0 || d.device >= ndev) return fail(WFS_ENODEV, "wfs_create_tiled: device ordinal out of range");
+    if (d.ny > 65535 || d.nz > 65535) return fail(WFS_EINVAL, "wfs_create_tiled: ny and nz must be at most 65535");
+    std::unique_ptr<wfs_solver> h(make_solver(d.storage));
+    if (!h) return fail(WFS_EINVAL, "wfs_create_tiled: unknown storage format");
+    h->d = d;
+    h->N = N;
+    h->px = (d.nx + 63) / 64 * 64;
+    h->tiled = true;
+    h->bx = 64;
+    build_tiles(h.get(), solid);
+    h->Np = h->T * 64;
+    if (h->Np == 0) return fail(WFS_EINVAL, "wfs_create_tiled: no fluid cell");
+    const unsigned long long rsz = d.storage == WFS_STORE_F64 ? 8ull : 4ull;
+    if (h->Np >= (1ll << 31) || (unsigned long long)h->Np * rsz >= (1ull << 32))
+        return fail(WFS_EINVAL, "wfs_create_tiled: stored cells exceed 4 GiB per field or population direction");
+    h->device = d.device;
+    DeviceGuard g(d.device);
+    WFS_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    int rc = h->alloc();
+    if (rc) return rc;
+    *out = h.release();
+    return WFS_OK;
+}
+
+int64_t wfs_stored_cells(const wfs_solver* h) { return h ? (int64_t)h->Np : -1; }
+
+int wfs_set_force_cells(wfs_solver* h, int64_t n, const int64_t* cells, const float* fx, const float* fy,
+                        const float* fz) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!cells || !fx || !fy || !fz))) return fail(WFS_EINVAL, "wfs_set_force_cells: bad arguments");
+    for (int64_t k = 0; k < n; ++k)
+        if (cells[k] < 0 || cells[k] >= h->N) return fail(WFS_EINVAL, "wfs_set_force_cells: cell index out of range");
+    DeviceGuard g(h->device);
+    return h->set_force_cells(n, cells, fx, fy, fz);
+}
+
 int wfs_set_block_x(wfs_solver* h, int32_t block_x) {
     int rc = check_handle(h, false);
     if (rc) return rc;
     if (block_x < 64 || block_x > 256 || block_x % 64)
         return fail(WFS_EINVAL, "wfs_set_block_x: block_x must be 64, 128, 192 or 256");
+    if (h->tiled && block_x != 64) return fail(WFS_EINVAL, "wfs_set_block_x: the tile-sparse layout runs 64-cell tiles");
     h->bx = block_x;
     return WFS_OK;
 }

@@ -38,7 +38,7 @@ class LBM3D:
     engine = "hip-lbm"
 
     def __init__(self, be, solid, Fx, Fy, Fz, uin, tau0, cs=0.17, cf_wall=None, *, storage="f32", block_x=0,
-                 device=0):
+                 device=0, layout="dense"):
         if be is not None and getattr(be, "name", None) == "torch":
             raise RuntimeError("LBM3D: the 3-D model runs on the HIP library, not on a torch backend")
         if storage not in nat.STORAGE:
@@ -55,14 +55,20 @@ class LBM3D:
         self.storage = storage
         self.nstep = 0
         self.closure = "smagorinsky"
+        if layout not in ("dense", "tiles"):
+            raise ValueError("LBM3D: layout must be 'dense' or 'tiles'")
+        self.layout = layout
         L = nat.lib()
         desc = nat.wfs_desc(self.nx, self.ny, self.nz, self.tau0, self.cs, nat.STORAGE[storage], int(block_x),
                             int(device))
         h = ctypes.c_void_p()
-        nat.check(L.wfs_create(ctypes.byref(desc), ctypes.byref(h)), "wfs_create")
+        sol = np.ascontiguousarray(solid.ravel(), dtype=np.uint8)
+        if layout == "tiles":
+            nat.check(L.wfs_create_tiled(ctypes.byref(desc), nat.ptr(sol), ctypes.byref(h)), "wfs_create_tiled")
+        else:
+            nat.check(L.wfs_create(ctypes.byref(desc), ctypes.byref(h)), "wfs_create")
         self._h = h
         A = lambda a: np.ascontiguousarray(np.asarray(a).ravel(), dtype=np.float32)  # noqa: E731
-        sol = np.ascontiguousarray(solid.ravel(), dtype=np.uint8)
         cfw = A(cf_wall) if cf_wall is not None else None
         if cfw is not None and cfw.size != self.N:
             raise ValueError("LBM3D: cf_wall must have the shape of solid")
@@ -70,10 +76,14 @@ class LBM3D:
         if self._uin.size != self.ny * self.nz:
             raise ValueError("LBM3D: uin must be (nz, ny)")
         nat.check(L.wfs_upload(h, nat.ptr(sol), nat.ptr(cfw), nat.ptr(self._uin)), "wfs_upload")
-        self._Fx, self._Fy, self._Fz = A(Fx), A(Fy), A(Fz)
-        if not (self._Fx.size == self._Fy.size == self._Fz.size == self.N):
-            raise ValueError("LBM3D: forces must have the shape of solid")
-        nat.check(L.wfs_set_force(h, nat.ptr(self._Fx), nat.ptr(self._Fy), nat.ptr(self._Fz)), "wfs_set_force")
+        if Fx is None:        # forces come later as a cell list (set_force_cells)
+            self._Fx = self._Fy = self._Fz = None
+        else:
+            self._Fx, self._Fy, self._Fz = A(Fx), A(Fy), A(Fz)
+            if not (self._Fx.size == self._Fy.size == self._Fz.size == self.N):
+                raise ValueError("LBM3D: forces must have the shape of solid")
+            nat.check(L.wfs_set_force(h, nat.ptr(self._Fx), nat.ptr(self._Fy), nat.ptr(self._Fz)),
+                      "wfs_set_force")
         self._uref = None          # host value assigned by a caller, uploaded at the next step
         self._dirty = set()
 
@@ -122,6 +132,19 @@ class LBM3D:
         self._Fx, self._Fy, self._Fz = A(Fx), A(Fy), A(Fz)
         self._dirty.update(("Fx", "Fy", "Fz"))
         self._flush()
+
+    def set_force_cells(self, cells, fx, fy, fz):
+        """Body forces (lattice units) given on a list of dense cell indices; every other cell gets none."""
+        idx = np.ascontiguousarray(cells, dtype=np.int64)
+        v = [np.ascontiguousarray(a, dtype=np.float32) for a in (fx, fy, fz)]
+        if not all(a.size == idx.size for a in v):
+            raise ValueError("set_force_cells: one value per cell and component")
+        nat.check(nat.lib().wfs_set_force_cells(self.handle, idx.size, nat.ptr(idx), *(nat.ptr(a) for a in v)))
+
+    @property
+    def stored_cells(self):
+        """Cells held on the device (padded rows, or the stored tiles x 64)."""
+        return int(nat.lib().wfs_stored_cells(self.handle))
 
     def _flush(self):
         """Validate and upload replaced inputs (the checks of lbm3d.py:392-394)."""

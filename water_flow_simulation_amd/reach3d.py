@@ -204,6 +204,8 @@ class Reach:
     W: float = 0.0
     A: float = 0.0
     V: float = 0.0
+    force_cells: tuple = None   # (dense cell index, Fx, Fy, Fz) in lattice units instead of Fx/Fy/Fz arrays
+    layout: str = "dense"       # "tiles": only the 64-cell x-runs that hold fluid are stored
 
     @property
     def vel(self):
@@ -455,13 +457,18 @@ class Run3D:
         of the learned closure (closure.py), evaluated inside the step.
         block_x: work-group width, 0 for the default, "auto" for the tuned one."""
         self.r, self.be = reach, be
-        if block_x == "auto":
+        if block_x == "auto" and reach.layout == "dense":
             from .autotune import block_x_for
             block_x = block_x_for(reach, storage, closure)
+        elif block_x == "auto":
+            block_x = 0
         nu_lb = max(1e-6 * reach.dt / reach.dx ** 2, 2e-4)
         uin_lb = (reach.uin_phys / reach.vel).astype(np.float32)
         self.lbm = LBM3D(be, reach.solid, reach.Fx, reach.Fy, reach.Fz, uin_lb, tau0=0.5 + 3 * nu_lb,
-                         cs=reach.cfg.cs, cf_wall=reach.cf, storage=storage, block_x=block_x, device=device)
+                         cs=reach.cfg.cs, cf_wall=reach.cf, storage=storage, block_x=block_x, device=device,
+                         layout=reach.layout)
+        if reach.force_cells is not None:
+            self.lbm.set_force_cells(*reach.force_cells)
         if closure is not None:
             from .closure import geometry
             geo, hcol = geometry(reach.solid)
