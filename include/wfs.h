@@ -86,9 +86,9 @@ int32_t wfs_block_x(const wfs_solver* h);
  * keeps only its channel. Same physics and the same results as wfs_create
  * (bit for bit); wfs_upload must then be given the same solid mask. The work
  * of one step is one wavefront per stored tile. Requires (nx - 1) % 64 != 0.
- * Not available on this layout: the learned closure, wfs_forces, training
- * samples, batches. No reference counterpart (the reference's reaches are
- * straight prisms, reach3d.py:213-216). */
+ * The learned closure and wfs_forces work on this layout; training samples
+ * and batches need the dense one. No reference counterpart (the reference's
+ * reaches are straight prisms, reach3d.py:213-216). */
 int wfs_create_tiled(const wfs_desc* desc, const uint8_t* solid, wfs_solver** out);
 /* Cells held on the device (padded rows, or stored tiles x 64). */
 int64_t wfs_stored_cells(const wfs_solver* h);

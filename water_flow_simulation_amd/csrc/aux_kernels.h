@@ -351,17 +351,19 @@ namespace wfs {
 // function's slip correction, evaluated here by the step's own gather), so
 // the wall receives the momentum c_opp(i) (g + f_i). Each block writes its
 // nine partial sums; k_sum_parts adds the blocks in a fixed order.
-template <class Codec>
+template <class Codec, bool kSparse>
 __global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __restrict__ fs,
-                                                    StepArgs<typename Codec::R> a,
+                                                    StepArgs<typename Codec::R> a, CellMap cm,
                                                     const uint8_t* __restrict__ body, double* __restrict__ part) {
     using R = typename Codec::R;
     __shared__ double red[9][256];
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c < a.N) {
-        const int row = int(c / a.px);
-        const int x = int(c - (long long)row * a.px);
+        int x;
+        long long j;
+        dense_of(cm, c, x, j);
+        const int row = int(j);
         const int z = row / a.ny, y = row - z * a.ny;
         const uint8_t fl = x < a.nx ? a.flag[c] : uint8_t(FLAG_SOLID);
         if (!(fl & FLAG_SOLID)) {
@@ -373,7 +375,8 @@ __global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __r
             if (fl & FLAG_NEARWALL) {
                 const uint32_t wm = a.wmask[c];
                 R f[Q];
-                gather<Codec>(fs, a, int(c), x, z, fl, wm, f);
+                gather<Codec, kSparse>(fs, a, int(c), x, z, fl, wm, f, kSparse ? a.nbr + 27 * (c >> 6) : nullptr,
+                                       int(c & 63));
                 static_for<Q>([&](auto I) {
                     constexpr int i = I;
                     if ((wm >> i) & 1u) {
@@ -382,7 +385,8 @@ __global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __r
                         const double m = double(g) + double(f[i]);
                         const int xs = x - kCX[i], ys = y - kCY[i], zs = z - kCZ[i];
                         const bool in = xs >= 0 && xs < a.nx && ys >= 0 && ys < a.ny && zs >= 0 && zs < a.nz;
-                        const bool hb = in && body && body[xs + (long long)a.px * (ys + (long long)a.ny * zs)];
+                        const long long sc = in ? cell_of(cm, xs, ys + (long long)a.ny * zs) : -1;
+                        const bool hb = sc >= 0 && body && body[sc];
                         const int k = hb ? 3 : 6;
                         acc[k] -= kCX[i] * m;
                         acc[k + 1] -= kCY[i] * m;

@@ -688,16 +688,19 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
 // The step with the learned closure (float arithmetic, pair-form collision).
 // The network runs on the whole wavefront between the moments and the
 // relaxation, so every lane reaches it; inactive lanes feed zero features.
-template <class Codec>
+// Tile-sparse layout as in step_tile (one 64-cell tile per work-group).
+template <class Codec, bool kSparse = false>
 __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __restrict__ fs,
                                                   typename Codec::T* __restrict__ fd, const StepArgs<float>& a,
-                                                  int row, int tile, const MlpPack* P) {
+                                                  int row, int tile, const MlpPack* P, int t = 0) {
     using R = float;
-    const int x = tile * int(blockDim.x) + int(threadIdx.x);
-    const int xw = tile * int(blockDim.x) + (int(threadIdx.x) & ~63);
+    const int lane = int(threadIdx.x) & 63;
+    const int x = kSparse ? tile * 64 + lane : tile * int(blockDim.x) + int(threadIdx.x);
+    const int xw = kSparse ? tile * 64 : tile * int(blockDim.x) + (int(threadIdx.x) & ~63);
     const bool inrow = x < a.nx;
     const int z = row / a.ny;
-    const int c = x + a.px * row;
+    const int c = kSparse ? t * 64 + lane : x + a.px * row;
+    const int* nb = kSparse ? a.nbr + 27 * t : nullptr;
     const uint8_t fl = inrow ? at(a.flag, c) : uint8_t(FLAG_SOLID);
     const bool active = !(fl & FLAG_SOLID);
     const bool forced = (fl & FLAG_FORCED) != 0;
@@ -708,7 +711,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     float feat[kNF] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (active) {
         wm = (fl & FLAG_NEARWALL) ? at(a.wmask, c) : 0u;
-        gather<Codec>(fs, a, c, x, z, fl, wm, f);
+        gather<Codec, kSparse>(fs, a, c, x, z, fl, wm, f, nb, lane);
         if (forced) { fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale; }
         pair_moments<R>(f, forced, fx, fy, fz, m);
         // resolved strain rate S = -3 Pi_dev / (2 rho tau), tau from the constant-Cs model
@@ -716,7 +719,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
         const R k = R(-1.5) * m.ri * __builtin_amdgcn_rcpf(tau_s);
         const R t3 = (m.pxx + m.pyy + m.pzz) * (R(1) / R(3));
         closure_features(k * (m.pxx - t3), k * (m.pyy - t3), k * (m.pzz - t3), k * m.pxy, k * m.pxz, k * m.pyz,
-                         m.ux, m.uy, m.uz, at(a.geo, c), at(a.hcol, c - a.px * a.ny * z), a.inv_u0, feat);
+                         m.ux, m.uy, m.uz, at(a.geo, c), at(a.hcol, x + a.px * (row - a.ny * z)), a.inv_u0, feat);
     }
     const float y = mlp_wave(feat, P);
     if (active) {
@@ -746,6 +749,18 @@ __global__ void __launch_bounds__(256) k_step_learned(const typename Codec::T* _
     if (!tile_of(blockIdx.x, a.tiles, a.rows, row, tile)) return;   // whole work-group
     stage_pack(a.mlp, &P);
     step_tile_learned<Codec>(fs, fd, a, row, tile, &P);
+}
+
+// The learned-closure step on the tile-sparse layout.
+template <class Codec>
+__global__ void __launch_bounds__(64) k_step_learned_sparse(const typename Codec::T* __restrict__ fs,
+                                                            typename Codec::T* __restrict__ fd, StepArgs<float> a) {
+    __shared__ MlpPack P;
+    const int t = a.blk2tile[blockIdx.x];
+    if (t < 0) return;   // whole work-group
+    stage_pack(a.mlp, &P);
+    const int tx = a.tcoord[3 * t], y = a.tcoord[3 * t + 1], z = a.tcoord[3 * t + 2];
+    step_tile_learned<Codec, true>(fs, fd, a, y + a.ny * z, tx, &P, t);
 }
 
 // The step on the tile-sparse layout: one wavefront per work-group, one

@@ -438,6 +438,13 @@ struct Solver final : wfs_solver {
     // One step launch: the constant-Cs kernel or, with the learned closure, k_step_learned.
     void launch_step(const dim3& grid, const StepArgs<R>& a) {
         if (tiled) {
+            if constexpr (std::is_same<R, float>::value) {
+                if (closure == WFS_CLOSURE_MLP) {
+                    hipLaunchKernelGGL(k_step_learned_sparse<Codec>, dim3((unsigned)h_blk2tile.size()), dim3(64), 0,
+                                       stream, fp(cur), fp(cur ^ 1), a);
+                    return;
+                }
+            }
             hipLaunchKernelGGL(k_step_sparse<Codec>, dim3((unsigned)h_blk2tile.size()), dim3(64), 0, stream,
                                fp(cur), fp(cur ^ 1), a);
             return;
@@ -618,7 +625,6 @@ struct Solver final : wfs_solver {
             return WFS_OK;
         }
         if (kind != WFS_CLOSURE_MLP) return fail(WFS_EINVAL, "wfs_set_closure: unknown closure kind");
-        if (tiled) return fail(WFS_EINVAL, "wfs_set_closure: the learned closure runs on the dense layout");
         if constexpr (!std::is_same<R, float>::value) {
             return fail(WFS_EINVAL, "wfs_set_closure: the learned closure runs in float arithmetic (not FP64 storage)");
         } else {
@@ -671,7 +677,6 @@ struct Solver final : wfs_solver {
     }
 
     int forces(double* out) override {
-        if (tiled) return fail(WFS_EINVAL, "wfs_forces: runs on the dense layout");
         StepArgs<R> a;
         fill_args(a);
         const long long blocks = (Np + 255) / 256;
@@ -679,8 +684,12 @@ struct Solver final : wfs_solver {
         int rc;
         if ((rc = alloc_buf(part, (size_t)blocks * 9 * sizeof(double))) || (rc = alloc_buf(sum, 9 * sizeof(double))))
             return rc;
-        hipLaunchKernelGGL(k_wall_force<Codec>, dim3((unsigned)blocks), dim3(256), 0, stream, fp(cur), a,
-                           static_cast<const uint8_t*>(body.p), static_cast<double*>(part.p));
+        if (tiled)
+            hipLaunchKernelGGL((k_wall_force<Codec, true>), dim3((unsigned)blocks), dim3(256), 0, stream, fp(cur), a,
+                               cmap(), static_cast<const uint8_t*>(body.p), static_cast<double*>(part.p));
+        else
+            hipLaunchKernelGGL((k_wall_force<Codec, false>), dim3((unsigned)blocks), dim3(256), 0, stream, fp(cur),
+                               a, cmap(), static_cast<const uint8_t*>(body.p), static_cast<double*>(part.p));
         WFS_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, stream, static_cast<const double*>(part.p), blocks, 9,
                            static_cast<double*>(sum.p));

@@ -193,7 +193,7 @@ def build_meander(mc: MeanderConfig) -> R.Reach:
     return R.Reach(sec=sec, cfg=cfg, fleet=fleet, h=h, Q=Q, S=mc.slope, dx=dx, dt=dt, nx=nx, ny=ny, nz=nz, y=y,
                    zc=zc, solid=solid, uin_phys=uin, Fx=None, Fy=None, Fz=None, cf=cf, bedwall=bedwall,
                    hulls=hulls, props=props, notes=[], W=w_top, A=A, V=V,
-                   force_cells=(idx, fv[0], fv[1], fv[2]), layout="tiles")
+                   force_cells=(idx, fv[0], fv[1], fv[2]), layout="tiles", hull_cells=hull_mask)
 
 
 def run(scale=1.0, steps=200, warmup=20, storage="f16dev", log=print):

@@ -205,6 +205,7 @@ class Reach:
     A: float = 0.0
     V: float = 0.0
     force_cells: tuple = None   # (dense cell index, Fx, Fy, Fz) in lattice units instead of Fx/Fy/Fz arrays
+    hull_cells: np.ndarray = None   # hull mask when the bed/bank section varies along x
     layout: str = "dense"       # "tiles": only the 64-cell x-runs that hold fluid are stored
 
     @property
@@ -564,6 +565,8 @@ class Run3D:
 
     def hull_mask(self):
         """Solid cells that are not part of the bed/bank cross-section (the hulls)."""
+        if self.r.hull_cells is not None:
+            return self.r.hull_cells
         s = self.r.solid
         return s & ~np.broadcast_to(s[:, :, :1], s.shape)
 
