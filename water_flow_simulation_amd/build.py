@@ -20,7 +20,7 @@ LIB = os.path.join(PKG, "libwfs.so")
 PROBE = os.path.join(PKG, "ceiling_probe")
 
 SOURCES = ["wfs_api.hip"]
-HEADERS = ["lattice.h", "step_kernels.h", "aux_kernels.h"]
+HEADERS = ["lattice.h", "closure.h", "step_kernels.h", "aux_kernels.h"]
 
 # -ffp-contract=off: no FMA contraction, so the FP32 kernels round like the
 # scalar IEEE formulas they implement. Packed FP32 instructions are disabled.

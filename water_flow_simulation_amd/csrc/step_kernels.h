@@ -751,25 +751,31 @@ __global__ void __launch_bounds__(256) k_step_learned(const typename Codec::T* _
     step_tile_learned<Codec>(fs, fd, a, row, tile, &P);
 }
 
-// The learned-closure step on the tile-sparse layout.
-template <class Codec>
-__global__ void __launch_bounds__(64) k_step_learned_sparse(const typename Codec::T* __restrict__ fs,
-                                                            typename Codec::T* __restrict__ fd, StepArgs<float> a) {
+// Tile-sparse layout: G wavefronts per work-group, one stored tile each
+// (blk2tile[group * G + wave]; -1 idle); the block map keeps every tile of a
+// row on one XCD. The plain step runs one tile per work-group; the learned
+// step four, so that one LDS staging of the network serves four tiles.
+constexpr int kTilesPerGroupLearned = 4;
+
+template <class Codec, int G>
+__global__ void __launch_bounds__(64 * G) k_step_learned_sparse(const typename Codec::T* __restrict__ fs,
+                                                               typename Codec::T* __restrict__ fd,
+                                                               StepArgs<float> a) {
     __shared__ MlpPack P;
-    const int t = a.blk2tile[blockIdx.x];
-    if (t < 0) return;   // whole work-group
     stage_pack(a.mlp, &P);
+    // the tile is uniform over the wavefront: scalar registers, scalar loads of its tables
+    const int t = __builtin_amdgcn_readfirstlane(a.blk2tile[blockIdx.x * G + (threadIdx.x >> 6)]);
+    if (t < 0) return;   // whole wavefront
     const int tx = a.tcoord[3 * t], y = a.tcoord[3 * t + 1], z = a.tcoord[3 * t + 2];
     step_tile_learned<Codec, true>(fs, fd, a, y + a.ny * z, tx, &P, t);
 }
 
-// The step on the tile-sparse layout: one wavefront per work-group, one
-// active tile per wavefront (blk2tile keeps every tile of a row on one XCD).
-template <class Codec>
-__global__ void __launch_bounds__(64) k_step_sparse(const typename Codec::T* __restrict__ fs,
-                                                    typename Codec::T* __restrict__ fd, StepArgs<typename Codec::R> a) {
-    const int t = a.blk2tile[blockIdx.x];
-    if (t < 0) return;   // whole work-group
+template <class Codec, int G>
+__global__ void __launch_bounds__(64 * G) k_step_sparse(const typename Codec::T* __restrict__ fs,
+                                                       typename Codec::T* __restrict__ fd,
+                                                       StepArgs<typename Codec::R> a) {
+    const int t = __builtin_amdgcn_readfirstlane(a.blk2tile[blockIdx.x * G + (threadIdx.x >> 6)]);
+    if (t < 0) return;   // whole wavefront
     const int tx = a.tcoord[3 * t], y = a.tcoord[3 * t + 1], z = a.tcoord[3 * t + 2];
     step_tile<Codec, true>(fs, fd, a, y + a.ny * z, tx, t);
 }

@@ -146,7 +146,7 @@ struct wfs_solver {
     bool tiled = false;
     int ntx = 0;                                         // x-tiles per row (px / 64)
     long long T = 0;                                     // stored tiles
-    std::vector<int> h_tmap, h_tcoord, h_nbr, h_blk2tile;
+    std::vector<int> h_tmap, h_tcoord, h_nbr, h_blk2tile, h_blk2tile4;   // block maps: 1 and 4 tiles per group
     virtual int set_force_cells(long long n, const int64_t* idx, const float* fx, const float* fy,
                                 const float* fz) = 0;
 
@@ -173,7 +173,7 @@ struct Solver final : wfs_solver {
     DevBuf sums[6], partial, out_x, counter, stage;
     DevBuf geo, hcol, mlp_params, mlp_pack, c2buf;   // learned closure (allocated when first set)
     DevBuf body;                                      // hull cells for the force budget (when set)
-    DevBuf tmap_d, tcoord_d, nbr_d, blk2tile_d;       // tile-sparse tables (when tiled)
+    DevBuf tmap_d, tcoord_d, nbr_d, blk2tile_d, blk2tile4_d;   // tile-sparse tables (when tiled)
 
     CellMap cmap() const {
         CellMap m;
@@ -213,7 +213,8 @@ struct Solver final : wfs_solver {
         if ((rc = alloc_buf(stage, (size_t)N * sizeof(double)))) return rc;  // dense host <-> device staging
         if (tiled) {
             for (auto pr : {std::make_pair(&tmap_d, &h_tmap), std::make_pair(&tcoord_d, &h_tcoord),
-                            std::make_pair(&nbr_d, &h_nbr), std::make_pair(&blk2tile_d, &h_blk2tile)}) {
+                            std::make_pair(&nbr_d, &h_nbr), std::make_pair(&blk2tile_d, &h_blk2tile),
+                            std::make_pair(&blk2tile4_d, &h_blk2tile4)}) {
                 if ((rc = alloc_buf(*pr.first, pr.second->size() * sizeof(int)))) return rc;
                 WFS_HIP(hipMemcpy(pr.first->p, pr.second->data(), pr.second->size() * sizeof(int),
                                   hipMemcpyHostToDevice));
@@ -440,12 +441,15 @@ struct Solver final : wfs_solver {
         if (tiled) {
             if constexpr (std::is_same<R, float>::value) {
                 if (closure == WFS_CLOSURE_MLP) {
-                    hipLaunchKernelGGL(k_step_learned_sparse<Codec>, dim3((unsigned)h_blk2tile.size()), dim3(64), 0,
-                                       stream, fp(cur), fp(cur ^ 1), a);
+                    constexpr int G = kTilesPerGroupLearned;
+                    StepArgs<R> b = a;
+                    b.blk2tile = static_cast<const int*>(blk2tile4_d.p);
+                    hipLaunchKernelGGL((k_step_learned_sparse<Codec, G>), dim3((unsigned)(h_blk2tile4.size() / G)),
+                                       dim3(64 * G), 0, stream, fp(cur), fp(cur ^ 1), b);
                     return;
                 }
             }
-            hipLaunchKernelGGL(k_step_sparse<Codec>, dim3((unsigned)h_blk2tile.size()), dim3(64), 0, stream,
+            hipLaunchKernelGGL((k_step_sparse<Codec, 1>), dim3((unsigned)h_blk2tile.size()), dim3(64), 0, stream,
                                fp(cur), fp(cur ^ 1), a);
             return;
         }
@@ -1091,16 +1095,21 @@ static void build_tiles(wfs_solver* h, const uint8_t* solid) {
                         h->h_tmap[(size_t)(a + ntx * (b + (long long)d.ny * c))];
                 }
     }
-    // work-group -> tile: rows dealt round-robin to the 8 XCD slots (work-group
-    // L runs on XCD L % 8), so every tile of a row shares one L2
+    // work-group -> tiles: rows dealt round-robin to the 8 XCD slots (work-group
+    // L runs on XCD L % 8), so every tile of a row shares one L2; a work-group
+    // takes G consecutive tiles of its slot
     std::vector<std::vector<int>> slot(8);
     for (size_t r = 0; r < row_tiles.size(); ++r)
         slot[r % 8].insert(slot[r % 8].end(), row_tiles[r].begin(), row_tiles[r].end());
-    size_t len = 0;
-    for (auto& s8 : slot) len = std::max(len, s8.size());
-    h->h_blk2tile.assign(8 * len, -1);
-    for (int s8 = 0; s8 < 8; ++s8)
-        for (size_t k = 0; k < slot[s8].size(); ++k) h->h_blk2tile[8 * k + s8] = slot[s8][k];
+    auto block_map = [&](size_t G, std::vector<int>& out) {
+        size_t groups = 0;
+        for (auto& s8 : slot) groups = std::max(groups, (s8.size() + G - 1) / G);
+        out.assign(8 * groups * G, -1);
+        for (int s8 = 0; s8 < 8; ++s8)
+            for (size_t k = 0; k < slot[s8].size(); ++k) out[(8 * (k / G) + s8) * G + k % G] = slot[s8][k];
+    };
+    block_map(1, h->h_blk2tile);
+    block_map(kTilesPerGroupLearned, h->h_blk2tile4);
 }
 
 int wfs_create_tiled(const wfs_desc* desc, const uint8_t* solid, wfs_solver** out) {

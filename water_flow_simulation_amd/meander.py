@@ -196,9 +196,12 @@ def build_meander(mc: MeanderConfig) -> R.Reach:
                    force_cells=(idx, fv[0], fv[1], fv[2]), layout="tiles", hull_cells=hull_mask)
 
 
-def run(scale=1.0, steps=200, warmup=20, storage="f16dev", log=print):
+def run(scale=1.0, steps=200, warmup=20, storage="f16dev", learned=False, log=print):
     """Builds the case, steps it inside the run cadence and reports MLUPS over
-    stored and fluid cells, device memory and FP64 mass/discharge."""
+    stored and fluid cells, device memory, and the FP64 budgets: mass,
+    discharge at the inlet and outlet, thrust and the forces on the hulls and
+    on bed and banks. learned: the committed learned closure instead of
+    constant-Cs Smagorinsky."""
     t0 = time.time()
     reach = meander_case(scale)
     t_build = time.time() - t0
@@ -207,7 +210,11 @@ def run(scale=1.0, steps=200, warmup=20, storage="f16dev", log=print):
         log(f"meander {reach.nx}x{reach.ny}x{reach.nz} box ({reach.cells / 1e9:.3f} G cells), {fluid / 1e9:.3f} G "
             f"fluid, {len(reach.props)} disks; built in {t_build:.1f} s")
     t1 = time.time()
-    run_ = R.Run3D(reach, storage=storage)
+    closure = None
+    if learned:
+        from .closure import load_params
+        closure = load_params()[0]
+    run_ = R.Run3D(reach, storage=storage, closure=closure)
     t_up = time.time() - t1
     L = run_.lbm
     stored = L.stored_cells
@@ -221,13 +228,15 @@ def run(scale=1.0, steps=200, warmup=20, storage="f16dev", log=print):
     L.check_finite()
     mass = L.total("rho")
     q = L.profile_x("ux")
+    forces = {k: [round(float(v), 1) for v in f] for k, f in run_.forces().items()}
     res = {"box": [reach.nx, reach.ny, reach.nz], "box_cells": reach.cells, "stored_cells": stored,
            "fluid_cells": fluid, "storage": storage, "steps": steps, "build_s": round(t_build, 1),
            "upload_s": round(t_up, 1), "wall_s": round(wall, 3), "kernel_ms": round(ms / max(n, 1), 4),
            "mlups_fluid": round(steps * fluid / wall / 1e6, 1), "mlups_stored": round(steps * stored / wall / 1e6, 1),
            "device_gb": round(L.memory_mb() / 1e3, 2), "mass_fp64": mass,
            "discharge_in_out": [float(q[2] * reach.vel * reach.dx ** 2), float(q[-3] * reach.vel * reach.dx ** 2)],
-           "Q": reach.Q}
+           "Q": reach.Q, "closure": "learned" if learned else "smagorinsky", "forces_N": forces,
+           "thrust_design_N": float(sum(p["T"] for p in reach.props))}
     run_.close()
     return res
 
@@ -237,8 +246,10 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--storage", default="f16dev")
+    ap.add_argument("--learned", action="store_true", help="the learned sub-grid closure")
     a = ap.parse_args()
-    print(json.dumps(run(a.scale, a.steps, storage=a.storage, log=lambda m: print(m, flush=True))), flush=True)
+    print(json.dumps(run(a.scale, a.steps, storage=a.storage, learned=a.learned,
+                         log=lambda m: print(m, flush=True))), flush=True)
 
 
 if __name__ == "__main__":
