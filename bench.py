@@ -12,7 +12,11 @@ the case builder; all state is resident in GPU memory before timing starts.
 Prints one JSON line: MLUPS (all cells, the reference's convention), the
 step kernel's HBM roofline fraction from HIP events around every launch on
 the solver's stream, and a CPU baseline (the OpenMP oracle in its
-Numba-like precision, on a bounded sample of the same reach).
+Numba-like precision, on a bounded sample of the same reach). Beside it:
+the FP32-fast, 16-bit and learned-closure variants on the same reach, the
+batched sweep against the same cases run one after another, and the
+meandering 1 km reach (1.56 G-cell box, 0.67 G fluid cells) on the
+tile-sparse layout, FP16 storage, with and without the learned closure.
 """
 from __future__ import annotations
 
@@ -134,6 +138,28 @@ def time_batched(cases, steps, seed=0, scale=0.5):
             "ensemble": f"seeded boat-test reaches at dx 0.2 m, lengths and widths x{scale}"}
 
 
+def time_meander(variants, steps, scale=1.0):
+    """The meandering-reach workload on the tile-sparse layout (1 km sine reach,
+    80 m top width, 4 m deep, dx 0.08 m, 8 pushers): per variant the step
+    kernel's average time, MLUPS over fluid and stored cells and the HBM
+    fraction of the step's algorithmic bytes per fluid cell."""
+    from water_flow_simulation_amd import meander
+    out = {}
+    for name in variants:
+        st, learned = name.split("+")[0], name.endswith("+mlp")
+        log(f"meander {name}")
+        r = meander.run(scale, steps, warmup=20, storage=st, learned=learned, log=log)
+        k_s = r["kernel_ms"] / 1e3
+        b_alg = B_ALG[st] + (2 if learned else 0)
+        out[name] = {"kernel_ms_avg": r["kernel_ms"], "mlups_fluid": r["mlups_fluid"],
+                     "mlups_stored": r["mlups_stored"],
+                     "roofline_frac": round(b_alg * r["fluid_cells"] / k_s / 1e9 / PEAK_HBM_GBS, 4),
+                     "device_gb": r["device_gb"], "setup_s": round(r["build_s"] + r["upload_s"], 1)}
+        box = {k: r[k] for k in ("box", "box_cells", "stored_cells", "fluid_cells", "steps")}
+    out["case"] = box
+    return out
+
+
 def traffic_per_launch():
     """HBM bytes per step-kernel launch from the committed rocprofv3 counter summary, if any."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -154,6 +180,10 @@ def main():
                     help="extra variants timed (comma list; storage format, '+mlp' for the learned closure)")
     ap.add_argument("--batch-cases", type=int, default=8, help="cases of the batched-sweep line (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=400)
+    ap.add_argument("--meander", default="f16dev,f16dev+mlp",
+                    help="variants timed on the meandering reach, tile-sparse layout ('' to skip)")
+    ap.add_argument("--meander-steps", type=int, default=200)
+    ap.add_argument("--meander-scale", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -206,6 +236,8 @@ def main():
     if a.batch_cases > 0:
         log(f"timing the batched sweep ({a.batch_cases} cases)")
         out["batched"] = time_batched(a.batch_cases, a.batch_steps)
+    if a.meander:
+        out["meander"] = time_meander([v for v in a.meander.split(",") if v], a.meander_steps, a.meander_scale)
     if not a.no_cpu_baseline:
         log("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(reach)

@@ -195,6 +195,24 @@ def test_geometry_codes_brute_force():
                     assert geo[z, y, x] >> 8 == int(np.rint(255 * zr))
 
 
+def test_geometry_codes_chunked_equal_whole():
+    """The x-slab distance transform (overlap nz + 1) gives the whole-box codes:
+    a deep open box where the nearest wall of mid cells is the floor, far
+    beyond one slab, and sparse random obstacles."""
+    rng = np.random.default_rng(12)
+    nz, ny, nx = 9, 40, 53
+    solid = rng.random((nz, ny, nx)) < 0.02
+    solid[:, :, 20:24] = False
+    solid[:, :6, :30] = True          # slabs whose first fluid rows start inside the box
+    solid[:, 31:, 10:] = True
+    solid[:, :, 45:48] = True         # an all-solid slab
+    whole = CL.geometry(solid, chunk=nx, workers=1)
+    for chunk, workers in ((3, 1), (5, 1), (7, 4), (16, 3)):
+        part = CL.geometry(solid, chunk=chunk, workers=workers)
+        np.testing.assert_array_equal(part[0], whole[0])
+        np.testing.assert_array_equal(part[1], whole[1])
+
+
 def test_smagorinsky_weights_give_cs():
     y, _ = C.forward(CL.smagorinsky_params(0.17), np.random.default_rng(9).normal(size=(10, 8)))
     np.testing.assert_allclose(C.c2_of(y), 0.17 ** 2, rtol=1e-6)

@@ -61,7 +61,7 @@ WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "closure_weig
 
 
 # ---------------------------------------------------------------- geometry codes
-def geometry(solid):
+def geometry(solid, chunk=256, workers=min(16, os.cpu_count() or 1)):
     """Per-cell code geo = round(255 / d) | round(255 z_rel) << 8 and per-column
     fluid-cell count hcol (ny, nx), for a solid mask (nz, ny, nx).
 
@@ -69,21 +69,50 @@ def geometry(solid):
     in cells, with walls as the step sees them: solid cells, the rows beyond
     the y edges and the layer below z = 0 (the lid and the x ends are open).
     z_rel = (z - z_bed + 1/2) / h, z_bed the lowest fluid cell of the column
-    and h its fluid-cell count."""
+    and h its fluid-cell count.
+
+    The wall layer below z = 0 puts a wall within nz cells of every cell, so
+    the distance transform runs on x-slabs of ``chunk`` cells with nz + 1
+    cells of overlap on each side, exactly, on ``workers`` threads. Each slab
+    is cut in y to its fluid rows and one solid row either side: a wall
+    beyond that row is farther than the row's own cell."""
     solid = np.asarray(solid, bool)
     nz, ny, nx = solid.shape
-    pad = np.ones((nz + 1, ny + 2, nx), bool)
-    pad[1:, 1:-1, :] = solid
-    d = ndimage.distance_transform_edt(~pad)[1:, 1:-1, :]
-    inv = np.where(solid, 0.0, np.clip(np.rint(255.0 / np.maximum(d, 1.0)), 0, 255))
-    fluid = ~solid
-    ncol = fluid.sum(0)
-    zb = np.argmax(fluid, axis=0)
+    geo = np.zeros((nz, ny, nx), np.uint16)
+    ncol = np.empty((ny, nx), np.int64)
+    m = nz + 1
     z = np.arange(nz)[:, None, None]
-    zrel = np.clip((z - zb[None] + 0.5) / np.maximum(ncol, 1)[None], 0.0, 1.0)
-    zq = np.where(solid, 0.0, np.rint(255 * zrel))
-    geo = inv.astype(np.uint16) | (zq.astype(np.uint16) << 8)
-    return np.ascontiguousarray(geo), np.minimum(ncol, 255).astype(np.uint8)
+
+    def slab(x0):
+        x1 = min(nx, x0 + chunk)
+        s = solid[:, :, x0:x1]
+        fluid = ~s
+        nc = fluid.sum(0)
+        ncol[:, x0:x1] = nc
+        a, b = max(0, x0 - m), min(nx, x1 + m)
+        rows = np.flatnonzero(~solid[:, :, a:b].all(axis=(0, 2)))
+        if rows.size == 0:
+            return
+        y0, y1 = max(int(rows[0]) - 1, 0), min(int(rows[-1]) + 2, ny)
+        pad = np.ones((nz + 1, y1 - y0 + 2, b - a), bool)
+        pad[1:, 1:-1, :] = solid[:, y0:y1, a:b]
+        d = ndimage.distance_transform_edt(~pad)[1:, 1:-1, x0 - a:x1 - a]
+        sc, nc = s[:, y0:y1], nc[y0:y1]
+        inv = np.where(sc, 0.0, np.clip(np.rint(255.0 / np.maximum(d, 1.0)), 0, 255))
+        zb = np.argmax(fluid[:, y0:y1], axis=0)
+        zrel = np.clip((z - zb[None] + 0.5) / np.maximum(nc, 1)[None], 0.0, 1.0)
+        zq = np.where(sc, 0.0, np.rint(255 * zrel))
+        geo[:, y0:y1, x0:x1] = inv.astype(np.uint16) | (zq.astype(np.uint16) << 8)
+
+    starts = range(0, nx, chunk)
+    if workers > 1 and nx > chunk:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(workers) as ex:
+            list(ex.map(slab, starts))
+    else:
+        for x0 in starts:
+            slab(x0)
+    return geo, np.minimum(ncol, 255).astype(np.uint8)
 
 
 def coarsen_solid(solid, ncx, ncy, ncz):
