@@ -62,13 +62,18 @@ __device__ __forceinline__ void store_pop(T* p, T v) {
 #endif
 }
 
-// Codec interface: ld/st/rt take the centre e of the stored direction (used
-// by the 16-bit formats only): ld decodes, st encodes and stores, rt is the
+// Codec interface: dec decodes a stored value with the centre e of its
+// direction (used by the 16-bit formats only), enc gives the value to store
+// (its key: store_key below), ld/st are the same through a pointer, rt is the
 // value a store followed by a load gives back.
 struct CodecF32 {
     using T = float;
     using R = float;
     static constexpr bool kExact = true;   // operand order of the reference update, bit-exact
+    template <int I>
+    __device__ static R dec(T v, float) { return v; }
+    template <int I>
+    __device__ static T enc(R v, float) { return v; }
     template <int I>
     __device__ static R ld(const T* p, float) { return *p; }
     template <int I>
@@ -87,6 +92,10 @@ struct CodecF64 {
     using T = double;
     using R = double;
     static constexpr bool kExact = true;
+    template <int I>
+    __device__ static R dec(T v, float) { return v; }
+    template <int I>
+    __device__ static T enc(R v, float) { return v; }
     template <int I>
     __device__ static R ld(const T* p, float) { return *p; }
     template <int I>
@@ -110,15 +119,18 @@ struct CodecF16Dev {
     using R = float;
     static constexpr bool kExact = false;
     template <int I>
-    __device__ static R ld(const T* p, float e) { return fmaf(__half2float(*p), 0.5f, e); }
+    __device__ static R dec(T v, float e) { return fmaf(__half2float(v), 0.5f, e); }
     // n2 = -2 e
-    __device__ static __half enc(R v, float n2) { return __float2half_rn(fmaf(v, 2.0f, n2)); }
+    template <int I>
+    __device__ static T enc(R v, float n2) { return __float2half_rn(fmaf(v, 2.0f, n2)); }
+    template <int I>
+    __device__ static R ld(const T* p, float e) { return dec<I>(*p, e); }
     template <int I>
     __device__ static void st(T* p, R v, float n2) {
-        store_pop(reinterpret_cast<unsigned short*>(p), __half_as_ushort(enc(v, n2)));
+        store_pop(reinterpret_cast<unsigned short*>(p), __half_as_ushort(enc<I>(v, n2)));
     }
     template <int I>
-    __device__ static R rt(R v, float e, float n2) { return fmaf(__half2float(enc(v, n2)), 0.5f, e); }
+    __device__ static R rt(R v, float e, float n2) { return dec<I>(enc<I>(v, n2), e); }
     static constexpr bool kStoreUsesN2 = true;
 };
 
@@ -127,13 +139,17 @@ struct CodecBF16Dev {
     using R = float;
     static constexpr bool kExact = false;
     template <int I>
-    __device__ static R ld(const T* p, float e) { return __bfloat162float(*p) + e; }
+    __device__ static R dec(T v, float e) { return __bfloat162float(v) + e; }
+    template <int I>
+    __device__ static T enc(R v, float e) { return __float2bfloat16(v - e); }
+    template <int I>
+    __device__ static R ld(const T* p, float e) { return dec<I>(*p, e); }
     template <int I>
     __device__ static void st(T* p, R v, float e) {
-        store_pop(reinterpret_cast<unsigned short*>(p), __bfloat16_as_ushort(__float2bfloat16(v - e)));
+        store_pop(reinterpret_cast<unsigned short*>(p), __bfloat16_as_ushort(enc<I>(v, e)));
     }
     template <int I>
-    __device__ static R rt(R v, float e, float) { return __bfloat162float(__float2bfloat16(v - e)) + e; }
+    __device__ static R rt(R v, float e, float) { return dec<I>(enc<I>(v, e), e); }
     static constexpr bool kStoreUsesN2 = false;
 };
 
@@ -191,6 +207,55 @@ __device__ __forceinline__ T* elem(T* base, uint32_t idx) {
 template <typename T>
 __device__ __forceinline__ T& at(T* base, int idx) {
     return *elem(base, (uint32_t)idx);
+}
+
+// Buffer access to one population direction: a resource in scalar registers
+// (base, size in bytes; word 3 = 32-bit data format, the gfx9-family raw
+// buffer) and a 32-bit byte offset per lane. Every direction of a cell then
+// shares one lane offset, and the step spends no vector instruction on
+// addresses. Offsets at or past the size read 0 and drop stores. The size is
+// that of one direction array, below 4 GiB (wfs_create keeps cells * 8 B below
+// 2^32), and every lane's offset is below it.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (sizeof(T) == 2) {
+        return __builtin_bit_cast(T, (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0));
+    } else if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    } else {
+        static_assert(sizeof(T) == 8, "2, 4 or 8 bytes");
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    }
+}
+// cache policy of population stores: non-temporal (nt) when WFS_NT_STORE
+constexpr int kPopStoreAux = WFS_NT_STORE ? 2 : 0;
+template <typename T>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, T v) {
+    if constexpr (sizeof(T) == 2) {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, off, 0, kPopStoreAux);
+    } else if constexpr (sizeof(T) == 4) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, kPopStoreAux);
+    } else {
+        using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, kPopStoreAux);
+    }
+}
+
+// Stores the post-collision populations of cell c (all lanes that call it).
+template <class Codec>
+__device__ __forceinline__ void store_cell(typename Codec::T* __restrict__ fd, long long N, const Center& ctr,
+                                           const typename Codec::R* f, int c) {
+    using T = typename Codec::T;
+    const uint32_t off = (uint32_t)c * (uint32_t)sizeof(T);
+    const uint32_t bytes = (uint32_t)N * (uint32_t)sizeof(T);
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        buf_store<T>(buf_rsrc(fd + (long long)i * N, bytes), off,
+                     Codec::template enc<i>(f[i], store_key<Codec>(ctr, i)));
+    });
 }
 
 template <typename R>
@@ -439,8 +504,19 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
         // interior: every source is the plain upstream neighbour
         static_for<Q>([&](auto I) {
             constexpr int i = I;
-            const int s = pull_source<kSparse, kCX[i], kCY[i], kCZ[i]>(c, px, plane, nb, lane);
-            f[i] = Codec::template ld<i>(elem(fs + (long long)i * N, (uint32_t)s), a.ctr.c[center_class(i)]);
+            if constexpr (!kSparse) {
+                // the pull offset is uniform: it moves the direction's buffer base
+                // (i N - offset >= 0 for every direction); one lane offset for all 19
+                using T = typename Codec::T;
+                const long long off = kCX[i] + (long long)px * kCY[i] + (long long)plane * kCZ[i];
+                // lane offsets c < N: the direction's size bounds them
+                const T v = buf_load<T>(buf_rsrc(fs + ((long long)i * N - off), (uint32_t)N * (uint32_t)sizeof(T)),
+                                        (uint32_t)c * (uint32_t)sizeof(T));
+                f[i] = Codec::template dec<i>(v, a.ctr.c[center_class(i)]);
+            } else {
+                const int s = pull_source<kSparse, kCX[i], kCY[i], kCZ[i]>(c, px, plane, nb, lane);
+                f[i] = Codec::template ld<i>(elem(fs + (long long)i * N, (uint32_t)s), a.ctr.c[center_class(i)]);
+            }
         });
     } else {
         static_for<Q>([&](auto I) {
@@ -554,12 +630,120 @@ __device__ __forceinline__ void finish(typename Codec::T* __restrict__ fd, const
         }
     }
 
-    if (active) {
+    if (active) store_cell<Codec>(fd, N, a.ctr, f, c);
+}
+
+// The update of one active cell: pull, wall function, collision; f gets the
+// post-collision populations; the macroscopic fields are written where the
+// next step or the caller reads them.
+template <class Codec, bool kSparse>
+__device__ __forceinline__ void cell_update(const typename Codec::T* __restrict__ fs,
+                                            const StepArgs<typename Codec::R>& a, int c, int x, int z, uint8_t fl,
+                                            typename Codec::R* f, const int* nb, int lane) {
+    using R = typename Codec::R;
+    const uint32_t wm = (fl & FLAG_NEARWALL) ? at(a.wmask, c) : 0u;
+    gather<Codec, kSparse>(fs, a, c, x, z, fl, wm, f, nb, lane);
+
+    R r, ux, uy, uz;
+    if constexpr (Codec::kExact) {
+        // 3. moments (Guo half-force shift)
+        r = f[0];
+        R mx = R(0), my = R(0), mz = R(0);
+        bool mx0 = true, my0 = true, mz0 = true;
+        static_for<Q - 1>([&](auto J) {
+            constexpr int i = J + 1;
+            r = r + f[i];
+            // first nonzero term starts the sum exactly (0 + v == v)
+            if constexpr (kCX[i] != 0) { mx = mx0 ? (kCX[i] > 0 ? f[i] : -f[i]) : accum<kCX[i]>(mx, f[i]); mx0 = false; }
+            if constexpr (kCY[i] != 0) { my = my0 ? (kCY[i] > 0 ? f[i] : -f[i]) : accum<kCY[i]>(my, f[i]); my0 = false; }
+            if constexpr (kCZ[i] != 0) { mz = mz0 ? (kCZ[i] > 0 ? f[i] : -f[i]) : accum<kCZ[i]>(mz, f[i]); mz0 = false; }
+        });
+        const bool forced = (fl & FLAG_FORCED) != 0;
+        R fx = R(0), fy = R(0), fz = R(0);
+        if (forced) {
+            fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale;
+            ux = (mx + R(0.5) * fx) / r; uy = (my + R(0.5) * fy) / r; uz = (mz + R(0.5) * fz) / r;
+        } else {
+            ux = mx / r; uy = my / r; uz = mz / r;
+        }
+        const R usq = ux * ux + uy * uy + uz * uz;
+
+        // non-equilibrium stress
+        R feq[Q];
+        R pxx = R(0), pyy = R(0), pzz = R(0), pxy = R(0), pxz = R(0), pyz = R(0);
+        bool pxx0 = true, pyy0 = true, pzz0 = true, pxy0 = true, pxz0 = true, pyz0 = true;
         static_for<Q>([&](auto I) {
             constexpr int i = I;
-            Codec::template st<i>(elem(fd + (long long)i * N, (uint32_t)c), f[i], store_key<Codec>(a.ctr, i));
+            constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+            const R cu = R(3) * dot3<cx, cy, cz>(ux, uy, uz);
+            feq[i] = weight<R>(i) * r * (R(1) + cu + R(0.5) * cu * cu - R(1.5) * usq);
+            const R fn = f[i] - feq[i];
+            if constexpr (cx != 0) { pxx = pxx0 ? fn : pxx + fn; pxx0 = false; }
+            if constexpr (cy != 0) { pyy = pyy0 ? fn : pyy + fn; pyy0 = false; }
+            if constexpr (cz != 0) { pzz = pzz0 ? fn : pzz + fn; pzz0 = false; }
+            if constexpr (cx * cy != 0) { pxy = pxy0 ? (cx * cy > 0 ? fn : -fn) : accum<cx * cy>(pxy, fn); pxy0 = false; }
+            if constexpr (cx * cz != 0) { pxz = pxz0 ? (cx * cz > 0 ? fn : -fn) : accum<cx * cz>(pxz, fn); pxz0 = false; }
+            if constexpr (cy * cz != 0) { pyz = pyz0 ? (cy * cz > 0 ? fn : -fn) : accum<cy * cz>(pyz, fn); pyz0 = false; }
         });
+
+        // Smagorinsky relaxation time
+        const R qn = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
+        const R tau = R(0.5) * (a.tau0 + sqrt(a.tau0sq + a.ksmag * qn / r));
+        const R om = R(1) / tau;
+        const R om1 = R(1) - om;
+
+        // regularized post-collision populations (kept in f until the store)
+        if (forced) {
+            const R sf = R(1) - R(0.5) * om;
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+                constexpr R third = R(1) / R(3);
+                constexpr R cxx = R(cx * cx) - third, cyy = R(cy * cy) - third, czz = R(cz * cz) - third;
+                R sp = cxx * pxx + cyy * pyy + czz * pzz;
+                if constexpr (cx * cy != 0) sp = sp + R(2) * (cx * cy > 0 ? pxy : -pxy);
+                else if constexpr (cx * cz != 0) sp = sp + R(2) * (cx * cz > 0 ? pxz : -pxz);
+                else if constexpr (cy * cz != 0) sp = sp + R(2) * (cy * cz > 0 ? pyz : -pyz);
+                constexpr R c45w = R(4.5) * weight<R>(i);
+                const R fneq = c45w * sp;
+                const R cu = dot3<cx, cy, cz>(ux, uy, uz);
+                const R cF = dot3<cx, cy, cz>(fx, fy, fz);
+                const R src = sf * weight<R>(i) *
+                              (R(3) * ((R(cx) - ux) * fx + (R(cy) - uy) * fy + (R(cz) - uz) * fz) + R(9) * cu * cF);
+                constexpr R cm15w = R(-1.5) * weight<R>(i);
+                const R fneq1 = cm15w * cF;
+                f[i] = feq[i] + om1 * (fneq + fneq1) + src;
+            });
+        } else {
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+                constexpr R third = R(1) / R(3);
+                constexpr R cxx = R(cx * cx) - third, cyy = R(cy * cy) - third, czz = R(cz * cz) - third;
+                R sp = cxx * pxx + cyy * pyy + czz * pzz;
+                if constexpr (cx * cy != 0) sp = sp + R(2) * (cx * cy > 0 ? pxy : -pxy);
+                else if constexpr (cx * cz != 0) sp = sp + R(2) * (cx * cz > 0 ? pxz : -pxz);
+                else if constexpr (cy * cz != 0) sp = sp + R(2) * (cy * cz > 0 ? pyz : -pyz);
+                constexpr R c45w = R(4.5) * weight<R>(i);
+                f[i] = feq[i] + om1 * (c45w * sp);
+            });
+        }
+    } else {
+        const bool forced = (fl & FLAG_FORCED) != 0;
+        R fx = R(0), fy = R(0), fz = R(0);
+        if (forced) { fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale; }
+        collide_pairs<R>(f, forced, fx, fy, fz, a.tau0, a.tau0sq, a.ksmag, r, ux, uy, uz);
     }
+    if (a.write_all || (wm != 0u && (fl & FLAG_CF))) {
+        at(a.rho, c) = r; at(a.ux, c) = ux; at(a.uy, c) = uy; at(a.uz, c) = uz;
+    }
+}
+
+// The wavefront holds the inlet row (x = 0) or the fused outlet row; xw is its
+// first x. Those wavefronts exchange post-collision populations across lanes.
+template <typename R>
+__device__ __forceinline__ bool boundary_wave(const StepArgs<R>& a, int xw) {
+    return xw == 0 || (a.bc_out_fused && xw <= a.nx - 1 && a.nx - 1 < xw + 64);
 }
 
 // One x-tile of one row: the whole step for its cells.
@@ -572,7 +756,8 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
     using R = typename Codec::R;
     const int lane = int(threadIdx.x) & 63;
     const int x = kSparse ? tile * 64 + lane : tile * int(blockDim.x) + int(threadIdx.x);
-    const int xw = kSparse ? tile * 64 : tile * int(blockDim.x) + (int(threadIdx.x) & ~63);
+    // first x of the wavefront, uniform (scalar register)
+    const int xw = __builtin_amdgcn_readfirstlane(kSparse ? tile * 64 : tile * int(blockDim.x) + (int(threadIdx.x) & ~63));
     const bool inrow = x < a.nx;
     const int z = row / a.ny;
     const int c = kSparse ? t * 64 + lane : x + a.px * row;
@@ -581,107 +766,17 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
     // for the boundary-row shuffles.
     const uint8_t fl = inrow ? at(a.flag, c) : uint8_t(FLAG_SOLID);
     const bool active = !(fl & FLAG_SOLID);
-    R f[Q];
-
-    if (active) {
-        const uint32_t wm = (fl & FLAG_NEARWALL) ? at(a.wmask, c) : 0u;
-        gather<Codec, kSparse>(fs, a, c, x, z, fl, wm, f, nb, lane);
-
-        R r, ux, uy, uz;
-        if constexpr (Codec::kExact) {
-            // 3. moments (Guo half-force shift)
-            r = f[0];
-            R mx = R(0), my = R(0), mz = R(0);
-            bool mx0 = true, my0 = true, mz0 = true;
-            static_for<Q - 1>([&](auto J) {
-                constexpr int i = J + 1;
-                r = r + f[i];
-                // first nonzero term starts the sum exactly (0 + v == v)
-                if constexpr (kCX[i] != 0) { mx = mx0 ? (kCX[i] > 0 ? f[i] : -f[i]) : accum<kCX[i]>(mx, f[i]); mx0 = false; }
-                if constexpr (kCY[i] != 0) { my = my0 ? (kCY[i] > 0 ? f[i] : -f[i]) : accum<kCY[i]>(my, f[i]); my0 = false; }
-                if constexpr (kCZ[i] != 0) { mz = mz0 ? (kCZ[i] > 0 ? f[i] : -f[i]) : accum<kCZ[i]>(mz, f[i]); mz0 = false; }
-            });
-            const bool forced = (fl & FLAG_FORCED) != 0;
-            R fx = R(0), fy = R(0), fz = R(0);
-            if (forced) {
-                fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale;
-                ux = (mx + R(0.5) * fx) / r; uy = (my + R(0.5) * fy) / r; uz = (mz + R(0.5) * fz) / r;
-            } else {
-                ux = mx / r; uy = my / r; uz = mz / r;
-            }
-            const R usq = ux * ux + uy * uy + uz * uz;
-
-            // non-equilibrium stress
-            R feq[Q];
-            R pxx = R(0), pyy = R(0), pzz = R(0), pxy = R(0), pxz = R(0), pyz = R(0);
-            bool pxx0 = true, pyy0 = true, pzz0 = true, pxy0 = true, pxz0 = true, pyz0 = true;
-            static_for<Q>([&](auto I) {
-                constexpr int i = I;
-                constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
-                const R cu = R(3) * dot3<cx, cy, cz>(ux, uy, uz);
-                feq[i] = weight<R>(i) * r * (R(1) + cu + R(0.5) * cu * cu - R(1.5) * usq);
-                const R fn = f[i] - feq[i];
-                if constexpr (cx != 0) { pxx = pxx0 ? fn : pxx + fn; pxx0 = false; }
-                if constexpr (cy != 0) { pyy = pyy0 ? fn : pyy + fn; pyy0 = false; }
-                if constexpr (cz != 0) { pzz = pzz0 ? fn : pzz + fn; pzz0 = false; }
-                if constexpr (cx * cy != 0) { pxy = pxy0 ? (cx * cy > 0 ? fn : -fn) : accum<cx * cy>(pxy, fn); pxy0 = false; }
-                if constexpr (cx * cz != 0) { pxz = pxz0 ? (cx * cz > 0 ? fn : -fn) : accum<cx * cz>(pxz, fn); pxz0 = false; }
-                if constexpr (cy * cz != 0) { pyz = pyz0 ? (cy * cz > 0 ? fn : -fn) : accum<cy * cz>(pyz, fn); pyz0 = false; }
-            });
-
-            // Smagorinsky relaxation time
-            const R qn = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2) * (pxy * pxy + pxz * pxz + pyz * pyz));
-            const R tau = R(0.5) * (a.tau0 + sqrt(a.tau0sq + a.ksmag * qn / r));
-            const R om = R(1) / tau;
-            const R om1 = R(1) - om;
-
-            // regularized post-collision populations (kept in f until the store)
-            if (forced) {
-                const R sf = R(1) - R(0.5) * om;
-                static_for<Q>([&](auto I) {
-                    constexpr int i = I;
-                    constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
-                    constexpr R third = R(1) / R(3);
-                    constexpr R cxx = R(cx * cx) - third, cyy = R(cy * cy) - third, czz = R(cz * cz) - third;
-                    R sp = cxx * pxx + cyy * pyy + czz * pzz;
-                    if constexpr (cx * cy != 0) sp = sp + R(2) * (cx * cy > 0 ? pxy : -pxy);
-                    else if constexpr (cx * cz != 0) sp = sp + R(2) * (cx * cz > 0 ? pxz : -pxz);
-                    else if constexpr (cy * cz != 0) sp = sp + R(2) * (cy * cz > 0 ? pyz : -pyz);
-                    constexpr R c45w = R(4.5) * weight<R>(i);
-                    const R fneq = c45w * sp;
-                    const R cu = dot3<cx, cy, cz>(ux, uy, uz);
-                    const R cF = dot3<cx, cy, cz>(fx, fy, fz);
-                    const R src = sf * weight<R>(i) *
-                                  (R(3) * ((R(cx) - ux) * fx + (R(cy) - uy) * fy + (R(cz) - uz) * fz) + R(9) * cu * cF);
-                    constexpr R cm15w = R(-1.5) * weight<R>(i);
-                    const R fneq1 = cm15w * cF;
-                    f[i] = feq[i] + om1 * (fneq + fneq1) + src;
-                });
-            } else {
-                static_for<Q>([&](auto I) {
-                    constexpr int i = I;
-                    constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
-                    constexpr R third = R(1) / R(3);
-                    constexpr R cxx = R(cx * cx) - third, cyy = R(cy * cy) - third, czz = R(cz * cz) - third;
-                    R sp = cxx * pxx + cyy * pyy + czz * pzz;
-                    if constexpr (cx * cy != 0) sp = sp + R(2) * (cx * cy > 0 ? pxy : -pxy);
-                    else if constexpr (cx * cz != 0) sp = sp + R(2) * (cx * cz > 0 ? pxz : -pxz);
-                    else if constexpr (cy * cz != 0) sp = sp + R(2) * (cy * cz > 0 ? pyz : -pyz);
-                    constexpr R c45w = R(4.5) * weight<R>(i);
-                    f[i] = feq[i] + om1 * (c45w * sp);
-                });
-            }
-        } else {
-            const bool forced = (fl & FLAG_FORCED) != 0;
-            R fx = R(0), fy = R(0), fz = R(0);
-            if (forced) { fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale; }
-            collide_pairs<R>(f, forced, fx, fy, fz, a.tau0, a.tau0sq, a.ksmag, r, ux, uy, uz);
+    if (!boundary_wave(a, xw)) {
+        // no cross-lane exchange: each active lane updates and stores its cell
+        if (active) {
+            R f[Q];
+            cell_update<Codec, kSparse>(fs, a, c, x, z, fl, f, nb, lane);
+            store_cell<Codec>(fd, a.N, a.ctr, f, c);
         }
-        if (a.write_all || (wm != 0u && (fl & FLAG_CF))) {
-            at(a.rho, c) = r; at(a.ux, c) = ux; at(a.uy, c) = uy; at(a.uz, c) = uz;
-        }
+        return;
     }
-
+    R f[Q];
+    if (active) cell_update<Codec, kSparse>(fs, a, c, x, z, fl, f, nb, lane);
     finish<Codec>(fd, a, f, c, x, row, xw, active);
 }
 
