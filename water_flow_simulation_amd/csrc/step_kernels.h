@@ -367,8 +367,11 @@ struct PairMoments {
     R r, ri, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz, qn;
 };
 
-template <typename R>
-__device__ __forceinline__ void pair_moments(const R* f, bool forced, R fx, R fy, R fz, PairMoments<R>& m) {
+// kForced: the wavefront holds a forced cell (fx, fy, fz zero on the others,
+// where the forced formulas give the unforced values); otherwise the force
+// terms are left out.
+template <typename R, bool kForced>
+__device__ __forceinline__ void pair_moments(const R* f, R fx, R fy, R fz, PairMoments<R>& m) {
     // pair sums and differences (i, i+1) for i = 1, 3, ..., 17
     R sp[9], dp[9];
 #pragma unroll
@@ -383,7 +386,7 @@ __device__ __forceinline__ void pair_moments(const R* f, bool forced, R fx, R fy
     const R mz = dp[2] + dp[5] - dp[6] + dp[7] - dp[8];
     const R ri = pair_rcp(r);
     R ux, uy, uz;
-    if (forced) {
+    if constexpr (kForced) {
         ux = fma(R(0.5), fx, mx) * ri; uy = fma(R(0.5), fy, my) * ri; uz = fma(R(0.5), fz, mz) * ri;
     } else {
         ux = mx * ri; uy = my * ri; uz = mz * ri;
@@ -401,9 +404,9 @@ __device__ __forceinline__ void pair_moments(const R* f, bool forced, R fx, R fy
 }
 
 // Relaxation of the pair form with eddy-viscosity constant ksmag = 18 sqrt(2) C^2.
-template <typename R>
-__device__ __forceinline__ void pair_relax(R* f, bool forced, R fx, R fy, R fz, const PairMoments<R>& m, R tau0,
-                                           R tau0sq, R ksmag) {
+template <typename R, bool kForced>
+__device__ __forceinline__ void pair_relax(R* f, R fx, R fy, R fz, const PairMoments<R>& m, R tau0, R tau0sq,
+                                           R ksmag) {
     const R r = m.r, ux = m.ux, uy = m.uy, uz = m.uz;
     const R pxx = m.pxx, pyy = m.pyy, pzz = m.pzz, pxy = m.pxy, pxz = m.pxz, pyz = m.pyz;
     const R tau = R(0.5) * (tau0 + pair_sqrt(fma(ksmag * m.qn, m.ri, tau0sq)));
@@ -415,7 +418,7 @@ __device__ __forceinline__ void pair_relax(R* f, bool forced, R fx, R fy, R fz, 
     const R wr0 = r * (R(1) / R(3)), wr1 = r * (R(1) / R(18)), wr2 = r * (R(1) / R(36));
     const R g1 = om1 * R(0.25), g2 = om1 * R(0.125);   // om1 * 4.5 w for w = 1/18, 1/36
     R sf = R(0), uF = R(0), h1 = R(0), h2 = R(0);
-    if (forced) {
+    if constexpr (kForced) {
         sf = R(1) - R(0.5) * om;
         uF = fma(ux, fx, fma(uy, fy, uz * fz));
         const R odd = fma(R(3), sf, R(-1.5) * om1);   // (3 sf - 1.5 om1)
@@ -425,14 +428,14 @@ __device__ __forceinline__ void pair_relax(R* f, bool forced, R fx, R fy, R fz, 
     // rest population
     {
         R e = fma(om1 * R(1.5), -t3, wr0 * k0);
-        if (forced) e = fma(sf * (R(1) / R(3)), R(-3) * uF, e);
+        if constexpr (kForced) e = fma(sf * (R(1) / R(3)), R(-3) * uF, e);
         f[0] = e;
     }
     auto pair = [&](int k, R cu, R nq, R cF, R wr, R g, R w, R h) {
         const R cu2 = cu * cu;
         R e = fma(g, nq, wr * fma(R(4.5), cu2, k0));
         R o = R(3) * wr * cu;
-        if (forced) {
+        if constexpr (kForced) {
             e = fma(w * sf, fma(R(9) * cu, cF, R(-3) * uF), e);
             o = fma(h, cF, o);
         }
@@ -452,12 +455,12 @@ __device__ __forceinline__ void pair_relax(R* f, bool forced, R fx, R fy, R fz, 
     pair(8, uy - uz, fma(R(-2), pyz, syz), fy - fz, wr2, g2, R(1) / R(36), h2);
 }
 
-template <typename R>
-__device__ __forceinline__ void collide_pairs(R* f, bool forced, R fx, R fy, R fz, R tau0, R tau0sq, R ksmag,
-                                              R& r, R& ux, R& uy, R& uz) {
+template <typename R, bool kForced>
+__device__ __forceinline__ void collide_pairs(R* f, R fx, R fy, R fz, R tau0, R tau0sq, R ksmag, R& r, R& ux, R& uy,
+                                              R& uz) {
     PairMoments<R> m;
-    pair_moments<R>(f, forced, fx, fy, fz, m);
-    pair_relax<R>(f, forced, fx, fy, fz, m, tau0, tau0sq, ksmag);
+    pair_moments<R, kForced>(f, fx, fy, fz, m);
+    pair_relax<R, kForced>(f, fx, fy, fz, m, tau0, tau0sq, ksmag);
     r = m.r; ux = m.ux; uy = m.uy; uz = m.uz;
 }
 
@@ -506,10 +509,10 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
             constexpr int i = I;
             if constexpr (!kSparse) {
                 // the pull offset is uniform: it moves the direction's buffer base
-                // (i N - offset >= 0 for every direction); one lane offset for all 19
+                // (i N - offset >= 0 for every direction); one lane offset for all
+                // 19, each below N, which the direction's size bounds
                 using T = typename Codec::T;
                 const long long off = kCX[i] + (long long)px * kCY[i] + (long long)plane * kCZ[i];
-                // lane offsets c < N: the direction's size bounds them
                 const T v = buf_load<T>(buf_rsrc(fs + ((long long)i * N - off), (uint32_t)N * (uint32_t)sizeof(T)),
                                         (uint32_t)c * (uint32_t)sizeof(T));
                 f[i] = Codec::template dec<i>(v, a.ctr.c[center_class(i)]);
@@ -730,9 +733,13 @@ __device__ __forceinline__ void cell_update(const typename Codec::T* __restrict_
         }
     } else {
         const bool forced = (fl & FLAG_FORCED) != 0;
-        R fx = R(0), fy = R(0), fz = R(0);
-        if (forced) { fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale; }
-        collide_pairs<R>(f, forced, fx, fy, fz, a.tau0, a.tau0sq, a.ksmag, r, ux, uy, uz);
+        if (__any(forced)) {   // uniform over the wavefront; forced cells are few
+            R fx = R(0), fy = R(0), fz = R(0);
+            if (forced) { fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale; }
+            collide_pairs<R, true>(f, fx, fy, fz, a.tau0, a.tau0sq, a.ksmag, r, ux, uy, uz);
+        } else {
+            collide_pairs<R, false>(f, R(0), R(0), R(0), a.tau0, a.tau0sq, a.ksmag, r, ux, uy, uz);
+        }
     }
     if (a.write_all || (wm != 0u && (fl & FLAG_CF))) {
         at(a.rho, c) = r; at(a.ux, c) = ux; at(a.uy, c) = uy; at(a.uz, c) = uz;
@@ -798,7 +805,8 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     const int* nb = kSparse ? a.nbr + 27 * t : nullptr;
     const uint8_t fl = inrow ? at(a.flag, c) : uint8_t(FLAG_SOLID);
     const bool active = !(fl & FLAG_SOLID);
-    const bool forced = (fl & FLAG_FORCED) != 0;
+    const bool forced = active && (fl & FLAG_FORCED) != 0;
+    const bool any_forced = __any(forced);   // uniform over the wavefront
     R f[Q];
     PairMoments<R> m;
     R fx = R(0), fy = R(0), fz = R(0);
@@ -808,7 +816,8 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
         wm = (fl & FLAG_NEARWALL) ? at(a.wmask, c) : 0u;
         gather<Codec, kSparse>(fs, a, c, x, z, fl, wm, f, nb, lane);
         if (forced) { fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale; }
-        pair_moments<R>(f, forced, fx, fy, fz, m);
+        if (any_forced) pair_moments<R, true>(f, fx, fy, fz, m);
+        else pair_moments<R, false>(f, fx, fy, fz, m);
         // resolved strain rate S = -3 Pi_dev / (2 rho tau), tau from the constant-Cs model
         const R tau_s = R(0.5) * (a.tau0 + pair_sqrt(fmaf(a.ksmag * m.qn, m.ri, a.tau0sq)));
         const R k = R(-1.5) * m.ri * __builtin_amdgcn_rcpf(tau_s);
@@ -819,7 +828,9 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     const float y = mlp_wave(feat, P);
     if (active) {
         const float c2 = kC2Min + kC2Scale * softplus_f(y);
-        pair_relax<R>(f, forced, fx, fy, fz, m, a.tau0, a.tau0sq, R(18) * R(1.41421356) * c2);
+        const R ks = R(18) * R(1.41421356) * c2;
+        if (any_forced) pair_relax<R, true>(f, fx, fy, fz, m, a.tau0, a.tau0sq, ks);
+        else pair_relax<R, false>(f, fx, fy, fz, m, a.tau0, a.tau0sq, ks);
         if (a.write_all || (wm != 0u && (fl & FLAG_CF))) {
             at(a.rho, c) = m.r; at(a.ux, c) = m.ux; at(a.uy, c) = m.uy; at(a.uz, c) = m.uz;
         }
