@@ -219,43 +219,54 @@ __device__ __forceinline__ T& at(T* base, int idx) {
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
 }
+// soff: a uniform byte offset added to the lane offset (scalar register)
 template <typename T>
-__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff = 0) {
     if constexpr (sizeof(T) == 2) {
-        return __builtin_bit_cast(T, (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0));
+        return __builtin_bit_cast(T, (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, off, soff, 0));
     } else if constexpr (sizeof(T) == 4) {
-        return __builtin_bit_cast(T, (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+        return __builtin_bit_cast(T, (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
     } else {
         static_assert(sizeof(T) == 8, "2, 4 or 8 bytes");
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
     }
 }
 // cache policy of population stores: non-temporal (nt) when WFS_NT_STORE
 constexpr int kPopStoreAux = WFS_NT_STORE ? 2 : 0;
 template <typename T>
-__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, T v) {
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, T v, uint32_t soff = 0) {
     if constexpr (sizeof(T) == 2) {
-        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, off, 0, kPopStoreAux);
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, off, soff, kPopStoreAux);
     } else if constexpr (sizeof(T) == 4) {
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, kPopStoreAux);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, soff, kPopStoreAux);
     } else {
         using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, kPopStoreAux);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, soff, kPopStoreAux);
     }
 }
 
 // Stores the post-collision populations of cell c (all lanes that call it).
+// pop32: one resource for the whole buffer, direction i at the uniform offset
+// i N (fewer scalar instructions); otherwise one resource per direction.
 template <class Codec>
 __device__ __forceinline__ void store_cell(typename Codec::T* __restrict__ fd, long long N, const Center& ctr,
-                                           const typename Codec::R* f, int c) {
+                                           const typename Codec::R* f, int c, bool pop32) {
     using T = typename Codec::T;
     const uint32_t off = (uint32_t)c * (uint32_t)sizeof(T);
     const uint32_t bytes = (uint32_t)N * (uint32_t)sizeof(T);
-    static_for<Q>([&](auto I) {
-        constexpr int i = I;
-        buf_store<T>(buf_rsrc(fd + (long long)i * N, bytes), off,
-                     Codec::template enc<i>(f[i], store_key<Codec>(ctr, i)));
-    });
+    if (pop32) {
+        const auto r = buf_rsrc(fd, (uint32_t)Q * bytes);
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            buf_store<T>(r, off, Codec::template enc<i>(f[i], store_key<Codec>(ctr, i)), (uint32_t)i * bytes);
+        });
+    } else {
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            buf_store<T>(buf_rsrc(fd + (long long)i * N, bytes), off,
+                         Codec::template enc<i>(f[i], store_key<Codec>(ctr, i)));
+        });
+    }
 }
 
 template <typename R>
@@ -283,6 +294,7 @@ struct StepArgs {
     const R* __restrict__ uin;  // inlet velocity per row (ny*nz)
     Center ctr;                 // per-direction centre of the 16-bit storage formats
     int bc_out_fused;           // outlet row inside the step ((nx-1) % 64 != 0)
+    int pop32;                  // all 19 directions of a population buffer lie within 4 GiB
     int write_all;
     // learned closure (k_step_learned only)
     const uint16_t* __restrict__ geo;   // per cell: 255/d | 255 z_rel << 8
@@ -504,23 +516,46 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
     const bool xlo = x == 0, xhi = x == a.nx - 1, top = z == a.nz - 1;
     // 1. pull streaming
     if (!(fl & FLAG_NEARWALL) && !xlo && !xhi && !top) {
-        // interior: every source is the plain upstream neighbour
-        static_for<Q>([&](auto I) {
+        // interior: every source is the plain upstream neighbour, at a pull base
+        // uniform over the wavefront plus the lane's own offset:
+        //   dense:  (i N - (cx + px cy + plane cz)) + c
+        //   sparse: (i N + 64 ts - cx) + lane, ts the tile of the (dy, dz) row;
+        // stored tiles are numbered row by row with x ascending, so for a cell
+        // with no wall link the x-neighbour tile of ts is ts -+ 1 and the lane
+        // that crosses into it needs no special case.
+        using T = typename Codec::T;
+        constexpr uint32_t ts = (uint32_t)sizeof(T);
+        const uint32_t voff = (uint32_t)(kSparse ? lane : c) * ts;
+        auto row_of = [&](auto I) -> int {   // sparse: source tile of direction I
             constexpr int i = I;
-            if constexpr (!kSparse) {
-                // the pull offset is uniform: it moves the direction's buffer base
-                // (i N - offset >= 0 for every direction); one lane offset for all
-                // 19, each below N, which the direction's size bounds
-                using T = typename Codec::T;
-                const long long off = kCX[i] + (long long)px * kCY[i] + (long long)plane * kCZ[i];
-                const T v = buf_load<T>(buf_rsrc(fs + ((long long)i * N - off), (uint32_t)N * (uint32_t)sizeof(T)),
-                                        (uint32_t)c * (uint32_t)sizeof(T));
-                f[i] = Codec::template dec<i>(v, a.ctr.c[center_class(i)]);
-            } else {
-                const int s = pull_source<kSparse, kCX[i], kCY[i], kCZ[i]>(c, px, plane, nb, lane);
-                f[i] = Codec::template ld<i>(elem(fs + (long long)i * N, (uint32_t)s), a.ctr.c[center_class(i)]);
-            }
-        });
+            return nb[1 + 3 * ((1 - kCY[i]) + 3 * (1 - kCZ[i]))];
+        };
+        if (a.pop32) {
+            // one resource for the buffer, the pull base as a uniform byte offset
+            const auto rs = buf_rsrc(fs, (uint32_t)Q * (uint32_t)N * ts);
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                uint32_t base;   // elements, exact modulo 2^32 (the true value is in [0, Q N))
+                if constexpr (kSparse)
+                    base = (uint32_t)i * (uint32_t)N + (uint32_t)row_of(I) * 64u - (uint32_t)kCX[i];
+                else
+                    base = (uint32_t)i * (uint32_t)N - (uint32_t)kCX[i] - (uint32_t)px * kCY[i] -
+                           (uint32_t)plane * kCZ[i];
+                f[i] = Codec::template dec<i>(buf_load<T>(rs, voff, base * ts), a.ctr.c[center_class(i)]);
+            });
+        } else {
+            // one resource per direction at its pull base (i N + ... >= 0)
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                long long base;
+                if constexpr (kSparse)
+                    base = (long long)i * N + (long long)row_of(I) * 64 - kCX[i];
+                else
+                    base = (long long)i * N - (kCX[i] + (long long)px * kCY[i] + (long long)plane * kCZ[i]);
+                f[i] = Codec::template dec<i>(buf_load<T>(buf_rsrc(fs + base, (uint32_t)N * ts), voff),
+                                              a.ctr.c[center_class(i)]);
+            });
+        }
     } else {
         static_for<Q>([&](auto I) {
             constexpr int i = I;
@@ -633,7 +668,7 @@ __device__ __forceinline__ void finish(typename Codec::T* __restrict__ fd, const
         }
     }
 
-    if (active) store_cell<Codec>(fd, N, a.ctr, f, c);
+    if (active) store_cell<Codec>(fd, N, a.ctr, f, c, a.pop32 != 0);
 }
 
 // The update of one active cell: pull, wall function, collision; f gets the
@@ -778,7 +813,7 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
         if (active) {
             R f[Q];
             cell_update<Codec, kSparse>(fs, a, c, x, z, fl, f, nb, lane);
-            store_cell<Codec>(fd, a.N, a.ctr, f, c);
+            store_cell<Codec>(fd, a.N, a.ctr, f, c, a.pop32 != 0);
         }
         return;
     }

@@ -424,6 +424,7 @@ struct Solver final : wfs_solver {
         a.N = Np;
         a.uin = rp(uin);
         a.bc_out_fused = (d.nx - 1) % 64 != 0 ? 1 : 0;
+        a.pop32 = (unsigned long long)Q * (unsigned long long)Np * sizeof(T) < (1ull << 32) ? 1 : 0;
         a.write_all = 0;
         a.ctr = ctr;
         a.geo = static_cast<const uint16_t*>(geo.p);
