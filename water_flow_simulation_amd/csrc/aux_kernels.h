@@ -408,13 +408,37 @@ __global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __r
     if (threadIdx.x < 9) part[(long long)blockIdx.x * 9 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// out[k] = sum over blocks b of part[b * nv + k], in block order.
-__global__ void k_sum_parts(const double* __restrict__ part, long long blocks, int nv, double* __restrict__ out) {
-    const int k = int(threadIdx.x);
-    if (k >= nv) return;
-    double s = 0.0;
-    for (long long b = 0; b < blocks; ++b) s += part[b * nv + k];
-    out[k] = s;
+// Fixed-order FP64 sum of per-block partials: out[g * NV + k] = sum of
+// part[b * NV + k] over the b of chunk g, [g chunk, (g + 1) chunk). Thread t
+// takes b = g chunk + t, t + 256, ...; the 256 thread sums meet in a fixed LDS
+// tree. Two launches (chunks, then one work-group over the chunk sums) give a
+// result that does not depend on timing.
+template <int NV>
+__global__ void __launch_bounds__(256) k_sum_parts(const double* __restrict__ part, long long blocks, long long chunk,
+                                                   double* __restrict__ out) {
+    __shared__ double red[NV][256];
+    const int t = int(threadIdx.x);
+    const long long b0 = (long long)blockIdx.x * chunk;
+    const long long b1 = b0 + chunk < blocks ? b0 + chunk : blocks;
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (long long b = b0 + t; b < b1; b += 256) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += part[b * NV + k];
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[k][t] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) red[k][t] += red[k][t + w];
+        }
+        __syncthreads();
+    }
+    if (t < NV) out[(long long)blockIdx.x * NV + t] = red[t][0];
 }
+constexpr long long kSumChunks = 1024;   // work-groups of the first pass
 
 }  // namespace wfs

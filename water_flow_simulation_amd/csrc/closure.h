@@ -28,6 +28,7 @@
 //     A = |S|^2 S:S,  B = |S| (T:S),  P = T:T,
 // so a sample carries (A, B, P) besides its features.
 #pragma once
+#include <cstddef>
 
 #include <hip/hip_runtime.h>
 
@@ -66,15 +67,18 @@ typedef uint32_t u4v __attribute__((ext_vector_type(4)));
 // B-operand slots for k-step s of the next 32x32x16 product; the matching
 // A-operand slots of W2 are packed in the same unit order, so no data moves
 // between lanes from layer 1 to layer 2.
+// The per-lane-row vectors come first, within reach of the LDS pair-read
+// offsets (1020 bytes) from one base per lane; w2 stays 16-byte aligned.
 struct MlpPack {
-    uint32_t w1[64 * 2];       // lane l: BF16 W1[l%32][4h .. 4h+3] (32x32x8 A operand)
-    uint32_t w2[2 * 64 * 4];   // step s, lane l: BF16 W2[l%32][unit(s, h, 0..7)]
     float b1[2 * 16];          // [h][r]: b1[row(r, h)], the layer-1 accumulator start
     float b2[2 * 16];          // [h][r]: b2[row(r, h)]
     float w3[2 * 16];          // [h][r]: w3[row(r, h)]
     float b3;
     float pad[3];
+    uint32_t w1[64 * 2];       // lane l: BF16 W1[l%32][4h .. 4h+3] (32x32x8 A operand)
+    uint32_t w2[2 * 64 * 4];   // step s, lane l: BF16 W2[l%32][unit(s, h, 0..7)]
 };
+static_assert(offsetof(MlpPack, w2) % 16 == 0, "w2 is read as 16-byte vectors");
 constexpr int kPackWords = (int)(sizeof(MlpPack) / 4);
 
 __host__ __device__ constexpr int mfma_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
@@ -114,6 +118,13 @@ __device__ __forceinline__ void closure_features(float sxx, float syy, float szz
     x[7] = float(hcol) * (1.f / 64.f);
 }
 
+// ReLU as one integer max on the bit pattern: negative floats (and -0) have
+// negative int32 patterns and become +0; others are unchanged. fmaxf with 0
+// costs two instructions here (the operand is first made a canonical NaN).
+__device__ __forceinline__ float relu_f(float v) {
+    return __int_as_float(max(__float_as_int(v), 0));
+}
+
 // Network output y for the cell of this lane. All 64 lanes of the wavefront
 // must call this together (it exchanges data between lanes and runs MFMA).
 // P points to the pack in LDS.
@@ -132,28 +143,32 @@ __device__ __forceinline__ float mlp_wave(const float* x, const MlpPack* P) {
     const s4v a1 = __builtin_bit_cast(s4v, u2v{P->w1[2 * lane], P->w1[2 * lane + 1]});
     const bf8v a2s0 = __builtin_bit_cast(bf8v, *reinterpret_cast<const u4v*>(&P->w2[(0 * 64 + lane) * 4]));
     const bf8v a2s1 = __builtin_bit_cast(bf8v, *reinterpret_cast<const u4v*>(&P->w2[(1 * 64 + lane) * 4]));
+    // this lane's rows of the biases and output weights (one LDS base, constant offsets)
+    const float* b1 = P->b1 + 16 * h;
+    const float* b2 = P->b2 + 16 * h;
+    const float* w3 = P->w3 + 16 * h;
     float y = 0.f;
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
         const u2v xb = (b == h) ? own : other;
         f16v acc;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] = P->b1[h * 16 + r];
+        for (int r = 0; r < 16; ++r) acc[r] = b1[r];
         acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, __builtin_bit_cast(s4v, xb), acc, 0, 0, 0);
         u4v hq0, hq1;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            hq0[q] = pk_bf16(fmaxf(acc[2 * q], 0.f), fmaxf(acc[2 * q + 1], 0.f));
-            hq1[q] = pk_bf16(fmaxf(acc[8 + 2 * q], 0.f), fmaxf(acc[8 + 2 * q + 1], 0.f));
+            hq0[q] = pk_bf16(relu_f(acc[2 * q]), relu_f(acc[2 * q + 1]));
+            hq1[q] = pk_bf16(relu_f(acc[8 + 2 * q]), relu_f(acc[8 + 2 * q + 1]));
         }
         f16v acc2;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc2[r] = P->b2[h * 16 + r];
+        for (int r = 0; r < 16; ++r) acc2[r] = b2[r];
         acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2s0, __builtin_bit_cast(bf8v, hq0), acc2, 0, 0, 0);
         acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2s1, __builtin_bit_cast(bf8v, hq1), acc2, 0, 0, 0);
         float part = 0.f;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) part = fmaf(P->w3[h * 16 + r], fmaxf(acc2[r], 0.f), part);
+        for (int r = 0; r < 16; ++r) part = fmaf(w3[r], relu_f(acc2[r]), part);
         const float yb = part + __shfl_xor(part, 32) + P->b3;
         y = (b == h) ? yb : y;
     }

@@ -687,7 +687,8 @@ struct Solver final : wfs_solver {
         const long long blocks = (Np + 255) / 256;
         DevBuf part, sum;
         int rc;
-        if ((rc = alloc_buf(part, (size_t)blocks * 9 * sizeof(double))) || (rc = alloc_buf(sum, 9 * sizeof(double))))
+        if ((rc = alloc_buf(part, (size_t)(blocks + kSumChunks) * 9 * sizeof(double))) ||
+            (rc = alloc_buf(sum, 9 * sizeof(double))))
             return rc;
         if (tiled)
             hipLaunchKernelGGL((k_wall_force<Codec, true>), dim3((unsigned)blocks), dim3(256), 0, stream, fp(cur), a,
@@ -696,8 +697,14 @@ struct Solver final : wfs_solver {
             hipLaunchKernelGGL((k_wall_force<Codec, false>), dim3((unsigned)blocks), dim3(256), 0, stream, fp(cur),
                                a, cmap(), static_cast<const uint8_t*>(body.p), static_cast<double*>(part.p));
         WFS_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, stream, static_cast<const double*>(part.p), blocks, 9,
-                           static_cast<double*>(sum.p));
+        // fixed-order two-pass sum: kSumChunks chunk sums, then their sum
+        const long long chunk = (blocks + kSumChunks - 1) / kSumChunks;
+        const long long nch = (blocks + chunk - 1) / chunk;
+        hipLaunchKernelGGL(k_sum_parts<9>, dim3((unsigned)nch), dim3(256), 0, stream, static_cast<const double*>(part.p),
+                           blocks, chunk, static_cast<double*>(part.p) + blocks * 9);
+        WFS_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_sum_parts<9>, dim3(1), dim3(256), 0, stream,
+                           static_cast<const double*>(part.p) + blocks * 9, nch, nch, static_cast<double*>(sum.p));
         WFS_LAUNCH_CHECK();
         WFS_HIP(hipMemcpyAsync(out, sum.p, 9 * sizeof(double), hipMemcpyDeviceToHost, stream));
         WFS_HIP(hipStreamSynchronize(stream));
