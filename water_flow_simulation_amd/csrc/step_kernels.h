@@ -557,31 +557,83 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
             });
         }
     } else {
-        static_for<Q>([&](auto I) {
+        // Link by link (in this order of precedence): the x-end rows keep their
+        // own population (xout: own cell, same direction), a wall link bounces
+        // back (own cell, opposite direction), a link from above the lid
+        // mirrors (in-plane source, z-mirrored direction; uniform, since top is
+        // the row's), any other link pulls. The stored direction is one of
+        // these constants, so a lane only selects among uniform offsets: the
+        // lowest candidate direction is the uniform base and each lane adds a
+        // non-negative multiple of N. No branches, no 64-bit products per lane.
+        using T = typename Codec::T;
+        constexpr uint32_t ts = (uint32_t)sizeof(T);
+        auto link = [&](auto I, int& s, bool& xout, bool& wall, float& e) {
             constexpr int i = I;
             constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
-            const bool xout = (cx == 1 && xlo) || (cx == -1 && xhi);
-            const bool wall = (wm >> i) & 1u;
-            int dir = i, s = c;
-            if (xout) {
-                dir = i;
-                s = c;
-            } else if (wall) {
-                dir = kOPP[i];
-                s = c;
-            } else if (cz == -1 && top) {
-                dir = kMIRZ[i];
-                s = pull_source<kSparse, cx, cy, 0>(c, px, plane, nb, lane);
+            constexpr int io = kOPP[i], im = kMIRZ[i];
+            xout = (cx == 1 && xlo) || (cx == -1 && xhi);
+            wall = !xout && ((wm >> i) & 1u);
+            const bool mirror = cz == -1 && top;
+            s = (xout || wall) ? c
+                : mirror ? pull_source<kSparse, cx, cy, 0>(c, px, plane, nb, lane)
+                         : pull_source<kSparse, cx, cy, cz>(c, px, plane, nb, lane);
+            (void)im;
+            // decode with the centre of the direction actually read (the mirror
+            // keeps c_x, so only the opposite direction has another centre); both
+            // centres pinned in scalar registers by an empty asm, or a select of
+            // the two loads becomes a per-lane load from a selected address
+            constexpr int ki = center_class(i), ko = center_class(io);
+            if constexpr (ki != ko) {
+                float ei = a.ctr.c[ki], eo = a.ctr.c[ko];
+                asm("" : "+s"(ei));
+                asm("" : "+s"(eo));
+                e = wall ? eo : ei;
             } else {
-                s = pull_source<kSparse, cx, cy, cz>(c, px, plane, nb, lane);
+                e = a.ctr.c[ki];
             }
-                // decode with the centre of the direction actually read
-            // (the mirror keeps c_x, so only the opposite direction has another centre)
-            constexpr int ki = center_class(i), ko = center_class(kOPP[i]);
-            const float e_i = a.ctr.c[ki];
-            const float e = (ki != ko && dir == kOPP[i]) ? a.ctr.c[ko] : e_i;
-            f[i] = Codec::template ld<i>(elem(fs + (long long)dir * N, (uint32_t)s), e);
-        });
+        };
+        if (a.pop32) {
+            const uint32_t nn = (uint32_t)N, nn2 = 2u * nn, nn3 = 3u * nn;
+            const auto rs = buf_rsrc(fs, (uint32_t)Q * nn * ts);
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                constexpr int io = kOPP[i], im = kMIRZ[i];
+                int s;
+                bool xout, wall;
+                float e;
+                link(I, s, xout, wall, e);
+                // direction read: i (xout), io (wall), else i or, under the lid,
+                // im. The lowest candidate is the uniform base; the others are at
+                // most 3 directions above it (opposites are adjacent pairs).
+                constexpr int lo0 = io < i ? io : i;
+                constexpr int lo1 = im < lo0 ? im : lo0;
+                constexpr int di = i - lo0, dop = io - lo0;
+                uint32_t adj = wall ? (dop ? nn : 0u) : (di ? nn : 0u);
+                uint32_t base = (uint32_t)lo0 * nn;
+                if constexpr (kCZ[i] == -1) {
+                    static_assert(i - lo1 <= 3 && io - lo1 <= 3 && im - lo1 <= 3, "candidates within 3");
+                    constexpr int ti = i - lo1, to = io - lo1, tm = im - lo1;
+                    auto times = [&](int k) { return k == 0 ? 0u : (k == 1 ? nn : (k == 2 ? nn2 : nn3)); };
+                    const uint32_t ai = times(ti), ao = times(to), am = times(tm);
+                    if (top) {
+                        adj = xout ? ai : (wall ? ao : am);
+                        base = (uint32_t)lo1 * nn;
+                    }
+                }
+                f[i] = Codec::template dec<i>(buf_load<T>(rs, ((uint32_t)s + adj) * ts, base * ts), e);
+            });
+        } else {
+            static_for<Q>([&](auto I) {
+                constexpr int i = I;
+                int s;
+                bool xout, wall;
+                float e;
+                link(I, s, xout, wall, e);
+                const T* bd = fs + (long long)((kCZ[i] == -1 && top) ? kMIRZ[i] : i) * N;
+                const T* b = xout ? fs + (long long)i * N : (wall ? fs + (long long)kOPP[i] * N : bd);
+                f[i] = Codec::template dec<i>(*elem(b, (uint32_t)s), e);
+            });
+        }
     }
 
     // 2. wall function
