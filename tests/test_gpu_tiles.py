@@ -85,3 +85,31 @@ def test_tiles_learned_closure_and_force_budget():
     assert np.abs(fa["body"]).max() > 0 and np.abs(fa["thrust"]).max() > 0
     for run in runs:
         run.close()
+
+
+def test_tiles_force_split_with_hull_longer_than_a_tile():
+    """A 12 m hull is 150 cells long, so some 64-cell runs inside it are all
+    solid and the tile layout stores no tile for them. The hull's share of the
+    momentum budget must still match the dense layout (the body mask is looked
+    up by dense cell, not by stored cell)."""
+    print("case: small meander, one 12 m hull, f32, forces dense vs tiles", flush=True)
+    from water_flow_simulation_amd.meander import MeanderConfig, build_meander
+    mc = MeanderConfig(length=40.0, amplitude=3.0, wavelength=40.0, bed=8.0, bank=2.0, depth=2.0,
+                       rows=(14.0,), per_row=1)
+    r = build_meander(mc)
+    hull = r.hull_cells
+    nz, ny, nx = r.solid.shape
+    ntx = nx // 64
+    runs_solid = r.solid[:, :, :ntx * 64].reshape(nz, ny, ntx, 64).all(-1)
+    runs_hull = hull[:, :, :ntx * 64].reshape(nz, ny, ntx, 64).any(-1)
+    assert (runs_solid & runs_hull).any(), "the case must hold an all-solid tile with hull cells"
+    runs = [R.Run3D(replace(r, layout=lay), storage="f32") for lay in ("dense", "tiles")]
+    for run in runs:
+        run.advance(20)
+    fa, fb = (run.forces() for run in runs)
+    for k in fa:
+        np.testing.assert_allclose(fb[k], fa[k], rtol=1e-12, atol=1e-9 * max(1.0, np.abs(fa[k]).max()))
+    assert np.abs(fa["body"][2]) > 0
+    for run in runs:
+        run.close()
+

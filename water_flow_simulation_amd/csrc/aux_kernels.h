@@ -354,7 +354,7 @@ namespace wfs {
 template <class Codec, bool kSparse>
 __global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __restrict__ fs,
                                                     StepArgs<typename Codec::R> a, CellMap cm,
-                                                    const uint8_t* __restrict__ body, double* __restrict__ part) {
+                                                    const uint32_t* __restrict__ body, double* __restrict__ part) {
     using R = typename Codec::R;
     __shared__ double red[9][256];
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -385,8 +385,9 @@ __global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __r
                         const double m = double(g) + double(f[i]);
                         const int xs = x - kCX[i], ys = y - kCY[i], zs = z - kCZ[i];
                         const bool in = xs >= 0 && xs < a.nx && ys >= 0 && ys < a.ny && zs >= 0 && zs < a.nz;
-                        const long long sc = in ? cell_of(cm, xs, ys + (long long)a.ny * zs) : -1;
-                        const bool hb = sc >= 0 && body && body[sc];
+                        // the wall cell's bit in the dense body mask
+                        const long long d = xs + (long long)a.nx * (ys + (long long)a.ny * zs);
+                        const bool hb = in && body && ((body[d >> 5] >> (d & 31)) & 1u);
                         const int k = hb ? 3 : 6;
                         acc[k] -= kCX[i] * m;
                         acc[k + 1] -= kCY[i] * m;

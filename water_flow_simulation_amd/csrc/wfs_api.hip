@@ -172,7 +172,7 @@ struct Solver final : wfs_solver {
     DevBuf f[2], flag, wmask, Fx, Fy, Fz, rho, ux, uy, uz, cfw, uref, uin, bedwall;
     DevBuf sums[6], partial, out_x, counter, stage;
     DevBuf geo, hcol, mlp_params, mlp_pack, c2buf;   // learned closure (allocated when first set)
-    DevBuf body;                                      // hull cells for the force budget (when set)
+    DevBuf body;                                      // hull cells for the force budget (when set): one bit per dense cell
     DevBuf tmap_d, tcoord_d, nbr_d, blk2tile_d, blk2tile4_d;   // tile-sparse tables (when tiled)
 
     CellMap cmap() const {
@@ -669,16 +669,24 @@ struct Solver final : wfs_solver {
         }
     }
 
+    // The mask is kept as one bit per dense cell (x + nx (y + ny z)), so that
+    // hull cells stay known where the tile-sparse layout stores no tile (a run
+    // of 64 solid cells): the budget looks up the wall cell of every link.
     int set_body_mask(const uint8_t* m) override {
         if (!m) {
             body = DevBuf();
             return WFS_OK;
         }
+        const size_t words = (size_t)((N + 31) / 32);
+        std::vector<uint32_t> bits(words, 0u);
+        for (long long k = 0; k < N; ++k)
+            if (m[k]) bits[(size_t)(k >> 5)] |= 1u << (k & 31);
         if (!body.p) {
-            int rc = alloc_buf(body, (size_t)Np);
+            int rc = alloc_buf(body, words * sizeof(uint32_t));
             if (rc) return rc;
         }
-        return put_bytes(body, m);
+        WFS_HIP(hipMemcpy(body.p, bits.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice));
+        return WFS_OK;
     }
 
     int forces(double* out) override {
@@ -692,10 +700,10 @@ struct Solver final : wfs_solver {
             return rc;
         if (tiled)
             hipLaunchKernelGGL((k_wall_force<Codec, true>), dim3((unsigned)blocks), dim3(256), 0, stream, fp(cur), a,
-                               cmap(), static_cast<const uint8_t*>(body.p), static_cast<double*>(part.p));
+                               cmap(), static_cast<const uint32_t*>(body.p), static_cast<double*>(part.p));
         else
             hipLaunchKernelGGL((k_wall_force<Codec, false>), dim3((unsigned)blocks), dim3(256), 0, stream, fp(cur),
-                               a, cmap(), static_cast<const uint8_t*>(body.p), static_cast<double*>(part.p));
+                               a, cmap(), static_cast<const uint32_t*>(body.p), static_cast<double*>(part.p));
         WFS_LAUNCH_CHECK();
         // fixed-order two-pass sum: kSumChunks chunk sums, then their sum
         const long long chunk = (blocks + kSumChunks - 1) / kSumChunks;

@@ -21,7 +21,8 @@ dx = 0.08 m, 8 pushers in two rows 200 m apart) is 12500 x 2502 x 50 cells in
 its box (1.56 x 10^9), of which 0.70 x 10^9 are stored and 0.67 x 10^9 fluid. ``meander_case(scale=...)``
 shrinks lengths and widths for tests.
 
-    python -m water_flow_simulation_amd.meander [--scale 1.0] [--steps 200] [--storage f16dev]
+    python -m water_flow_simulation_amd.meander [--scale 1.0] [--steps 200] [--storage f16dev] [--learned]
+        [--report-every 2000]
 """
 from __future__ import annotations
 
@@ -196,12 +197,25 @@ def build_meander(mc: MeanderConfig) -> R.Reach:
                    force_cells=(idx, fv[0], fv[1], fv[2]), layout="tiles", hull_cells=hull_mask)
 
 
-def run(scale=1.0, steps=200, warmup=20, storage="f16dev", learned=False, log=print):
+def budgets(run_, reach):
+    """FP64 budgets of the current state: mass (lattice units), discharge
+    through the second and the third-last cross-section (m^3/s) and the
+    thrust, hull and bed/bank forces (N)."""
+    L = run_.lbm
+    q = L.profile_x("ux")
+    return {"mass_fp64": L.total("rho"),
+            "discharge_in_out": [float(q[2] * reach.vel * reach.dx ** 2), float(q[-3] * reach.vel * reach.dx ** 2)],
+            "forces_N": {k: [round(float(v), 1) for v in f] for k, f in run_.forces().items()}}
+
+
+def run(scale=1.0, steps=200, warmup=20, storage="f16dev", learned=False, log=print, report_every=0):
     """Builds the case, steps it inside the run cadence and reports MLUPS over
     stored and fluid cells, device memory, and the FP64 budgets: mass,
     discharge at the inlet and outlet, thrust and the forces on the hulls and
     on bed and banks. learned: the committed learned closure instead of
-    constant-Cs Smagorinsky."""
+    constant-Cs Smagorinsky. report_every > 0: every that many steps, check
+    the state is finite and log one JSON line with the step, simulated time
+    and the budgets."""
     t0 = time.time()
     reach = meander_case(scale)
     t_build = time.time() - t0
@@ -221,21 +235,28 @@ def run(scale=1.0, steps=200, warmup=20, storage="f16dev", learned=False, log=pr
     run_.advance(warmup)
     L.timing(True)
     t2 = time.time()
-    run_.advance(steps)
+    if report_every > 0:
+        done = 0
+        while done < steps:
+            n = min(report_every, steps - done)
+            run_.advance(n)
+            done += n
+            L.check_finite()
+            if log:
+                log(json.dumps({"step": warmup + done, "t_s": round(run_.t, 3), "wall_s": round(time.time() - t2, 1),
+                                **budgets(run_, reach)}))
+    else:
+        run_.advance(steps)
     wall = time.time() - t2
     ms, n = L.timing_read()
     L.timing(False)
     L.check_finite()
-    mass = L.total("rho")
-    q = L.profile_x("ux")
-    forces = {k: [round(float(v), 1) for v in f] for k, f in run_.forces().items()}
     res = {"box": [reach.nx, reach.ny, reach.nz], "box_cells": reach.cells, "stored_cells": stored,
            "fluid_cells": fluid, "storage": storage, "steps": steps, "build_s": round(t_build, 1),
            "upload_s": round(t_up, 1), "wall_s": round(wall, 3), "kernel_ms": round(ms / max(n, 1), 4),
            "mlups_fluid": round(steps * fluid / wall / 1e6, 1), "mlups_stored": round(steps * stored / wall / 1e6, 1),
-           "device_gb": round(L.memory_mb() / 1e3, 2), "mass_fp64": mass,
-           "discharge_in_out": [float(q[2] * reach.vel * reach.dx ** 2), float(q[-3] * reach.vel * reach.dx ** 2)],
-           "Q": reach.Q, "closure": "learned" if learned else "smagorinsky", "forces_N": forces,
+           "device_gb": round(L.memory_mb() / 1e3, 2), **budgets(run_, reach), "t_s": round(run_.t, 3),
+           "dt_s": reach.dt, "Q": reach.Q, "closure": "learned" if learned else "smagorinsky",
            "thrust_design_N": float(sum(p["T"] for p in reach.props))}
     run_.close()
     return res
@@ -247,9 +268,10 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--storage", default="f16dev")
     ap.add_argument("--learned", action="store_true", help="the learned sub-grid closure")
+    ap.add_argument("--report-every", type=int, default=0, help="steps between budget reports (0: none)")
     a = ap.parse_args()
     print(json.dumps(run(a.scale, a.steps, storage=a.storage, learned=a.learned,
-                         log=lambda m: print(m, flush=True))), flush=True)
+                         log=lambda m: print(m, flush=True), report_every=a.report_every)), flush=True)
 
 
 if __name__ == "__main__":
