@@ -57,11 +57,12 @@ def solver_args(reach):
             0.5 + 3 * nu_lb, reach.cfg.cs, reach.cf)
 
 
-def chunks(lbm, nsteps, cadence=20):
+def chunks(lbm, nsteps, cadence=20, per_step_column_u=False):
     k = 0
     while k < nsteps:
         m = min(cadence, nsteps - k)
-        lbm.update_uref()
+        if not per_step_column_u:
+            lbm.update_uref()
         lbm.step(m)
         lbm.accumulate()
         k += m
@@ -73,23 +74,28 @@ def closure_weights():
     return CL.load_params()[0] if os.path.exists(CL.WEIGHTS) else CL.init_params(0)
 
 
-def time_gpu(reach, storage, steps, warmup, block_x, learned=False):
+def time_gpu(reach, storage, steps, warmup, block_x, learned=False, column_u=False):
+    """column_u: the wall model's column speed refreshed every step inside the
+    step kernel instead of once per 20-step chunk."""
     from water_flow_simulation_amd.lbm3d import LBM3D
     lbm = LBM3D(None, *solver_args(reach), storage=storage, block_x=block_x)
     lbm.set_bedwall(reach.bedwall)
+    if column_u:
+        lbm.update_uref()
+        lbm.set_column_u(True)
     if learned:
         from water_flow_simulation_amd import closure as CL
         geo, hcol = CL.geometry(reach.solid)
         uin = solver_args(reach)[4]
         lbm.set_closure("mlp", closure_weights(), geo, hcol, float(uin[uin > 0].mean()))
-    chunks(lbm, warmup)
+    chunks(lbm, warmup, per_step_column_u=column_u)
     lbm.synchronize()
     lbm.check_finite()
     lbm.reset_sums()
     lbm.timing(True)
     lbm.synchronize()
     t0 = time.perf_counter()
-    chunks(lbm, steps)
+    chunks(lbm, steps, per_step_column_u=column_u)
     lbm.synchronize()
     wall = time.perf_counter() - t0
     kernel_ms, launches = lbm.timing_read()
@@ -176,8 +182,9 @@ def main():
     ap.add_argument("--workload", default="trapezoid_tug")
     ap.add_argument("--storage", default="f32", choices=sorted(B_ALG))
     ap.add_argument("--block-x", type=int, default=0)
-    ap.add_argument("--variants", default="f32fast,f16dev,bf16dev,f32+mlp,f16dev+mlp",
-                    help="extra variants timed (comma list; storage format, '+mlp' for the learned closure)")
+    ap.add_argument("--variants", default="f32fast,f16dev,bf16dev,f32+mlp,f16dev+mlp,f16dev+colu",
+                    help="extra variants timed (comma list; storage format, '+mlp' for the learned closure, "
+                         "'+colu' for the per-step column speed)")
     ap.add_argument("--batch-cases", type=int, default=8, help="cases of the batched-sweep line (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=400)
     ap.add_argument("--meander", default="f16dev,f16dev+mlp",
@@ -222,11 +229,13 @@ def main():
     }
     variants = {}
     for name in [v for v in a.variants.split(",") if v and v != a.storage]:
-        st, learned = name.split("+")[0], name.endswith("+mlp")
+        st, opts = name.split("+")[0], name.split("+")[1:]
+        learned, colu = "mlp" in opts, "colu" in opts
         log(f"timing variant {name}")
-        rv = time_gpu(reach, st, a.steps, a.warmup, a.block_x, learned)
+        rv = time_gpu(reach, st, a.steps, a.warmup, a.block_x, learned, colu)
         kv = rv["kernel_ms"] / max(rv["launches"], 1) / 1e3
-        b_alg = B_ALG[st] + (2 if learned else 0)       # + the 2-byte geometry code of the closure
+        # + the 2-byte geometry code of the closure; + ux, uy written for the per-step column speed
+        b_alg = B_ALG[st] + (2 if learned else 0) + (8 if colu else 0)
         variants[name] = {"mlups": round(a.steps * reach.cells / rv["wall"] / 1e6, 1),
                           "kernel_ms_avg": round(kv * 1e3, 5),
                           "roofline_frac": round(b_alg * n_fluid / kv / 1e9 / PEAK_HBM_GBS, 4),

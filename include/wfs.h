@@ -130,6 +130,17 @@ int wfs_set_bedwall(wfs_solver* h, const uint8_t* bedwall);
  * column speed |(sum_z ux, sum_z uy)| / n_fluid on bed/bank cells, 0 elsewhere. */
 int wfs_update_uref(wfs_solver* h);
 
+/* Per-step column velocity for the wall model. per_step != 0: every step
+ * computes the same column speed as wfs_update_uref from the previous step's
+ * velocity inside the step kernel (each step writes its ux, uy; each bed/bank
+ * wall-model cell sums its column), so uref is never older than one step;
+ * wfs_update_uref and wfs_set_uref then have no effect on the step. Starts from
+ * the current velocity field (call after wfs_step). Needs wfs_set_bedwall and
+ * at most 255 cells per column. per_step == 0: back to the run cadence.
+ * No reference counterpart: the reference refreshes uref once per 20-step
+ * chunk (reach3d.py:427-440); with a cadence of one step the two agree. */
+int wfs_set_column_u(wfs_solver* h, int32_t per_step);
+
 /* n solver steps (lbm3d.py:390-412): fused collide-stream, inlet/outlet rows,
  * buffer swap. Asynchronous. The macroscopic fields hold the last step's
  * values for every cell when the call's work completes. */

@@ -86,6 +86,7 @@ _SIGS = {
     "wfs_stored_cells": ([_P], ctypes.c_int64),
     "wfs_set_force_cells": ([_P, ctypes.c_int64, _P, _P, _P, _P], ctypes.c_int),
     "wfs_set_body_mask": ([_P, _P], ctypes.c_int),
+    "wfs_set_column_u": ([_P, ctypes.c_int32], ctypes.c_int),
     "wfs_forces": ([_P, _P], ctypes.c_int),
     "wfs_set_block_x": ([_P, _I32], ctypes.c_int),
     "wfs_block_x": ([_P], _I32),

@@ -179,6 +179,31 @@ __global__ void k_uref(const R* __restrict__ ux, const R* __restrict__ uy, const
     }
 }
 
+// Fluid cells per column (at least 1), for the per-step column velocity.
+__global__ void k_colcount(const uint8_t* __restrict__ flag, uint8_t* __restrict__ count, CellMap m, int nz) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= m.nx) return;
+    int n = 0;
+    for (int z = 0; z < nz; ++z) {
+        const long long c = cell_of(m, x, y + (long long)m.ny * z);
+        if (c >= 0 && !(flag[c] & FLAG_SOLID)) ++n;
+    }
+    count[x + (long long)m.px * y] = uint8_t(n > 1 ? n : 1);
+}
+
+// The column velocity's first input: the current velocity of fluid cells,
+// zero elsewhere.
+template <typename R>
+__global__ void k_colu_init(const uint8_t* __restrict__ flag, const R* __restrict__ ux, const R* __restrict__ uy,
+                            R* __restrict__ cx, R* __restrict__ cy, long long N) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const bool fluid = !(flag[c] & FLAG_SOLID);
+    cx[c] = fluid ? ux[c] : R(0);
+    cy[c] = fluid ? uy[c] : R(0);
+}
+
 // Running sums for the time means (floodsim/reach3d.py:443-451).
 template <typename R>
 __global__ void k_accumulate(const R* __restrict__ rho, const R* __restrict__ ux, const R* __restrict__ uy,
@@ -375,7 +400,7 @@ __global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __r
             if (fl & FLAG_NEARWALL) {
                 const uint32_t wm = a.wmask[c];
                 R f[Q];
-                gather<Codec, kSparse>(fs, a, int(c), x, z, fl, wm, f, kSparse ? a.nbr + 27 * (c >> 6) : nullptr,
+                gather<Codec, kSparse>(fs, a, int(c), x, y, z, fl, wm, f, kSparse ? a.nbr + 27 * (c >> 6) : nullptr,
                                        int(c & 63));
                 static_for<Q>([&](auto I) {
                     constexpr int i = I;

@@ -307,6 +307,18 @@ struct StepArgs {
     const int* __restrict__ nbr;        // per tile: 27 neighbour tiles (dx+1) + 3 ((dy+1) + 3 (dz+1)), -1 none
     const int* __restrict__ tcoord;     // per tile: x-tile, y, z
     const int* __restrict__ blk2tile;   // work-group -> tile (-1: idle), rows kept on one XCD
+    const int* __restrict__ tmap;       // tile-sparse layout: x-tile + ntx * row -> stored tile or -1
+    int ntx;
+    // per-step column velocity for the wall model (off when colu_in_x is null):
+    // every step writes ux, uy of its fluid cells to colu_out_*, and a bed/bank
+    // wall-model cell takes uref from its column of colu_in_* (the previous
+    // step), as the run cadence does every 20 steps
+    const R* __restrict__ colu_in_x;
+    const R* __restrict__ colu_in_y;
+    R* __restrict__ colu_out_x;
+    R* __restrict__ colu_out_y;
+    const uint8_t* __restrict__ bedwall;   // per cell: bed or bank cell of the wall model
+    const uint8_t* __restrict__ colcount;  // per column (x + px y): fluid cells, at least 1
 };
 
 // Work-group -> (row, x-tile) map. Work-groups are dispatched round-robin over
@@ -505,9 +517,33 @@ __device__ __forceinline__ int pull_source(int c, int px, int plane, const int* 
     }
 }
 
+// Depth-averaged speed of column (x, y) from the previous step's velocity,
+// the column reduction of the run cadence (floodsim/reach3d.py:412-418) in
+// the order and arithmetic of k_uref: sums over z upward, solid cells
+// contributing zero, |(sum ux, sum uy)| / fluid cells.
+template <bool kSparse, typename R>
+__device__ __forceinline__ R column_u(const StepArgs<R>& a, int x, int y) {
+    R sx = R(0), sy = R(0);
+    for (int zz = 0; zz < a.nz; ++zz) {
+        const long long row = y + (long long)a.ny * zz;
+        long long cc;
+        if constexpr (kSparse) {
+            const int t = a.tmap[(x >> 6) + (long long)a.ntx * row];
+            if (t < 0) continue;   // tile not stored: solid
+            cc = (long long)t * 64 + (x & 63);
+        } else {
+            cc = x + (long long)a.px * row;
+        }
+        sx += at(a.colu_in_x, int(cc));
+        sy += at(a.colu_in_y, int(cc));
+    }
+    const R ncol = R(a.colcount[x + (long long)a.px * y]);
+    return sqrt(sx * sx + sy * sy) / ncol;
+}
+
 template <class Codec, bool kSparse = false>
 __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs, const StepArgs<typename Codec::R>& a,
-                                       int c, int x, int z, uint8_t fl, uint32_t wm, typename Codec::R* f,
+                                       int c, int x, int y, int z, uint8_t fl, uint32_t wm, typename Codec::R* f,
                                        const int* nb = nullptr, int lane = 0) {
     using R = typename Codec::R;
     const int px = a.px;
@@ -669,7 +705,8 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
                     }
                 });
                 if (K > R(0)) {
-                    const R ur = at(a.uref, c);
+                    const R ur = !a.colu_in_x ? at(a.uref, c)
+                                 : (at(a.bedwall, c) ? column_u<kSparse>(a, x, y) : R(0));
                     const R U = ur > R(0) ? ur : ut;
                     const R us0 = (R(-2) * D - rw * cf * U * U * area) / (R(6) * rw * K);
                     const R us = fmin(fmax(us0, -ut), ut);
@@ -728,11 +765,11 @@ __device__ __forceinline__ void finish(typename Codec::T* __restrict__ fd, const
 // next step or the caller reads them.
 template <class Codec, bool kSparse>
 __device__ __forceinline__ void cell_update(const typename Codec::T* __restrict__ fs,
-                                            const StepArgs<typename Codec::R>& a, int c, int x, int z, uint8_t fl,
-                                            typename Codec::R* f, const int* nb, int lane) {
+                                            const StepArgs<typename Codec::R>& a, int c, int x, int y, int z,
+                                            uint8_t fl, typename Codec::R* f, const int* nb, int lane) {
     using R = typename Codec::R;
     const uint32_t wm = (fl & FLAG_NEARWALL) ? at(a.wmask, c) : 0u;
-    gather<Codec, kSparse>(fs, a, c, x, z, fl, wm, f, nb, lane);
+    gather<Codec, kSparse>(fs, a, c, x, y, z, fl, wm, f, nb, lane);
 
     R r, ux, uy, uz;
     if constexpr (Codec::kExact) {
@@ -831,6 +868,7 @@ __device__ __forceinline__ void cell_update(const typename Codec::T* __restrict_
     if (a.write_all || (wm != 0u && (fl & FLAG_CF))) {
         at(a.rho, c) = r; at(a.ux, c) = ux; at(a.uy, c) = uy; at(a.uz, c) = uz;
     }
+    if (a.colu_out_x) { at(a.colu_out_x, c) = ux; at(a.colu_out_y, c) = uy; }
 }
 
 // The wavefront holds the inlet row (x = 0) or the fused outlet row; xw is its
@@ -864,13 +902,13 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
         // no cross-lane exchange: each active lane updates and stores its cell
         if (active) {
             R f[Q];
-            cell_update<Codec, kSparse>(fs, a, c, x, z, fl, f, nb, lane);
+            cell_update<Codec, kSparse>(fs, a, c, x, row - a.ny * z, z, fl, f, nb, lane);
             store_cell<Codec>(fd, a.N, a.ctr, f, c, a.pop32 != 0);
         }
         return;
     }
     R f[Q];
-    if (active) cell_update<Codec, kSparse>(fs, a, c, x, z, fl, f, nb, lane);
+    if (active) cell_update<Codec, kSparse>(fs, a, c, x, row - a.ny * z, z, fl, f, nb, lane);
     finish<Codec>(fd, a, f, c, x, row, xw, active);
 }
 
@@ -901,7 +939,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     float feat[kNF] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (active) {
         wm = (fl & FLAG_NEARWALL) ? at(a.wmask, c) : 0u;
-        gather<Codec, kSparse>(fs, a, c, x, z, fl, wm, f, nb, lane);
+        gather<Codec, kSparse>(fs, a, c, x, row - a.ny * z, z, fl, wm, f, nb, lane);
         if (forced) { fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale; }
         if (any_forced) pair_moments<R, true>(f, fx, fy, fz, m);
         else pair_moments<R, false>(f, fx, fy, fz, m);
@@ -922,6 +960,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
             at(a.rho, c) = m.r; at(a.ux, c) = m.ux; at(a.uy, c) = m.uy; at(a.uz, c) = m.uz;
         }
         if (a.write_all && a.c2out) at(a.c2out, c) = c2;
+        if (a.colu_out_x) { at(a.colu_out_x, c) = m.ux; at(a.colu_out_y, c) = m.uy; }
     }
     finish<Codec>(fd, a, f, c, x, row, xw, active);
 }

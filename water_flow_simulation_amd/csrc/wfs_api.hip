@@ -136,11 +136,13 @@ struct wfs_solver {
     virtual int check_finite() = 0;
     virtual int set_closure(int kind, const float* params, const uint16_t* geo, const uint8_t* hcol, double u0) = 0;
     virtual int set_body_mask(const uint8_t* body) = 0;
+    virtual int set_column_u(int per_step) = 0;
     virtual int forces(double* out) = 0;
     virtual int get_closure_field(float* c2) = 0;
     virtual int closure_samples(const uint16_t* geo, const uint8_t* hcol, int ncx, int ncy, int ncz, double kv,
                                 double u0, float* X, float* ABP, uint8_t* valid) = 0;
     int closure = WFS_CLOSURE_SMAGORINSKY;
+    bool colu_on = false;   // per-step column velocity for the wall model
     int bx = 128;   // step-kernel work-group width (64..256)
     // tile-sparse layout: only the 64-cell x-runs of a row that hold fluid are stored
     bool tiled = false;
@@ -173,6 +175,7 @@ struct Solver final : wfs_solver {
     DevBuf sums[6], partial, out_x, counter, stage;
     DevBuf geo, hcol, mlp_params, mlp_pack, c2buf;   // learned closure (allocated when first set)
     DevBuf body;                                      // hull cells for the force budget (when set): one bit per dense cell
+    DevBuf colu[2][2], colcount;                      // per-step column velocity: [parity][ux, uy]; fluid cells per column
     DevBuf tmap_d, tcoord_d, nbr_d, blk2tile_d, blk2tile4_d;   // tile-sparse tables (when tiled)
 
     CellMap cmap() const {
@@ -435,6 +438,25 @@ struct Solver final : wfs_solver {
         a.nbr = static_cast<const int*>(nbr_d.p);
         a.tcoord = static_cast<const int*>(tcoord_d.p);
         a.blk2tile = static_cast<const int*>(blk2tile_d.p);
+        a.tmap = static_cast<const int*>(tmap_d.p);
+        a.ntx = ntx;
+        a.bedwall = static_cast<const uint8_t*>(bedwall.p);
+        a.colcount = static_cast<const uint8_t*>(colcount.p);
+        set_colu(a);
+    }
+
+    // The column velocity this step reads (written by the previous step) and
+    // the one it writes, tied to the population parity.
+    void set_colu(StepArgs<R>& a) {
+        if (!colu_on) {
+            a.colu_in_x = a.colu_in_y = nullptr;
+            a.colu_out_x = a.colu_out_y = nullptr;
+            return;
+        }
+        a.colu_in_x = rp(colu[cur][0]);
+        a.colu_in_y = rp(colu[cur][1]);
+        a.colu_out_x = rp(colu[cur ^ 1][0]);
+        a.colu_out_y = rp(colu[cur ^ 1][1]);
     }
 
     // One step launch: the constant-Cs kernel or, with the learned closure, k_step_learned.
@@ -470,6 +492,7 @@ struct Solver final : wfs_solver {
         const dim3 grid((unsigned)groups);
         for (int k = 0; k < n; ++k) {
             a.write_all = (k == n - 1) ? 1 : 0;
+            set_colu(a);
             if (timing) {
                 if (ev_used % 2) return fail(WFS_EHIP, "step timing: unpaired event");
                 int rc = mark();
@@ -669,6 +692,37 @@ struct Solver final : wfs_solver {
         }
     }
 
+    // Per-step column velocity: off, the wall model reads uref, which the run
+    // cadence refreshes (wfs_update_uref); on, every step refreshes it from the
+    // previous step inside the step kernel. Switching on starts from the
+    // current velocity field (valid after a wfs_step call).
+    int set_column_u(int per_step) override {
+        if (!per_step) {
+            colu_on = false;
+            for (auto& p : colu)
+                for (auto& b : p) b = DevBuf();
+            colcount = DevBuf();
+            return WFS_OK;
+        }
+        if (d.nz > 255) return fail(WFS_EINVAL, "wfs_set_column_u: more than 255 cells per column");
+        int rc;
+        for (auto& p : colu)
+            for (auto& b : p)
+                if (!b.p && (rc = alloc_buf(b, (size_t)Np * sizeof(R)))) return rc;
+        if (!colcount.p && (rc = alloc_buf(colcount, (size_t)px * d.ny))) return rc;
+        hipLaunchKernelGGL(k_colcount, dim3(blocks_for(d.nx, 64), d.ny), dim3(64), 0, stream, flags(),
+                           static_cast<uint8_t*>(colcount.p), cmap(), d.nz);
+        WFS_LAUNCH_CHECK();
+        for (int k = 0; k < 2; ++k) {
+            hipLaunchKernelGGL(k_colu_init<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, flags(), rp(ux),
+                               rp(uy), rp(colu[k][0]), rp(colu[k][1]), Np);
+            WFS_LAUNCH_CHECK();
+        }
+        WFS_HIP(hipStreamSynchronize(stream));
+        colu_on = true;
+        return WFS_OK;
+    }
+
     // The mask is kept as one bit per dense cell (x + nx (y + ny z)), so that
     // hull cells stay known where the tile-sparse layout stores no tile (a run
     // of 64 solid cells): the budget looks up the wall cell of every link.
@@ -821,6 +875,7 @@ struct Batch final : BatchBase {
             if (bx0 < 0) bx0 = c->bx;
             if (c->bx != bx0) return fail(WFS_EINVAL, "wfs_batch_create: cases must share block_x");
             if (c->tiled) return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the dense layout");
+            if (c->colu_on) return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the run cadence's uref");
             if (c->closure != WFS_CLOSURE_SMAGORINSKY)
                 return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the constant-Cs closure");
             WFS_HIP(hipStreamSynchronize(c->stream));
@@ -1535,6 +1590,12 @@ int wfs_set_closure(wfs_solver* h, int32_t kind, const float* params, const uint
     if (rc) return rc;
     DeviceGuard g(h->device);
     return h->set_closure(kind, params, geo, hcol, u0);
+}
+
+int wfs_set_column_u(wfs_solver* h, int32_t per_step) {
+    if (!h) return fail(WFS_EINVAL, "wfs_set_column_u: null handle");
+    DeviceGuard g(h->device);
+    return h->set_column_u(per_step);
 }
 
 int wfs_set_body_mask(wfs_solver* h, const uint8_t* body) {

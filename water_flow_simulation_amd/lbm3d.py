@@ -274,6 +274,14 @@ class LBM3D:
         nat.check(nat.lib().wfs_timing_read(self.handle, ctypes.byref(ms), ctypes.byref(n)))
         return float(ms.value), int(n.value)
 
+    def set_column_u(self, per_step=True):
+        """per_step: the wall model's column speed is refreshed every step inside
+        the step kernel from the previous step's velocity (update_uref and the
+        uref attribute then do not reach the step); False: back to uref as the
+        run cadence sets it. Starts from the current velocity; needs
+        set_bedwall."""
+        nat.check(nat.lib().wfs_set_column_u(self.handle, int(bool(per_step))))
+
     # ------------------------------------------------------------ momentum budget
     def set_body_mask(self, body):
         """Mark the solid cells (e.g. hulls) whose force forces() reports apart."""

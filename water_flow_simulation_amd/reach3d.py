@@ -453,10 +453,17 @@ class Run3D:
 
     cadence = 20
 
-    def __init__(self, reach: Reach, be=None, *, storage="f32", block_x=0, device=0, closure=None):
+    def __init__(self, reach: Reach, be=None, *, storage="f32", block_x=0, device=0, closure=None,
+                 column_u="cadence"):
         """closure: None for the constant-Cs Smagorinsky model, or the weights
         of the learned closure (closure.py), evaluated inside the step.
-        block_x: work-group width, 0 for the default, "auto" for the tuned one."""
+        block_x: work-group width, 0 for the default, "auto" for the tuned one.
+        column_u: "cadence" refreshes the wall model's column speed once per
+        20-step chunk, as the reference does (reach3d.py:427-440); "per_step"
+        refreshes it every step inside the step kernel."""
+        if column_u not in ("cadence", "per_step"):
+            raise ValueError("column_u: 'cadence' or 'per_step'")
+        self.column_u = column_u
         self.r, self.be = reach, be
         if block_x == "auto" and reach.layout == "dense":
             from .autotune import block_x_for
@@ -477,6 +484,8 @@ class Run3D:
         if reach.bedwall is not None:
             self.lbm.set_bedwall(reach.bedwall)
             self.lbm.update_uref()
+            if column_u == "per_step":
+                self.lbm.set_column_u(True)
         self.navg = 0
         self.t = 0.0
         self.t_total = reach.cfg.run_flows * reach.cfg.length / max(reach.V, 0.05)
@@ -506,7 +515,7 @@ class Run3D:
                 s = np.float32(s * s * (3 - 2 * s))
                 L.set_force_scale(s)
                 self._ramp_done = s >= 1.0
-            if self.r.bedwall is not None:
+            if self.r.bedwall is not None and self.column_u == "cadence":
                 L.update_uref()
             L.step(m)
             k += m
