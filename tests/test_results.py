@@ -1,0 +1,114 @@
+"""Sweep outputs against the reference's own result files (fixture written
+by tests/golden/make_results_fixture.py from results/wallmodel/test1 and
+results/web3d/cache of the reference).
+
+The case builder reproduces the reference's reaches bit for bit, so every
+field of a web-viewer record that depends only on the geometry must equal
+the reference's record exactly, including which pooled blocks are marked
+solid. Fields that depend on the simulated flow are checked for their
+encoding (decode -> values within half a quantum)."""
+import csv
+import hashlib
+import io
+import json
+import os
+
+import numpy as np
+import pytest
+
+from water_flow_simulation_amd import reach3d as R
+from water_flow_simulation_amd import results as RS
+from water_flow_simulation_amd.geometry import Section
+
+FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "results_formats.json"), encoding="utf-8"))
+
+
+def _reach(with_boats):
+    sec = Section(B=50, Hb=4, z=2, Bf=300, n=0.03, nf=0.06)
+    if with_boats:   # boat test 1, case 1: three long-tails abreast
+        fleet = R.Fleet(R.boat_type("longtail"), rows=1, per_row=3, row_spacing=30.0, x_first=40.0)
+    else:
+        fleet = R.Fleet(R.boat_type("pontoon"), x_first=40.0, enabled=False)
+    return R.build_reach(sec, 3.0, 128.89374645934603, 0.00022222222222222817, fleet,
+                         R.ReachConfig(length=150.0, max_cells=2_000_000))
+
+
+class _Run:
+    """A finished run's interface (reach and time-mean fields) with a
+    synthetic flow: a sheared stream with a cross current."""
+    def __init__(self, r):
+        self.r = r
+        z, y, x = np.meshgrid(np.arange(r.nz), np.arange(r.ny), np.arange(r.nx), indexing="ij")
+        self.f = {"ux": 0.8 + 0.02 * z - 1e-4 * x, "uy": 0.05 * np.sin(0.1 * y), "uz": 0.01 * np.cos(0.05 * x),
+                  "rho": 3.0 + 1e-4 * x}
+        self.f["sp"] = np.sqrt(self.f["ux"] ** 2 + self.f["uy"] ** 2 + self.f["uz"] ** 2)
+        self.f["uxx"] = self.f["ux"] ** 2
+        for k in self.f:
+            self.f[k] = np.where(r.solid, 0.0, self.f[k])
+
+    def field(self, name, mean=False):
+        return self.f[name]
+
+
+def test_results_csv_rewrites_the_reference_file():
+    text = FIX["csv"]["text"]
+    rows = list(csv.DictReader(io.StringIO(text)))
+    assert list(rows[0].keys()) == RS.RESULT_COLUMNS
+    results = [{k: (v if k == "label" else float(v)) for k, v in row.items()} for row in rows]
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "boat_results.csv")
+        RS.write_results_csv(p, results)
+        raw = open(p, "rb").read()
+        RS.write_cases_txt(os.path.join(d, "cases.txt"), results)
+        cases = open(os.path.join(d, "cases.txt"), encoding="utf-8").read().splitlines()
+    assert raw.startswith(b"\xef\xbb\xbf")
+    # the csv module ends rows with CR LF, as the reference's writer does; the
+    # copy in the reference's repository has LF line ends
+    assert raw.decode("utf-8-sig").replace("\r\n", "\n") == text.replace("\r\n", "\n")
+    assert cases == [f"case{i}: {c['label']}" for i, c in enumerate(results, 1)]
+
+
+@pytest.mark.parametrize("key,with_boats", [("base", False), ("t1c1", True)])
+def test_web_record_geometry_matches_reference(key, with_boats):
+    ref = FIX["records"][key]
+    r = _reach(with_boats)
+    run = _Run(r)
+    res = R.profiles(run)
+    c = {"n": 3, "T_kN": 19.7, "flag": True, "bad": float("nan")} if with_boats else None
+    rec = json.loads(json.dumps(RS.export_case(run, "case", c, res)))   # as stored
+    for k in ("nx", "ny", "nz", "dxs", "dz", "y0", "L", "W", "h", "V", "Q", "S", "zb", "hulls", "props", "jet"):
+        assert rec[k] == ref[k], k
+    assert sorted(rec.keys()) == ref["keys"]
+    vel, fluid = RS.decode_velocity(rec)
+    solid = ~fluid
+    assert int(solid.sum()) == ref["solid_cells"]
+    assert hashlib.sha256(np.packbits(solid.ravel()).tobytes()).hexdigest() == ref["solid_sha256"]
+    assert len(rec["u"]) == ref["u_b64_len"] and len(rec["prof"]["x"]) == ref["prof_len"]
+    assert RS.decode_tau(rec).size == ref["tau_len"]
+    # encodings: the int8 field is the block mean within half a quantum, the shear within 0.005 N/m^2
+    u = np.stack([run.field(k, True) for k in ("ux", "uy", "uz")], -1)
+    mean_u, fl = RS.pool(u, (~r.solid).astype(np.float32))
+    assert np.array_equal(fl, fluid)
+    # half a quantum, plus the float32 rounding of the block mean the record quantizes
+    f32 = 4 * np.finfo(np.float32).eps * np.abs(mean_u).max()
+    assert np.abs(vel[fluid] - mean_u[fluid]).max() <= 0.5 * rec["scale"] + f32
+    t = res["tau_bed"][:2 * rec["ny"], :2 * rec["nx"]].reshape(rec["ny"], 2, rec["nx"], 2).max((1, 3))
+    assert np.abs(RS.decode_tau(rec) - t).max() <= 0.005 + 1e-9
+    if with_boats:
+        assert rec["stats"] == {"n": 3.0, "T_kN": 19.7, "bad": None}
+
+
+def test_base_profiles_schema_matches_reference(tmp_path):
+    r = _reach(False)
+    res = R.profiles(_Run(r))
+    p = str(tmp_path / "base_profiles.npz")
+    RS.save_base_profiles(p, res)
+    back = RS.load_base_profiles(p)
+    assert {k: [list(v.shape), str(v.dtype)] for k, v in back.items()} == FIX["base_profiles"]
+
+
+def test_case_tag_matches_reference_name():
+    sec = Section(B=50, Hb=4, z=2, Bf=300, n=0.03)
+    assert RS.case_tag(3, sec, 3.0, 2_000_000, 150.0) == FIX["tag"]
+    assert os.path.basename(RS.cache_path("cache", "t1c1", FIX["tag"])) == f"t1c1_{FIX['tag']}.json"
