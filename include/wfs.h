@@ -141,6 +141,21 @@ int wfs_update_uref(wfs_solver* h);
  * chunk (reach3d.py:427-440); with a cadence of one step the two agree. */
 int wfs_set_column_u(wfs_solver* h, int32_t per_step);
 
+/* Particle tracer (Run3D.init_particles / move_particles / particles_xyz,
+ * reach3d.py:500-541). wfs_particles_set uploads n positions (f64, x y z per
+ * particle, metres from the domain corner) and the rows y + ny*z of the inlet
+ * cells particles restart in (the fluid cells of the plane x = 1). Each
+ * wfs_particles_move advances every particle by float32(float32(u * vel) * dt)
+ * with u the lattice velocity of the cell holding it (vel: m/s per lattice
+ * unit, dt in seconds, dx the cell size in metres) and restarts the particles
+ * that left the box or entered a solid cell at a random inlet cell (a hash
+ * of seed, particle and move count); *respawned gets their number. Positions
+ * stay on the device until wfs_particles_get. */
+int wfs_particles_set(wfs_solver* h, int64_t n, const double* xyz, int64_t ninlet, const int32_t* inlet_rows,
+                      uint64_t seed);
+int wfs_particles_move(wfs_solver* h, double dt, double vel, double dx, int64_t* respawned);
+int wfs_particles_get(wfs_solver* h, double* xyz);
+
 /* n solver steps (lbm3d.py:390-412): fused collide-stream, inlet/outlet rows,
  * buffer swap. Asynchronous. The macroscopic fields hold the last step's
  * values for every cell when the call's work completes. */

@@ -1,0 +1,97 @@
+"""The device particle tracer against the tracer oracle
+(oracle/tracer_oracle.py, the reference's Run3D.move_particles restated):
+same positions bit for bit for particles that stay in the water, the same
+set of particles restarting, restarts inside inlet cells, repeatable draws,
+and the tile-sparse layout equal to the dense one."""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from oracle import tracer_oracle as T
+from tests.helpers import channel128
+from water_flow_simulation_amd import reach3d as R
+from water_flow_simulation_amd.meander import MeanderConfig, build_meander
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(run):
+    r = run.r
+    _, ux, uy, uz = run.lbm.macro()
+    return [np.asarray(a, np.float32).reshape(r.nz, r.ny, r.nx) for a in (ux, uy, uz)]
+
+
+def _check_restarts(r, Q, bad):
+    """Restarted particles: x in [0, 2 dx), (y, z) inside a fluid cell of the plane x = 1."""
+    q = Q[bad]
+    assert (q[:, 0] >= 0).all() and (q[:, 0] < 2 * r.dx).all()
+    iy, iz = (q[:, 1] / r.dx).astype(int), (q[:, 2] / r.dx).astype(int)
+    assert not r.solid[iz, iy, 1].any()
+
+
+def test_tracer_matches_oracle_in_a_channel_flow():
+    print("case: 128x32x16 channel, 100 steps, 4000 particles, 6 moves vs oracle", flush=True)
+    r = channel128()
+    run = R.Run3D(r)
+    run.advance(100)
+    run.init_particles(4000, seed=7)
+    P = run.P
+    assert P.shape == (4000, 3) and not r.solid[(P[:, 2] / r.dx).astype(int), (P[:, 1] / r.dx).astype(int),
+                                                (P[:, 0] / r.dx).astype(int)].any()
+    dt = 8 * r.dt
+    total = 0
+    for k in range(6):
+        f = _fields(run)
+        Q_orc, bad = T.move(P, *f, r.solid, r.dx, r.vel, dt)
+        n = run.move_particles(dt)
+        Q = run.P
+        assert n == int(bad.sum())
+        assert np.array_equal(Q[~bad], Q_orc[~bad])
+        _check_restarts(r, Q, bad)
+        total += n
+        P = Q
+        run.advance(20)
+    assert total > 0   # the inlet-to-outlet flow carries particles out in 6 moves
+    run.close()
+
+
+def test_uniform_field_and_forced_restarts():
+    print("case: 128x32x16 channel, uniform field, closed-form step and restarts", flush=True)
+    r = channel128()
+    run = R.Run3D(r)
+    N = r.cells
+    u0 = np.float32(0.02)
+    run.lbm.set_macro(np.ones(N, np.float32), np.full(N, u0), np.zeros(N, np.float32), np.zeros(N, np.float32))
+    run.init_particles(3000, seed=1)
+    P = run.P
+    dt = 30 * r.dt
+    step = float(np.float32(np.float32(u0 * np.float32(r.vel)) * np.float32(dt)))
+    n = run.move_particles(dt)
+    Q = run.P
+    moved = np.isclose(Q[:, 1], P[:, 1]) & np.isclose(Q[:, 2], P[:, 2])
+    stay = (P[:, 0] + step) / r.dx < r.nx - 1
+    stay &= ~r.solid[(P[:, 2] / r.dx).astype(int), (P[:, 1] / r.dx).astype(int),
+                     np.minimum(((P[:, 0] + step) / r.dx).astype(int), r.nx - 1)]
+    assert np.array_equal(Q[stay, 0], P[stay, 0] + step) and moved[stay].all()
+    assert n == int((~stay).sum()) > 0
+    _check_restarts(r, Q, ~stay)
+    run.close()
+
+
+def test_restart_draws_repeat_with_the_seed_and_tiles_match_dense():
+    print("case: small meander, 2000 particles, dense vs tiles and two seeds", flush=True)
+    mc = MeanderConfig(length=30.0, amplitude=2.0, wavelength=30.0, bed=6.0, bank=2.0, depth=1.6,
+                       rows=(10.0,), per_row=1)
+    r = build_meander(mc)
+    out = {}
+    for lay, seed in (("dense", 3), ("tiles", 3), ("dense", 4)):
+        run = R.Run3D(replace(r, layout=lay))
+        run.advance(60)
+        run.init_particles(2000, seed=seed)
+        for _ in range(5):
+            run.move_particles(40 * r.dt)
+        out[(lay, seed)] = run.P
+        run.close()
+    assert np.array_equal(out[("dense", 3)], out[("tiles", 3)])
+    assert not np.array_equal(out[("dense", 3)], out[("dense", 4)])

@@ -112,3 +112,36 @@ def test_case_tag_matches_reference_name():
     sec = Section(B=50, Hb=4, z=2, Bf=300, n=0.03)
     assert RS.case_tag(3, sec, 3.0, 2_000_000, 150.0) == FIX["tag"]
     assert os.path.basename(RS.cache_path("cache", "t1c1", FIX["tag"])) == f"t1c1_{FIX['tag']}.json"
+
+
+def test_vtk_layout(tmp_path):
+    """Legacy VTK structured points: header lines, then velocity vectors, speed
+    and the solid mask as big-endian float32 with x fastest."""
+    r = _reach(True)
+    run = _Run(r)
+    p = str(tmp_path / "case.vtk")
+    RS.write_vtk(p, run, mean=True)
+    raw = open(p, "rb").read()
+    head = (f"# vtk DataFile Version 3.0\nFlood Sim 3D\nBINARY\nDATASET STRUCTURED_POINTS\n"
+            f"DIMENSIONS 600 250 12\nORIGIN {0.5 * r.dx} {r.y[0]} {0.5 * r.dx}\n"
+            f"SPACING 0.25 0.25 0.25\nPOINT_DATA {600 * 250 * 12}\nVECTORS velocity float\n").encode()
+    assert raw.startswith(head)
+    n = r.cells
+    o = len(head)
+    vec = np.frombuffer(raw[o:o + 12 * n], ">f4").reshape(r.nz, r.ny, r.nx, 3)
+    o += 12 * n
+    t1 = b"\nSCALARS speed float 1\nLOOKUP_TABLE default\n"
+    assert raw[o:o + len(t1)] == t1
+    o += len(t1)
+    sp = np.frombuffer(raw[o:o + 4 * n], ">f4").reshape(r.nz, r.ny, r.nx)
+    o += 4 * n
+    t2 = b"\nSCALARS solid float 1\nLOOKUP_TABLE default\n"
+    assert raw[o:o + len(t2)] == t2
+    o += len(t2)
+    sol = np.frombuffer(raw[o:o + 4 * n], ">f4").reshape(r.nz, r.ny, r.nx)
+    assert o + 4 * n == len(raw)
+    assert np.array_equal(vec[..., 0], run.field("ux", True).astype(np.float32))
+    assert np.array_equal(vec[..., 2], run.field("uz", True).astype(np.float32))
+    assert np.allclose(sp, np.sqrt((vec.astype(np.float64) ** 2).sum(-1)), rtol=1e-6)
+    assert np.array_equal(sol.astype(bool), r.solid)
+

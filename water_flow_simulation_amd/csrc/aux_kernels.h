@@ -434,6 +434,78 @@ __global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __r
     if (threadIdx.x < 9) part[(long long)blockIdx.x * 9 + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// ---------------------------------------------------------------- particle tracer
+// Forward Euler on the cell velocity, as the reference's tracer
+// (floodsim/reach3d.py:500-536): P (metres from the domain corner, x y z per
+// particle, FP64) moves by float32(float32(u * vel) * dt), u the velocity of
+// the cell holding P (indices truncated, clamped to the box; 0 where no tile
+// is stored), a non-finite u * vel counting as 0. A particle that ends
+// outside [0, nx-1) x [0, ny) x [0, nz) or in a solid cell restarts at a
+// random inlet cell (rows of x = 1 in `inlet`): x in [0, 2 dx), y and z
+// uniform within the cell. The random numbers come from a counter-based hash
+// of (seed, particle, step), so a run repeats exactly.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double unit_of(uint64_t h) { return double(h >> 11) * 0x1.0p-53; }
+__device__ __forceinline__ float finite_or_zero(float v) { return isfinite(v) ? v : 0.f; }
+
+template <typename R>
+__global__ void k_particles(double* __restrict__ P, long long n, const R* __restrict__ ux, const R* __restrict__ uy,
+                            const R* __restrict__ uz, const uint8_t* __restrict__ flag, CellMap m, int nz, double dx,
+                            float vel, float dt, const int* __restrict__ inlet, long long ninlet, uint64_t seed,
+                            long long step, unsigned long long* __restrict__ respawned) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double p[3] = {P[3 * k], P[3 * k + 1], P[3 * k + 2]};
+    const int dims[3] = {m.nx, m.ny, nz};
+    int ic[3];
+    for (int a = 0; a < 3; ++a) {
+        const double q = p[a] / dx;
+        const double t = isfinite(q) ? fmin(fmax(trunc(q), 0.0), double(dims[a] - 1)) : 0.0;
+        ic[a] = int(t);
+    }
+    const long long c = cell_of(m, ic[0], ic[1] + (long long)m.ny * ic[2]);
+    float u[3] = {0.f, 0.f, 0.f};
+    if (c >= 0) {
+        u[0] = float(ux[c]);
+        u[1] = float(uy[c]);
+        u[2] = float(uz[c]);
+    }
+    for (int a = 0; a < 3; ++a) p[a] += double(finite_or_zero(u[a] * vel) * dt);
+    // leaving the box or entering a solid cell: restart at the inlet
+    bool bad = false;
+    long long jc[3];
+    for (int a = 0; a < 3; ++a) {
+        if (!isfinite(p[a])) { bad = true; break; }
+        const double q = trunc(p[a] / dx);
+        if (q < 0.0 || q >= double(a == 0 ? dims[a] - 1 : dims[a])) { bad = true; break; }
+        jc[a] = (long long)q;
+    }
+    if (!bad) {
+        const long long cc = cell_of(m, int(jc[0]), jc[1] + (long long)m.ny * jc[2]);
+        bad = cc < 0 || (flag[cc] & FLAG_SOLID);
+    }
+    if (bad && ninlet > 0) {
+        uint64_t h = mix64(seed ^ mix64(uint64_t(k) * 0x100000001b3ull + uint64_t(step)));
+        const long long row = inlet[h % uint64_t(ninlet)];
+        const long long y = row % m.ny, z = row / m.ny;
+        h = mix64(h);
+        p[0] = unit_of(h) * 2.0 * dx;
+        h = mix64(h);
+        p[1] = (double(y) + unit_of(h)) * dx;
+        h = mix64(h);
+        p[2] = (double(z) + unit_of(h)) * dx;
+        atomicAdd(respawned, 1ull);
+    }
+    P[3 * k] = p[0];
+    P[3 * k + 1] = p[1];
+    P[3 * k + 2] = p[2];
+}
+
 // Fixed-order FP64 sum of per-block partials: out[g * NV + k] = sum of
 // part[b * NV + k] over the b of chunk g, [g chunk, (g + 1) chunk). Thread t
 // takes b = g chunk + t, t + 256, ...; the 256 thread sums meet in a fixed LDS

@@ -137,6 +137,10 @@ struct wfs_solver {
     virtual int set_closure(int kind, const float* params, const uint16_t* geo, const uint8_t* hcol, double u0) = 0;
     virtual int set_body_mask(const uint8_t* body) = 0;
     virtual int set_column_u(int per_step) = 0;
+    virtual int particles_set(long long n, const double* xyz, long long ninlet, const int32_t* inlet,
+                              uint64_t seed) = 0;
+    virtual int particles_move(double dt, double vel, double dx, long long* respawned) = 0;
+    virtual int particles_get(double* xyz) = 0;
     virtual int forces(double* out) = 0;
     virtual int get_closure_field(float* c2) = 0;
     virtual int closure_samples(const uint16_t* geo, const uint8_t* hcol, int ncx, int ncy, int ncz, double kv,
@@ -176,6 +180,9 @@ struct Solver final : wfs_solver {
     DevBuf geo, hcol, mlp_params, mlp_pack, c2buf;   // learned closure (allocated when first set)
     DevBuf body;                                      // hull cells for the force budget (when set): one bit per dense cell
     DevBuf colu[2][2], colcount;                      // per-step column velocity: [parity][ux, uy]; fluid cells per column
+    DevBuf part_xyz, part_inlet, part_count;          // particle tracer: positions, inlet rows, respawn counter
+    long long part_n = 0, part_ninlet = 0, part_step = 0;
+    uint64_t part_seed = 0;
     DevBuf tmap_d, tcoord_d, nbr_d, blk2tile_d, blk2tile4_d;   // tile-sparse tables (when tiled)
 
     CellMap cmap() const {
@@ -720,6 +727,56 @@ struct Solver final : wfs_solver {
         }
         WFS_HIP(hipStreamSynchronize(stream));
         colu_on = true;
+        return WFS_OK;
+    }
+
+    // Particle tracer: positions (metres from the domain corner, x y z per
+    // particle) live on the device; each move reads the velocity field in
+    // place, so only positions cross to the host.
+    int particles_set(long long n, const double* xyz, long long ninlet, const int32_t* inlet, uint64_t seed) override {
+        if (n < 0 || ninlet < 0) return fail(WFS_EINVAL, "wfs_particles_set: negative count");
+        part_xyz = DevBuf();
+        part_inlet = DevBuf();
+        part_n = 0;
+        if (n == 0) return WFS_OK;
+        int rc;
+        if ((rc = alloc_buf(part_xyz, (size_t)n * 3 * sizeof(double))) ||
+            (ninlet > 0 && (rc = alloc_buf(part_inlet, (size_t)ninlet * sizeof(int32_t)))) ||
+            (!part_count.p && (rc = alloc_buf(part_count, sizeof(unsigned long long)))))
+            return rc;
+        for (long long k = 0; k < ninlet; ++k)
+            if (inlet[k] < 0 || inlet[k] >= rows()) return fail(WFS_EINVAL, "wfs_particles_set: inlet row out of range");
+        WFS_HIP(hipMemcpy(part_xyz.p, xyz, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice));
+        if (ninlet > 0)
+            WFS_HIP(hipMemcpy(part_inlet.p, inlet, (size_t)ninlet * sizeof(int32_t), hipMemcpyHostToDevice));
+        part_n = n;
+        part_ninlet = ninlet;
+        part_seed = seed;
+        part_step = 0;
+        return WFS_OK;
+    }
+
+    int particles_move(double dt, double vel, double dx, long long* respawned) override {
+        if (part_n == 0) return fail(WFS_EINVAL, "wfs_particles_move: no particles (wfs_particles_set)");
+        if (!(dx > 0)) return fail(WFS_EINVAL, "wfs_particles_move: dx must be positive");
+        WFS_HIP(hipMemsetAsync(part_count.p, 0, sizeof(unsigned long long), stream));
+        hipLaunchKernelGGL(k_particles<R>, dim3(blocks_for(part_n, 256)), dim3(256), 0, stream,
+                           static_cast<double*>(part_xyz.p), part_n, rp(ux), rp(uy), rp(uz), flags(), cmap(), d.nz,
+                           dx, (float)vel, (float)dt, static_cast<const int*>(part_inlet.p), part_ninlet, part_seed,
+                           part_step, static_cast<unsigned long long*>(part_count.p));
+        WFS_LAUNCH_CHECK();
+        ++part_step;
+        unsigned long long cnt = 0;
+        WFS_HIP(hipMemcpyAsync(&cnt, part_count.p, sizeof(cnt), hipMemcpyDeviceToHost, stream));
+        WFS_HIP(hipStreamSynchronize(stream));
+        if (respawned) *respawned = (long long)cnt;
+        return WFS_OK;
+    }
+
+    int particles_get(double* xyz) override {
+        if (part_n == 0) return WFS_OK;
+        WFS_HIP(hipMemcpyAsync(xyz, part_xyz.p, (size_t)part_n * 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+        WFS_HIP(hipStreamSynchronize(stream));
         return WFS_OK;
     }
 
@@ -1590,6 +1647,28 @@ int wfs_set_closure(wfs_solver* h, int32_t kind, const float* params, const uint
     if (rc) return rc;
     DeviceGuard g(h->device);
     return h->set_closure(kind, params, geo, hcol, u0);
+}
+
+int wfs_particles_set(wfs_solver* h, int64_t n, const double* xyz, int64_t ninlet, const int32_t* inlet_rows,
+                      uint64_t seed) {
+    if (!h || (n > 0 && !xyz) || (ninlet > 0 && !inlet_rows)) return fail(WFS_EINVAL, "wfs_particles_set: null argument");
+    DeviceGuard g(h->device);
+    return h->particles_set(n, xyz, ninlet, inlet_rows, seed);
+}
+
+int wfs_particles_move(wfs_solver* h, double dt, double vel, double dx, int64_t* respawned) {
+    if (!h) return fail(WFS_EINVAL, "wfs_particles_move: null handle");
+    DeviceGuard g(h->device);
+    long long r = 0;
+    const int rc = h->particles_move(dt, vel, dx, &r);
+    if (respawned) *respawned = r;
+    return rc;
+}
+
+int wfs_particles_get(wfs_solver* h, double* xyz) {
+    if (!h || !xyz) return fail(WFS_EINVAL, "wfs_particles_get: null argument");
+    DeviceGuard g(h->device);
+    return h->particles_get(xyz);
 }
 
 int wfs_set_column_u(wfs_solver* h, int32_t per_step) {

@@ -282,6 +282,28 @@ class LBM3D:
         set_bedwall."""
         nat.check(nat.lib().wfs_set_column_u(self.handle, int(bool(per_step))))
 
+    # ------------------------------------------------------------ particle tracer
+    def particles_set(self, xyz, inlet_rows, seed=0):
+        """Upload particle positions (n x 3, metres from the domain corner) and
+        the rows (y + ny z) of the inlet cells they restart in."""
+        p = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+        rows = np.ascontiguousarray(np.asarray(inlet_rows, dtype=np.int32).ravel())
+        self._np = len(p)
+        nat.check(nat.lib().wfs_particles_set(self.handle, len(p), nat.ptr(p), rows.size, nat.ptr(rows),
+                                             int(seed) & 0xFFFFFFFFFFFFFFFF))
+
+    def particles_move(self, dt, vel, dx):
+        """One tracer step on the device; returns how many particles restarted."""
+        n = np.zeros(1, np.int64)
+        nat.check(nat.lib().wfs_particles_move(self.handle, float(dt), float(vel), float(dx), nat.ptr(n)))
+        return int(n[0])
+
+    def particles_get(self):
+        out = np.empty((getattr(self, "_np", 0), 3))
+        if out.size:
+            nat.check(nat.lib().wfs_particles_get(self.handle, nat.ptr(out)))
+        return out
+
     # ------------------------------------------------------------ momentum budget
     def set_body_mask(self, body):
         """Mark the solid cells (e.g. hulls) whose force forces() reports apart."""

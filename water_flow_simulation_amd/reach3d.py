@@ -572,6 +572,37 @@ class Run3D:
         a = self.field(name, mean)
         return a[idx] if axis == 0 else (a[:, idx, :] if axis == 1 else a[:, :, idx])
 
+    # ---- tracer particles (floodsim/reach3d.py:500-541), advected on the GPU
+    def init_particles(self, n, seed=0):
+        """n particles in random fluid cells (the reference's seeded draw, so the
+        same start positions); ``seed`` feeds the device's restart draws."""
+        r = self.r
+        rng = np.random.default_rng(0)
+        fluid = np.argwhere(~r.solid)
+        pick = fluid[rng.integers(0, len(fluid), n)]
+        P = (pick[:, ::-1] + rng.random((n, 3))) * r.dx   # (x, y index, z) in metres from the domain corner
+        inlet = np.argwhere(~r.solid[:, :, 1])             # (z, y) of the cells particles restart in
+        self.lbm.particles_set(P, inlet[:, 1] + r.ny * inlet[:, 0], seed)
+        self.particles = True
+
+    def move_particles(self, dt_phys):
+        """One tracer step of dt_phys seconds through the current velocity field;
+        returns the number of particles restarted at the inlet."""
+        if not getattr(self, "particles", False):
+            return 0
+        return self.lbm.particles_move(dt_phys, self.r.vel, self.r.dx)
+
+    @property
+    def P(self):
+        """Particle positions (n x 3, metres from the domain corner)."""
+        return self.lbm.particles_get()
+
+    def particles_xyz(self):
+        """x from the inlet, y from the channel centre line, z from the bottom."""
+        r = self.r
+        P = self.P
+        return P[:, 0], P[:, 1] + r.y[0] - 0.5 * r.dx, P[:, 2]
+
     def hull_mask(self):
         """Solid cells that are not part of the bed/bank cross-section (the hulls)."""
         if self.r.hull_cells is not None:

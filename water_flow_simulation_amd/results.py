@@ -13,6 +13,9 @@
   (floodsim/web3d.py:186-196).
 - ``save_base_profiles``: the no-boat profiles, numpy .npz with the
   reference's dtypes (floodsim/web3d.py:236-242).
+- ``write_vtk``: the velocity field as VTK legacy binary structured points
+  for ParaView (floodsim/gui.py:1234-1248): velocity vectors, speed and the
+  solid mask, big-endian float32, x fastest.
 
 The records read the same as the reference's, so results of this build and
 the ones in the reference's results/ folder can be set side by side.
@@ -188,4 +191,26 @@ def write_sweep(out_dir: str, results) -> list[str]:
     write_results_csv(paths[0], results)
     write_cases_txt(paths[1], results)
     return paths
+
+
+def vtk_header(r) -> str:
+    """Header of the VTK file of reach r, up to the point data."""
+    return (f"# vtk DataFile Version 3.0\nFlood Sim 3D\nBINARY\nDATASET STRUCTURED_POINTS\n"
+            f"DIMENSIONS {r.nx} {r.ny} {r.nz}\nORIGIN {0.5 * r.dx} {r.y[0]} {0.5 * r.dx}\n"
+            f"SPACING {r.dx} {r.dx} {r.dx}\nPOINT_DATA {r.cells}\n")
+
+
+def write_vtk(path: str, run, mean: bool = False) -> None:
+    """The run's velocity (instantaneous or time mean) as a VTK file."""
+    r = run.r
+    u = np.stack([run.field(k, mean) for k in ("ux", "uy", "uz")], -1)
+    speed = np.sqrt((u * u).sum(-1))
+    with open(path, "wb") as f:
+        f.write(vtk_header(r).encode())
+        f.write(b"VECTORS velocity float\n")
+        f.write(u.astype(">f4").tobytes())
+        f.write(b"\nSCALARS speed float 1\nLOOKUP_TABLE default\n")
+        f.write(speed.astype(">f4").tobytes())
+        f.write(b"\nSCALARS solid float 1\nLOOKUP_TABLE default\n")
+        f.write(r.solid.astype(">f4").tobytes())
 
