@@ -95,3 +95,18 @@ def test_restart_draws_repeat_with_the_seed_and_tiles_match_dense():
         run.close()
     assert np.array_equal(out[("dense", 3)], out[("tiles", 3)])
     assert not np.array_equal(out[("dense", 3)], out[("dense", 4)])
+
+
+def test_interactive_tick():
+    print("case: 128x32x16 channel, interactive frames with tracer particles", flush=True)
+    r = channel128()
+    run = R.Run3D(r)
+    run.init_particles(500)
+    P0 = run.P
+    st = [run.tick(budget_s=0.02) for _ in range(3)]
+    assert all(s["steps"] % 20 == 0 and s["steps"] >= 20 for s in st)
+    assert st[-1]["nstep"] == sum(s["steps"] for s in st) == run.lbm.nstep
+    assert abs(st[-1]["t"] - run.lbm.nstep * r.dt) < 1e-9 and st[-1]["mlups"] > 0
+    assert not np.array_equal(run.P, P0)
+    run.close()
+

@@ -540,6 +540,23 @@ class Run3D:
             if cancel and cancel():
                 break
 
+    def tick(self, budget_s=0.08, chunk=20):
+        """One frame of an interactive run (the reference's GUI loop,
+        floodsim/gui.py:823-842): whole chunks of ``chunk`` steps until
+        ``budget_s`` seconds of wall time are spent (at least one), tracer
+        particles moved by the simulated time covered, and the figures a
+        status line shows."""
+        t0 = time.time()
+        n = 0
+        while True:
+            self.advance(chunk)
+            n += chunk
+            if time.time() - t0 >= budget_s:
+                break
+        respawned = self.move_particles(n * self.r.dt)
+        return {"steps": n, "t": self.t, "t_total": self.t_total, "nstep": self.lbm.nstep,
+                "mlups": self.mlups(), "navg": self.navg, "respawned": respawned}
+
     def mlups(self):
         """Million lattice updates per second, all cells counted (reach3d.py:469-470)."""
         return self.lbm.nstep * self.r.cells / max(self.steps_wall, 1e-9) / 1e6
