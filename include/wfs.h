@@ -141,6 +141,20 @@ int wfs_update_uref(wfs_solver* h);
  * chunk (reach3d.py:427-440); with a cadence of one step the two agree. */
 int wfs_set_column_u(wfs_solver* h, int32_t per_step);
 
+/* In-place streaming. on != 0: the step keeps one population buffer instead
+ * of two (the second is freed) and alternates two kernels: an odd step pulls
+ * from the neighbours' exchanged slots and pushes its results to where the
+ * next step reads them; an even step reads and writes the cell's own slots.
+ * Two steps equal two steps of the two-buffer scheme bit for bit (macros,
+ * forces and populations). wfs_get_pops, wfs_set_pops, wfs_forces and
+ * switching off need an even number of in-place steps since switching on
+ * (WFS_EINVAL otherwise); in wfs_get_pops a population leaving the box
+ * through an x-end row (x = 0 or nx - 1), which no step reads, reads as its
+ * opposite. Dense layout, constant-Cs closure, nx - 1 not a multiple of 64,
+ * no batches. No reference counterpart: the reference streams between two
+ * arrays (floodsim/lbm3d.py:50-60); this halves the population memory. */
+int wfs_set_inplace(wfs_solver* h, int32_t on);
+
 /* Particle tracer (Run3D.init_particles / move_particles / particles_xyz,
  * reach3d.py:500-541). wfs_particles_set uploads n positions (f64, x y z per
  * particle, metres from the domain corner) and the rows y + ny*z of the inlet

@@ -541,11 +541,68 @@ __device__ __forceinline__ R column_u(const StepArgs<R>& a, int x, int y) {
     return sqrt(sx * sx + sy * sy) / ncol;
 }
 
+// The wall function (floodsim/lbm3d.py:67-101) on the populations f of a
+// cell whose wall links are already bounced back: the slip correction of the
+// links to the wall from the cell's previous-step velocity and the wall
+// model's reference speed.
+template <class Codec, bool kSparse>
+__device__ __forceinline__ void wall_function(const StepArgs<typename Codec::R>& a, int c, int x, int y, uint8_t fl,
+                                              uint32_t wm, typename Codec::R* f) {
+    using R = typename Codec::R;
+    if (wm != 0u && (fl & FLAG_CF)) {
+        const R cf = at(a.cfw, c);
+        int inx = 0, iny = 0, inz = 0;
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            if ((wm >> i) & 1u) { inx -= kCX[i]; iny -= kCY[i]; inz -= kCZ[i]; }
+        });
+        const R nwx = R(inx), nwy = R(iny), nwz = R(inz);
+        const R nn = sqrt(nwx * nwx + nwy * nwy + nwz * nwz);
+        const R afx = R(((wm >> 1) & 1u) + ((wm >> 2) & 1u));
+        const R afy = R(((wm >> 3) & 1u) + ((wm >> 4) & 1u));
+        const R afz = R(((wm >> 5) & 1u) + ((wm >> 6) & 1u));
+        const R area = sqrt(afx * afx + afy * afy + afz * afz);
+        if (nn > R(0) && area > R(0)) {
+            const R ex = nwx / nn, ey = nwy / nn, ez = nwz / nn;
+            const R ox = at(a.ux, c), oy = at(a.uy, c), oz = at(a.uz, c);   // previous step, own cell
+            const R un = ox * ex + oy * ey + oz * ez;
+            R tx = ox - un * ex, ty = oy - un * ey, tz = oz - un * ez;
+            const R ut = sqrt(tx * tx + ty * ty + tz * tz);
+            if (ut > R(1e-7)) {
+                tx /= ut; ty /= ut; tz /= ut;
+                const R rw = at(a.rho, c);
+                R D = R(0), K = R(0);
+                static_for<Q>([&](auto I) {
+                    constexpr int i = I;
+                    if ((wm >> i) & 1u) {
+                        const R ct = dot3<kCX[i], kCY[i], kCZ[i]>(tx, ty, tz);
+                        D += ct * f[i];
+                        K += weight<R>(i) * ct * ct;
+                    }
+                });
+                if (K > R(0)) {
+                    const R ur = !a.colu_in_x ? at(a.uref, c)
+                                 : (at(a.bedwall, c) ? column_u<kSparse>(a, x, y) : R(0));
+                    const R U = ur > R(0) ? ur : ut;
+                    const R us0 = (R(-2) * D - rw * cf * U * U * area) / (R(6) * rw * K);
+                    const R us = fmin(fmax(us0, -ut), ut);
+                    static_for<Q>([&](auto I) {
+                        constexpr int i = I;
+                        if ((wm >> i) & 1u) {
+                            constexpr R c6w = R(6) * weight<R>(i);
+                            f[i] = f[i] + c6w * rw * dot3<kCX[i], kCY[i], kCZ[i]>(tx, ty, tz) * us;
+                        }
+                    });
+                }
+            }
+        }
+    }
+}
+
 template <class Codec, bool kSparse = false>
 __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs, const StepArgs<typename Codec::R>& a,
                                        int c, int x, int y, int z, uint8_t fl, uint32_t wm, typename Codec::R* f,
                                        const int* nb = nullptr, int lane = 0) {
-    using R = typename Codec::R;
     const int px = a.px;
     const int plane = px * a.ny;
     const long long N = a.N;
@@ -672,64 +729,15 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
         }
     }
 
-    // 2. wall function
-    if (wm != 0u && (fl & FLAG_CF)) {
-        const R cf = at(a.cfw, c);
-        int inx = 0, iny = 0, inz = 0;
-        static_for<Q>([&](auto I) {
-            constexpr int i = I;
-            if ((wm >> i) & 1u) { inx -= kCX[i]; iny -= kCY[i]; inz -= kCZ[i]; }
-        });
-        const R nwx = R(inx), nwy = R(iny), nwz = R(inz);
-        const R nn = sqrt(nwx * nwx + nwy * nwy + nwz * nwz);
-        const R afx = R(((wm >> 1) & 1u) + ((wm >> 2) & 1u));
-        const R afy = R(((wm >> 3) & 1u) + ((wm >> 4) & 1u));
-        const R afz = R(((wm >> 5) & 1u) + ((wm >> 6) & 1u));
-        const R area = sqrt(afx * afx + afy * afy + afz * afz);
-        if (nn > R(0) && area > R(0)) {
-            const R ex = nwx / nn, ey = nwy / nn, ez = nwz / nn;
-            const R ox = at(a.ux, c), oy = at(a.uy, c), oz = at(a.uz, c);   // previous step, own cell
-            const R un = ox * ex + oy * ey + oz * ez;
-            R tx = ox - un * ex, ty = oy - un * ey, tz = oz - un * ez;
-            const R ut = sqrt(tx * tx + ty * ty + tz * tz);
-            if (ut > R(1e-7)) {
-                tx /= ut; ty /= ut; tz /= ut;
-                const R rw = at(a.rho, c);
-                R D = R(0), K = R(0);
-                static_for<Q>([&](auto I) {
-                    constexpr int i = I;
-                    if ((wm >> i) & 1u) {
-                        const R ct = dot3<kCX[i], kCY[i], kCZ[i]>(tx, ty, tz);
-                        D += ct * f[i];
-                        K += weight<R>(i) * ct * ct;
-                    }
-                });
-                if (K > R(0)) {
-                    const R ur = !a.colu_in_x ? at(a.uref, c)
-                                 : (at(a.bedwall, c) ? column_u<kSparse>(a, x, y) : R(0));
-                    const R U = ur > R(0) ? ur : ut;
-                    const R us0 = (R(-2) * D - rw * cf * U * U * area) / (R(6) * rw * K);
-                    const R us = fmin(fmax(us0, -ut), ut);
-                    static_for<Q>([&](auto I) {
-                        constexpr int i = I;
-                        if ((wm >> i) & 1u) {
-                            constexpr R c6w = R(6) * weight<R>(i);
-                            f[i] = f[i] + c6w * rw * dot3<kCX[i], kCY[i], kCZ[i]>(tx, ty, tz) * us;
-                        }
-                    });
-                }
-            }
-        }
-    }
+    wall_function<Codec, kSparse>(a, c, x, y, fl, wm, f);
 }
 
 // Fused inlet/outlet rows and the stores of one tile (every lane of the
 // wavefront calls this; inactive lanes only take part in the shifts).
 template <class Codec>
-__device__ __forceinline__ void finish(typename Codec::T* __restrict__ fd, const StepArgs<typename Codec::R>& a,
-                                       typename Codec::R* f, int c, int x, int row, int xw, bool active) {
+__device__ __forceinline__ void boundary_rows(const StepArgs<typename Codec::R>& a, typename Codec::R* f, int x,
+                                              int row, int xw, bool active) {
     using R = typename Codec::R;
-    const long long N = a.N;
     // Inlet and outlet rows, fused: the boundary lane takes its neighbour's
     // post-collision populations (rounded through the storage format, as a
     // separate pass would read them) by a cross-lane shift. The outlet is done
@@ -756,21 +764,24 @@ __device__ __forceinline__ void finish(typename Codec::T* __restrict__ fd, const
             if (x == a.nx - 1 && active && nb) boundary_row<R>(f, v, 1, R(0), R(1));
         }
     }
-
-    if (active) store_cell<Codec>(fd, N, a.ctr, f, c, a.pop32 != 0);
 }
 
-// The update of one active cell: pull, wall function, collision; f gets the
-// post-collision populations; the macroscopic fields are written where the
-// next step or the caller reads them.
-template <class Codec, bool kSparse>
-__device__ __forceinline__ void cell_update(const typename Codec::T* __restrict__ fs,
-                                            const StepArgs<typename Codec::R>& a, int c, int x, int y, int z,
-                                            uint8_t fl, typename Codec::R* f, const int* nb, int lane) {
-    using R = typename Codec::R;
-    const uint32_t wm = (fl & FLAG_NEARWALL) ? at(a.wmask, c) : 0u;
-    gather<Codec, kSparse>(fs, a, c, x, y, z, fl, wm, f, nb, lane);
+template <class Codec>
+__device__ __forceinline__ void finish(typename Codec::T* __restrict__ fd, const StepArgs<typename Codec::R>& a,
+                                       typename Codec::R* f, int c, int x, int row, int xw, bool active) {
+    boundary_rows<Codec>(a, f, x, row, xw, active);
+    if (active) store_cell<Codec>(fd, a.N, a.ctr, f, c, a.pop32 != 0);
+}
 
+// Collision of one active cell whose incoming populations are in f (Guo
+// half-force shift, regularized BGK with Smagorinsky relaxation time, Guo
+// source; floodsim/lbm3d.py:103-137): f becomes the post-collision
+// populations; the macroscopic fields are written where the next step or the
+// caller reads them.
+template <class Codec>
+__device__ __forceinline__ void collide_cell(const StepArgs<typename Codec::R>& a, int c, uint8_t fl, uint32_t wm,
+                                             typename Codec::R* f) {
+    using R = typename Codec::R;
     R r, ux, uy, uz;
     if constexpr (Codec::kExact) {
         // 3. moments (Guo half-force shift)
@@ -871,6 +882,18 @@ __device__ __forceinline__ void cell_update(const typename Codec::T* __restrict_
     if (a.colu_out_x) { at(a.colu_out_x, c) = ux; at(a.colu_out_y, c) = uy; }
 }
 
+// The update of one active cell: pull, wall function, collision; f gets the
+// post-collision populations; the macroscopic fields are written where the
+// next step or the caller reads them.
+template <class Codec, bool kSparse>
+__device__ __forceinline__ void cell_update(const typename Codec::T* __restrict__ fs,
+                                            const StepArgs<typename Codec::R>& a, int c, int x, int y, int z,
+                                            uint8_t fl, typename Codec::R* f, const int* nb, int lane) {
+    const uint32_t wm = (fl & FLAG_NEARWALL) ? at(a.wmask, c) : 0u;
+    gather<Codec, kSparse>(fs, a, c, x, y, z, fl, wm, f, nb, lane);
+    collide_cell<Codec>(a, c, fl, wm, f);
+}
+
 // The wavefront holds the inlet row (x = 0) or the fused outlet row; xw is its
 // first x. Those wavefronts exchange post-collision populations across lanes.
 template <typename R>
@@ -910,6 +933,153 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
     R f[Q];
     if (active) cell_update<Codec, kSparse>(fs, a, c, x, row - a.ny * z, z, fl, f, nb, lane);
     finish<Codec>(fd, a, f, c, x, row, xw, active);
+}
+
+// ---------------------------------------------------------------- in-place streaming
+// One population buffer, two alternating steps (the AA pattern). Slot k of
+// cell x is f[k * N + x]; a slot always holds a whole population p, decoded
+// and encoded with p's centre.
+//
+// State S (after an even step, and the A-B layout with each pair of slots
+// i / opp(i) exchanged): slot opp(i) of x holds the post-collision f*_i(x);
+// at the x-end rows, where a link i enters from outside the box, slot i also
+// holds f*_i(x) (the reference keeps the cell's own population there).
+//
+// Odd step, from S: every link reads where the pull step of the A-B scheme
+// reads, with the slot exchanged: a plain link i from (x - c_i, opp(i)); a
+// wall link from (x, i) (= f*_opp(i)(x), bounce-back); a link through the
+// lid from (x - (cx, cy, 0), opp(MIRZ(i))); an x-end link from (x, i).
+// After the collision f*_i(x) goes to where its reader of the next step
+// looks: (x + c_i, i) for a plain link; (x, opp(i)) when x + c_i is a wall;
+// (x + (cx, cy, 0), MIRZ(i)) when it leaves through the lid; (x, i) also at
+// an x-end row. Each of these slots is read in this step by the writing cell
+// alone, so the step needs no second buffer.
+//
+// Even step: every incoming population is in the cell's own slot i (state
+// after an odd step); the results go back to the cell's own slots, giving S.
+// Two steps equal two A-B steps, bit for bit.
+
+// Converts between the A-B layout (slot k holds f_k) and state S, both ways:
+// the slots of each opposite pair are exchanged, except that a slot whose
+// link enters from outside the box at this x-end row keeps its own value.
+// S -> A-B -> S is exact; A-B -> S drops only populations that leave the box
+// through an x-end row (no step reads them), which A-B -> S -> A-B then gives
+// as the entering population of the pair.
+template <typename T>
+__global__ void k_aa_exchange(T* __restrict__ f, long long N, int nx, int px) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const int x = int(c % px);
+    const bool xlo = x == 0, xhi = x == nx - 1;
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        if constexpr (i % 2 == 1) {
+            constexpr int o = kOPP[i];
+            static_assert(o == i + 1, "opposite directions are adjacent pairs");
+            const bool keep_i = (kCX[i] == 1 && xlo) || (kCX[i] == -1 && xhi);
+            const bool keep_o = (kCX[o] == 1 && xlo) || (kCX[o] == -1 && xhi);
+            T* pi = f + (long long)i * N + c;
+            T* po = f + (long long)o * N + c;
+            const T vi = *pi, vo = *po;
+            *pi = keep_i ? vi : vo;
+            *po = keep_o ? vo : vi;
+        }
+    });
+}
+
+// Slot k of cell c (dense layout).
+template <typename T>
+__device__ __forceinline__ T* aa_slot(T* f, long long N, int k, long long c) {
+    return f + (long long)k * N + c;
+}
+
+template <class Codec, bool kOdd>
+__device__ __forceinline__ void aa_gather(const typename Codec::T* f, const StepArgs<typename Codec::R>& a, int c,
+                                          int x, int z, uint32_t wm, typename Codec::R* out) {
+    const int px = a.px;
+    const int plane = px * a.ny;
+    const long long N = a.N;
+    const bool xlo = x == 0, xhi = x == a.nx - 1, top = z == a.nz - 1;
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+        const bool xout = (cx == 1 && xlo) || (cx == -1 && xhi);
+        const bool wall = !xout && ((wm >> i) & 1u);
+        constexpr int ki = center_class(i), ko = center_class(kOPP[i]);
+        const float e = (ki != ko && wall) ? a.ctr.c[ko] : a.ctr.c[ki];
+        long long src = c;
+        int slot = i;
+        if constexpr (kOdd) {
+            if (!xout && !wall) {
+                if (cz == -1 && top) {
+                    src = c - cx - (long long)px * cy;
+                    slot = kOPP[kMIRZ[i]];
+                } else {
+                    src = c - cx - (long long)px * cy - (long long)plane * cz;
+                    slot = kOPP[i];
+                }
+            }
+        }
+        out[i] = Codec::template ld<i>(aa_slot(f, N, slot, src), e);
+    });
+}
+
+template <class Codec, bool kOdd>
+__device__ __forceinline__ void aa_store(typename Codec::T* f, const StepArgs<typename Codec::R>& a, int c, int x,
+                                         int z, uint32_t wm, const typename Codec::R* v) {
+    const int px = a.px;
+    const int plane = px * a.ny;
+    const long long N = a.N;
+    const bool xlo = x == 0, xhi = x == a.nx - 1, top = z == a.nz - 1;
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+        const float key = store_key<Codec>(a.ctr, i);
+        if constexpr (!kOdd) {
+            Codec::template st<i>(aa_slot(f, N, kOPP[i], c), v[i], key);
+        } else {
+            if ((wm >> kOPP[i]) & 1u) {
+                Codec::template st<i>(aa_slot(f, N, kOPP[i], c), v[i], key);   // bounce-back: own slot
+            } else if (cz == 1 && top) {
+                if (x + cx >= 0 && x + cx < a.nx)                               // through the lid
+                    Codec::template st<i>(aa_slot(f, N, kMIRZ[i], c + cx + (long long)px * cy), v[i], key);
+            } else if (x + cx >= 0 && x + cx < a.nx) {
+                Codec::template st<i>(aa_slot(f, N, i, c + cx + (long long)px * cy + (long long)plane * cz), v[i],
+                                      key);
+            }
+        }
+    });
+    // x-end rows: a link entering from outside the box keeps the cell's own population
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        if constexpr (kCX[i] != 0) {
+            if ((kCX[i] == 1 && xlo) || (kCX[i] == -1 && xhi))
+                Codec::template st<i>(aa_slot(f, N, i, c), v[i], store_key<Codec>(a.ctr, i));
+        }
+    });
+}
+
+template <class Codec, bool kOdd>
+__global__ void __launch_bounds__(256) k_step_aa(typename Codec::T* __restrict__ f, StepArgs<typename Codec::R> a) {
+    using R = typename Codec::R;
+    int row, tile;
+    if (!tile_of(blockIdx.x, a.tiles, a.rows, row, tile)) return;   // whole work-group
+    const int x = tile * int(blockDim.x) + int(threadIdx.x);
+    const int xw = __builtin_amdgcn_readfirstlane(tile * int(blockDim.x) + (int(threadIdx.x) & ~63));
+    const bool inrow = x < a.nx;
+    const int z = row / a.ny;
+    const int c = x + a.px * row;
+    const uint8_t fl = inrow ? at(a.flag, c) : uint8_t(FLAG_SOLID);
+    const bool active = !(fl & FLAG_SOLID);
+    const uint32_t wm = (active && (fl & FLAG_NEARWALL)) ? at(a.wmask, c) : 0u;
+    R v[Q];
+    if (active) {
+        aa_gather<Codec, kOdd>(f, a, c, x, z, wm, v);
+        wall_function<Codec, false>(a, c, x, row - a.ny * z, fl, wm, v);
+        collide_cell<Codec>(a, c, fl, wm, v);
+    }
+    if (boundary_wave(a, xw)) boundary_rows<Codec>(a, v, x, row, xw, active);
+    if (active) aa_store<Codec, kOdd>(f, a, c, x, z, wm, v);
 }
 
 // The step with the learned closure (float arithmetic, pair-form collision).

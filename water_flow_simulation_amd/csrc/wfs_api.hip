@@ -137,6 +137,7 @@ struct wfs_solver {
     virtual int set_closure(int kind, const float* params, const uint16_t* geo, const uint8_t* hcol, double u0) = 0;
     virtual int set_body_mask(const uint8_t* body) = 0;
     virtual int set_column_u(int per_step) = 0;
+    virtual int set_inplace(int on) = 0;
     virtual int particles_set(long long n, const double* xyz, long long ninlet, const int32_t* inlet,
                               uint64_t seed) = 0;
     virtual int particles_move(double dt, double vel, double dx, long long* respawned) = 0;
@@ -147,6 +148,11 @@ struct wfs_solver {
                                 double u0, float* X, float* ABP, uint8_t* valid) = 0;
     int closure = WFS_CLOSURE_SMAGORINSKY;
     bool colu_on = false;   // per-step column velocity for the wall model
+    int colu_par = 0;       // which column-velocity buffer the next step reads
+    // in-place streaming: one population buffer, steps alternate odd / even;
+    // aa_phase 0 = state S (an even number of in-place steps taken)
+    bool inplace = false;
+    int aa_phase = 0;
     int bx = 128;   // step-kernel work-group width (64..256)
     // tile-sparse layout: only the 64-cell x-runs of a row that hold fluid are stored
     bool tiled = false;
@@ -305,9 +311,14 @@ struct Solver final : wfs_solver {
                            static_cast<uint32_t*>(wmask.p), cmap(), d.nz, Np);
         WFS_LAUNCH_CHECK();
         for (int k = 0; k < 2; ++k) {
+            if (!f[k].p) continue;   // in-place streaming keeps one buffer
             hipLaunchKernelGGL(k_init<Codec>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, fp(k), flags(),
                                rp(uin), cmap(), Np, ctr);
             WFS_LAUNCH_CHECK();
+        }
+        if (inplace) {
+            cur = f[0].p ? 0 : 1;
+            aa_exchange();
         }
         hipLaunchKernelGGL(k_fill_macro<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, rp(rho), rp(ux),
                            rp(uy), rp(uz), Np);
@@ -316,7 +327,9 @@ struct Solver final : wfs_solver {
             if ((rc = zero(*b))) return rc;
         if ((rc = reset_sums())) return rc;
         WFS_HIP(hipStreamSynchronize(stream));
-        cur = 0;
+        if (!inplace) cur = 0;
+        aa_phase = 0;
+        colu_par = 0;
         nstep = 0;
         fscale = R(1);
         have_bedwall = false;
@@ -453,17 +466,65 @@ struct Solver final : wfs_solver {
     }
 
     // The column velocity this step reads (written by the previous step) and
-    // the one it writes, tied to the population parity.
+    // the one it writes; the pair alternates every step.
     void set_colu(StepArgs<R>& a) {
         if (!colu_on) {
             a.colu_in_x = a.colu_in_y = nullptr;
             a.colu_out_x = a.colu_out_y = nullptr;
             return;
         }
-        a.colu_in_x = rp(colu[cur][0]);
-        a.colu_in_y = rp(colu[cur][1]);
-        a.colu_out_x = rp(colu[cur ^ 1][0]);
-        a.colu_out_y = rp(colu[cur ^ 1][1]);
+        a.colu_in_x = rp(colu[colu_par][0]);
+        a.colu_in_y = rp(colu[colu_par][1]);
+        a.colu_out_x = rp(colu[colu_par ^ 1][0]);
+        a.colu_out_y = rp(colu[colu_par ^ 1][1]);
+    }
+
+    // A-B layout <-> state S of the in-place buffer (the same exchange both ways).
+    int aa_exchange() {
+        hipLaunchKernelGGL(k_aa_exchange<T>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, fp(cur), Np, d.nx, px);
+        WFS_LAUNCH_CHECK();
+        return WFS_OK;
+    }
+
+    // Readouts and uploads of populations see the A-B layout; in-place
+    // streaming converts around them, which needs state S.
+    int aa_readable(const char* what) {
+        if (inplace && aa_phase != 0)
+            return fail(WFS_EINVAL, std::string(what) +
+                                        ": in-place streaming holds a readable population state only after an even "
+                                        "number of steps");
+        return WFS_OK;
+    }
+
+    // In-place streaming on: the population buffer not in use is freed and
+    // the state converted to S; off: the second buffer comes back. The
+    // constant-Cs closure on the dense layout, with the outlet row fused into
+    // the step (nx - 1 not a multiple of 64).
+    int set_inplace(int on) override {
+        if (!on == !inplace) return WFS_OK;
+        int rc;
+        if (on) {
+            if (tiled) return fail(WFS_EINVAL, "wfs_set_inplace: in-place streaming runs on the dense layout");
+            if (closure != WFS_CLOSURE_SMAGORINSKY)
+                return fail(WFS_EINVAL, "wfs_set_inplace: in-place streaming runs the constant-Cs closure");
+            if ((d.nx - 1) % 64 == 0)
+                return fail(WFS_EINVAL, "wfs_set_inplace: nx - 1 must not be a multiple of 64 (the outlet row is "
+                                        "fused into the step)");
+            WFS_HIP(hipStreamSynchronize(stream));
+            if (uploaded && (rc = aa_exchange())) return rc;
+            WFS_HIP(hipStreamSynchronize(stream));
+            bytes -= f[cur ^ 1].bytes;
+            f[cur ^ 1] = DevBuf();
+            inplace = true;
+            aa_phase = 0;
+            return WFS_OK;
+        }
+        if ((rc = aa_readable("wfs_set_inplace"))) return rc;
+        if ((rc = alloc_buf(f[cur ^ 1], (size_t)Np * Q * sizeof(T)))) return rc;
+        if (uploaded && (rc = aa_exchange())) return rc;
+        WFS_HIP(hipStreamSynchronize(stream));
+        inplace = false;
+        return WFS_OK;
     }
 
     // One step launch: the constant-Cs kernel or, with the learned closure, k_step_learned.
@@ -505,7 +566,14 @@ struct Solver final : wfs_solver {
                 int rc = mark();
                 if (rc) return rc;
             }
-            launch_step(grid, a);
+            if (inplace) {
+                if (aa_phase == 0)
+                    hipLaunchKernelGGL((k_step_aa<Codec, true>), grid, dim3(bx), 0, stream, fp(cur), a);
+                else
+                    hipLaunchKernelGGL((k_step_aa<Codec, false>), grid, dim3(bx), 0, stream, fp(cur), a);
+            } else {
+                launch_step(grid, a);
+            }
             WFS_LAUNCH_CHECK();
             if (timing) {
                 int rc = mark();
@@ -516,7 +584,11 @@ struct Solver final : wfs_solver {
                                    flags(), rp(uin), d.nx, px, a.rows, Np, R(1), 2, ctr);
                 WFS_LAUNCH_CHECK();
             }
-            cur ^= 1;
+            if (inplace)
+                aa_phase ^= 1;
+            else
+                cur ^= 1;
+            colu_par ^= 1;
         }
         nstep += n;
         return WFS_OK;
@@ -540,6 +612,19 @@ struct Solver final : wfs_solver {
     int get_uref(float* u) override { return get_cells(uref, u, false); }
 
     int get_pops(void* out, bool f64) override {
+        int rc = aa_readable("wfs_get_pops");
+        if (rc) return rc;
+        if (inplace && (rc = aa_exchange())) return rc;
+        rc = get_pops_ab(out, f64);
+        if (inplace) {
+            int rc2 = aa_exchange();
+            if (!rc) rc = rc2;
+            if (hipStreamSynchronize(stream) != hipSuccess && !rc) rc = fail(WFS_EHIP, "wfs_get_pops: synchronize");
+        }
+        return rc;
+    }
+
+    int get_pops_ab(void* out, bool f64) {
         const size_t n = (size_t)N * Q;
         DevBuf tmp;
         const size_t esz = f64 ? sizeof(double) : sizeof(float);
@@ -561,6 +646,17 @@ struct Solver final : wfs_solver {
     }
 
     int set_pops(const float* in) override {
+        int rc = aa_readable("wfs_set_pops");
+        if (rc) return rc;
+        if ((rc = set_pops_ab(in))) return rc;
+        if (inplace) {
+            if ((rc = aa_exchange())) return rc;
+            WFS_HIP(hipStreamSynchronize(stream));
+        }
+        return WFS_OK;
+    }
+
+    int set_pops_ab(const float* in) {
         const size_t n = (size_t)N * Q;
         DevBuf tmp;
         if (hipMalloc(&tmp.p, n * sizeof(float)) != hipSuccess) {
@@ -660,6 +756,7 @@ struct Solver final : wfs_solver {
             return WFS_OK;
         }
         if (kind != WFS_CLOSURE_MLP) return fail(WFS_EINVAL, "wfs_set_closure: unknown closure kind");
+        if (inplace) return fail(WFS_EINVAL, "wfs_set_closure: in-place streaming runs the constant-Cs closure");
         if constexpr (!std::is_same<R, float>::value) {
             return fail(WFS_EINVAL, "wfs_set_closure: the learned closure runs in float arithmetic (not FP64 storage)");
         } else {
@@ -801,6 +898,19 @@ struct Solver final : wfs_solver {
     }
 
     int forces(double* out) override {
+        int rc = aa_readable("wfs_forces");
+        if (rc) return rc;
+        if (inplace && (rc = aa_exchange())) return rc;
+        rc = forces_ab(out);
+        if (inplace) {
+            int rc2 = aa_exchange();
+            if (!rc) rc = rc2;
+            if (hipStreamSynchronize(stream) != hipSuccess && !rc) rc = fail(WFS_EHIP, "wfs_forces: synchronize");
+        }
+        return rc;
+    }
+
+    int forces_ab(double* out) {
         StepArgs<R> a;
         fill_args(a);
         const long long blocks = (Np + 255) / 256;
@@ -933,6 +1043,7 @@ struct Batch final : BatchBase {
             if (c->bx != bx0) return fail(WFS_EINVAL, "wfs_batch_create: cases must share block_x");
             if (c->tiled) return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the dense layout");
             if (c->colu_on) return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the run cadence's uref");
+            if (c->inplace) return fail(WFS_EINVAL, "wfs_batch_create: batched cases use two population buffers");
             if (c->closure != WFS_CLOSURE_SMAGORINSKY)
                 return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the constant-Cs closure");
             WFS_HIP(hipStreamSynchronize(c->stream));
@@ -1675,6 +1786,12 @@ int wfs_set_column_u(wfs_solver* h, int32_t per_step) {
     if (!h) return fail(WFS_EINVAL, "wfs_set_column_u: null handle");
     DeviceGuard g(h->device);
     return h->set_column_u(per_step);
+}
+
+int wfs_set_inplace(wfs_solver* h, int32_t on) {
+    if (!h) return fail(WFS_EINVAL, "wfs_set_inplace: null handle");
+    DeviceGuard g(h->device);
+    return h->set_inplace(on);
 }
 
 int wfs_set_body_mask(wfs_solver* h, const uint8_t* body) {

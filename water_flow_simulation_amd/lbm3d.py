@@ -55,6 +55,7 @@ class LBM3D:
         self.storage = storage
         self.nstep = 0
         self.closure = "smagorinsky"
+        self.inplace = False
         if layout not in ("dense", "tiles"):
             raise ValueError("LBM3D: layout must be 'dense' or 'tiles'")
         self.layout = layout
@@ -281,6 +282,15 @@ class LBM3D:
         run cadence sets it. Starts from the current velocity; needs
         set_bedwall."""
         nat.check(nat.lib().wfs_set_column_u(self.handle, int(bool(per_step))))
+
+    def set_inplace(self, on=True):
+        """on: stream in place in one population buffer (the second is freed;
+        two steps equal two two-buffer steps bit for bit); False: back to two
+        buffers. populations, set_populations, forces and switching off need
+        an even number of steps since switching on. Dense layout and the
+        constant-Cs closure only; nx - 1 must not be a multiple of 64."""
+        nat.check(nat.lib().wfs_set_inplace(self.handle, int(bool(on))))
+        self.inplace = bool(on)
 
     # ------------------------------------------------------------ particle tracer
     def particles_set(self, xyz, inlet_rows, seed=0):

@@ -454,15 +454,20 @@ class Run3D:
     cadence = 20
 
     def __init__(self, reach: Reach, be=None, *, storage="f32", block_x=0, device=0, closure=None,
-                 column_u="cadence"):
+                 column_u="cadence", streaming="ab"):
         """closure: None for the constant-Cs Smagorinsky model, or the weights
         of the learned closure (closure.py), evaluated inside the step.
         block_x: work-group width, 0 for the default, "auto" for the tuned one.
         column_u: "cadence" refreshes the wall model's column speed once per
         20-step chunk, as the reference does (reach3d.py:427-440); "per_step"
-        refreshes it every step inside the step kernel."""
+        refreshes it every step inside the step kernel.
+        streaming: "ab" streams between two population buffers, as the
+        reference does; "inplace" keeps one (half the population memory, the
+        same results bit for bit; constant-Cs closure, dense layout)."""
         if column_u not in ("cadence", "per_step"):
             raise ValueError("column_u: 'cadence' or 'per_step'")
+        if streaming not in ("ab", "inplace"):
+            raise ValueError("streaming: 'ab' or 'inplace'")
         self.column_u = column_u
         self.r, self.be = reach, be
         if block_x == "auto" and reach.layout == "dense":
@@ -477,6 +482,8 @@ class Run3D:
                          layout=reach.layout)
         if reach.force_cells is not None:
             self.lbm.set_force_cells(*reach.force_cells)
+        if streaming == "inplace":
+            self.lbm.set_inplace(True)
         if closure is not None:
             from .closure import geometry
             geo, hcol = geometry(reach.solid)
