@@ -74,11 +74,14 @@ def closure_weights():
     return CL.load_params()[0] if os.path.exists(CL.WEIGHTS) else CL.init_params(0)
 
 
-def time_gpu(reach, storage, steps, warmup, block_x, learned=False, column_u=False):
+def time_gpu(reach, storage, steps, warmup, block_x, learned=False, column_u=False, inplace=False):
     """column_u: the wall model's column speed refreshed every step inside the
-    step kernel instead of once per 20-step chunk."""
+    step kernel instead of once per 20-step chunk; inplace: one population
+    buffer streamed in place."""
     from water_flow_simulation_amd.lbm3d import LBM3D
     lbm = LBM3D(None, *solver_args(reach), storage=storage, block_x=block_x)
+    if inplace:
+        lbm.set_inplace(True)
     lbm.set_bedwall(reach.bedwall)
     if column_u:
         lbm.update_uref()
@@ -182,9 +185,10 @@ def main():
     ap.add_argument("--workload", default="trapezoid_tug")
     ap.add_argument("--storage", default="f32", choices=sorted(B_ALG))
     ap.add_argument("--block-x", type=int, default=0)
-    ap.add_argument("--variants", default="f32fast,f16dev,bf16dev,f32+mlp,f16dev+mlp,f16dev+colu",
+    ap.add_argument("--variants", default="f32fast,f16dev,bf16dev,f32+mlp,f16dev+mlp,f16dev+colu,f32+inplace,"
+                                          "f16dev+inplace",
                     help="extra variants timed (comma list; storage format, '+mlp' for the learned closure, "
-                         "'+colu' for the per-step column speed)")
+                         "'+colu' for the per-step column speed, '+inplace' for one population buffer)")
     ap.add_argument("--batch-cases", type=int, default=8, help="cases of the batched-sweep line (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=400)
     ap.add_argument("--meander", default="f16dev,f16dev+mlp",
@@ -230,9 +234,9 @@ def main():
     variants = {}
     for name in [v for v in a.variants.split(",") if v and v != a.storage]:
         st, opts = name.split("+")[0], name.split("+")[1:]
-        learned, colu = "mlp" in opts, "colu" in opts
+        learned, colu, inplace = "mlp" in opts, "colu" in opts, "inplace" in opts
         log(f"timing variant {name}")
-        rv = time_gpu(reach, st, a.steps, a.warmup, a.block_x, learned, colu)
+        rv = time_gpu(reach, st, a.steps, a.warmup, a.block_x, learned, colu, inplace)
         kv = rv["kernel_ms"] / max(rv["launches"], 1) / 1e3
         # + the 2-byte geometry code of the closure; + ux, uy written for the per-step column speed
         b_alg = B_ALG[st] + (2 if learned else 0) + (8 if colu else 0)

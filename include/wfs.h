@@ -150,8 +150,9 @@ int wfs_set_column_u(wfs_solver* h, int32_t per_step);
  * switching off need an even number of in-place steps since switching on
  * (WFS_EINVAL otherwise); in wfs_get_pops a population leaving the box
  * through an x-end row (x = 0 or nx - 1), which no step reads, reads as its
- * opposite. Dense layout, constant-Cs closure, nx - 1 not a multiple of 64,
- * no batches. No reference counterpart: the reference streams between two
+ * opposite. Called between wfs_create and wfs_upload, the second buffer is
+ * never allocated. Dense layout, constant-Cs closure, nx - 1 not a multiple
+ * of 64, no batches. No reference counterpart: the reference streams between two
  * arrays (floodsim/lbm3d.py:50-60); this halves the population memory. */
 int wfs_set_inplace(wfs_solver* h, int32_t on);
 

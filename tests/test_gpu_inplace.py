@@ -22,11 +22,9 @@ pytestmark = pytest.mark.gpu
 def _pair(reach, storage="f32"):
     a = solver_inputs(reach)
     out = []
-    for inplace in (False, True):
+    for streaming in ("ab", "inplace"):
         g = LBM3D(None, a["solid"], a["Fx"], a["Fy"], a["Fz"], a["uin"], a["tau0"], a["cs"], a["cf_wall"],
-                  storage=storage)
-        if inplace:
-            g.set_inplace(True)
+                  storage=storage, streaming=streaming)
         if reach.bedwall is not None:
             g.set_bedwall(reach.bedwall)
             g.update_uref()
@@ -60,9 +58,10 @@ def test_inplace_matches_two_buffers_on_channel(storage):
     ab, aa = _pair(reach, storage)
     fluid = ~reach.solid.ravel()
     kept = _kept(reach.solid.shape) & np.tile(fluid, 19)
-    # half the population memory
+    # half the population memory (the second buffer is never allocated)
     esz = {"f32": 4, "f32fast": 4, "f16dev": 2, "bf16dev": 2, "f64": 8}[storage]
     assert ab.memory_mb() - aa.memory_mb() > 0.99 * 19 * reach.solid.size * esz / 1e6
+    assert aa.inplace and not ab.inplace
     for n in (1, 6, 13, 20):          # odd and even running counts: 1, 7, 20, 40
         ab.step(n)
         aa.step(n)
@@ -191,6 +190,31 @@ def test_run3d_inplace_matches_two_buffers():
         assert np.array_equal(fa[k], fb[k]), k
     for r in runs:
         r.close()
+
+
+def test_inplace_beyond_32_bit_offsets():
+    """A population buffer over 4 GiB (FP64, 28.4 M cells) takes the kernels
+    with 64-bit addresses; smaller ones take 32-bit buffer offsets."""
+    nz, ny, nx = 300, 296, 320
+    print(f"case: {nx}x{ny}x{nz} open channel, f64 (buffer > 4 GiB), in-place vs two buffers, 4 steps", flush=True)
+    assert 19 * nx * ny * nz * 8 >= 2 ** 32
+    solid = np.zeros((nz, ny, nx), bool)
+    solid[0] = True
+    solid[:, :8, 100:120] = True                   # a groyne from the left bank
+    z3 = np.zeros(solid.shape, np.float32)
+    z3[150, 140:150, 60] = 1e-5                    # a point force
+    uin = np.where(solid[:, :, 0], 0.0, 0.04).astype(np.float32)
+    out = []
+    for inplace in (False, True):
+        g = LBM3D(None, solid, z3, z3, z3, uin, 0.51, storage="f64")
+        if inplace:               # switched on after the upload: the state is converted, a buffer freed
+            g.set_inplace(True)
+        g.step(4)
+        out.append(g.macro(np.float64))
+        g.close()
+    fluid = ~solid.ravel()
+    for u, v in zip(*out):
+        assert np.array_equal(u[fluid], v[fluid])
 
 
 def test_inplace_rejections():

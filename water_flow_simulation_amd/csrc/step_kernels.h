@@ -1059,7 +1059,130 @@ __device__ __forceinline__ void aa_store(typename Codec::T* f, const StepArgs<ty
     });
 }
 
+// The same two steps with 32-bit addressing (Q * N * sizeof(T) < 4 GiB): one
+// buffer resource for the whole buffer, the slot's offset uniform in a scalar
+// register and only the cell's offset per lane. Where a link has two or three
+// candidate slots, the lowest is the uniform base and a lane adds 0..3 N.
+// Cells with no wall link away from the x-ends and the lid take the plain
+// link of every direction.
 template <class Codec, bool kOdd>
+__device__ __forceinline__ void aa_gather32(const typename Codec::T* f, const StepArgs<typename Codec::R>& a, int c,
+                                            int x, int z, uint32_t wm, typename Codec::R* out) {
+    using T = typename Codec::T;
+    constexpr uint32_t ts = (uint32_t)sizeof(T);
+    const uint32_t nn = (uint32_t)a.N;
+    const auto rs = buf_rsrc(f, (uint32_t)Q * nn * ts);
+    const int px = a.px;
+    const int plane = px * a.ny;
+    const bool xlo = x == 0, xhi = x == a.nx - 1, top = z == a.nz - 1;
+    const uint32_t vc = (uint32_t)c * ts;
+    auto times = [&](int k) { return k == 0 ? 0u : (k == 1 ? nn : (k == 2 ? 2u * nn : 3u * nn)); };
+    if (!kOdd || (wm == 0u && !xlo && !xhi && !top)) {
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int ki = center_class(i), ko = center_class(kOPP[i]);
+            const float e = (ki != ko && ((wm >> i) & 1u)) ? a.ctr.c[ko] : a.ctr.c[ki];
+            if constexpr (kOdd) {
+                // slot opp(i) of the cell at x - c_i
+                const int off = kCX[i] + px * kCY[i] + plane * kCZ[i];
+                out[i] = Codec::template dec<i>(buf_load<T>(rs, vc, ((uint32_t)kOPP[i] * nn - (uint32_t)off) * ts), e);
+            } else {
+                out[i] = Codec::template dec<i>(buf_load<T>(rs, vc, (uint32_t)i * nn * ts), e);
+            }
+        });
+        return;
+    }
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+        constexpr int io = kOPP[i], mo = kOPP[kMIRZ[i]];
+        const bool xout = (cx == 1 && xlo) || (cx == -1 && xhi);
+        const bool wall = !xout && ((wm >> i) & 1u);
+        const bool own = xout || wall;
+        constexpr int ki = center_class(i), ko = center_class(io);
+        const float e = (ki != ko && wall) ? a.ctr.c[ko] : a.ctr.c[ki];
+        constexpr int lo0 = io < i ? io : i;
+        uint32_t base = (uint32_t)lo0 * nn;
+        int src = c - cx - px * cy - plane * cz;
+        uint32_t adj = own ? times(i - lo0) : times(io - lo0);
+        if constexpr (cz == -1) {
+            if (top) {   // through the lid: slot opp(MIRZ(i)) of the in-plane source
+                constexpr int lo1 = mo < lo0 ? mo : lo0;
+                static_assert(i - lo1 <= 3 && mo - lo1 <= 3, "candidates within 3");
+                base = (uint32_t)lo1 * nn;
+                src = c - cx - px * cy;
+                adj = own ? times(i - lo1) : times(mo - lo1);
+            }
+        }
+        const int cell = own ? c : src;
+        out[i] = Codec::template dec<i>(buf_load<T>(rs, ((uint32_t)cell + adj) * ts, base * ts), e);
+    });
+}
+
+template <class Codec, bool kOdd>
+__device__ __forceinline__ void aa_store32(typename Codec::T* f, const StepArgs<typename Codec::R>& a, int c, int x,
+                                           int z, uint32_t wm, const typename Codec::R* v) {
+    using T = typename Codec::T;
+    constexpr uint32_t ts = (uint32_t)sizeof(T);
+    const uint32_t nn = (uint32_t)a.N;
+    const auto rs = buf_rsrc(f, (uint32_t)Q * nn * ts);
+    const int px = a.px;
+    const int plane = px * a.ny;
+    const bool xlo = x == 0, xhi = x == a.nx - 1, top = z == a.nz - 1;
+    const uint32_t vc = (uint32_t)c * ts;
+    auto times = [&](int k) { return k == 0 ? 0u : (k == 1 ? nn : (k == 2 ? 2u * nn : 3u * nn)); };
+    if (!kOdd || (wm == 0u && !xlo && !xhi && !top)) {
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            const T w = Codec::template enc<i>(v[i], store_key<Codec>(a.ctr, i));
+            if constexpr (kOdd) {   // slot i of the cell at x + c_i
+                const int off = kCX[i] + px * kCY[i] + plane * kCZ[i];
+                buf_store<T>(rs, vc, w, ((uint32_t)i * nn + (uint32_t)off) * ts);
+            } else {
+                buf_store<T>(rs, vc, w, (uint32_t)kOPP[i] * nn * ts);
+            }
+        });
+    } else {
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+            constexpr int io = kOPP[i], im = kMIRZ[i];
+            const bool bounce = (wm >> io) & 1u;
+            const bool inx = !((cx == 1 && xhi) || (cx == -1 && xlo));
+            constexpr int lo0 = io < i ? io : i;
+            uint32_t base = (uint32_t)lo0 * nn;
+            int dst = c + cx + px * cy + plane * cz;
+            uint32_t adj = bounce ? times(io - lo0) : times(i - lo0);
+            if constexpr (cz == 1) {
+                if (top) {   // through the lid: slot MIRZ(i) of the in-plane target
+                    constexpr int lo1 = im < lo0 ? im : lo0;
+                    static_assert(i - lo1 <= 3 && io - lo1 <= 3 && im - lo1 <= 3, "candidates within 3");
+                    base = (uint32_t)lo1 * nn;
+                    dst = c + cx + px * cy;
+                    adj = bounce ? times(io - lo1) : times(im - lo1);
+                }
+            }
+            if (bounce || inx) {
+                const int cell = bounce ? c : dst;
+                buf_store<T>(rs, ((uint32_t)cell + adj) * ts,
+                             Codec::template enc<i>(v[i], store_key<Codec>(a.ctr, i)), base * ts);
+            }
+        });
+    }
+    // x-end rows: a link entering from outside the box keeps the cell's own population
+    if (xlo || xhi) {
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            if constexpr (kCX[i] != 0) {
+                if ((kCX[i] == 1 && xlo) || (kCX[i] == -1 && xhi))
+                    buf_store<T>(rs, vc, Codec::template enc<i>(v[i], store_key<Codec>(a.ctr, i)),
+                                 (uint32_t)i * nn * ts);
+            }
+        });
+    }
+}
+
+template <class Codec, bool kOdd, bool kPop32>
 __global__ void __launch_bounds__(256) k_step_aa(typename Codec::T* __restrict__ f, StepArgs<typename Codec::R> a) {
     using R = typename Codec::R;
     int row, tile;
@@ -1074,12 +1197,20 @@ __global__ void __launch_bounds__(256) k_step_aa(typename Codec::T* __restrict__
     const uint32_t wm = (active && (fl & FLAG_NEARWALL)) ? at(a.wmask, c) : 0u;
     R v[Q];
     if (active) {
-        aa_gather<Codec, kOdd>(f, a, c, x, z, wm, v);
+        if constexpr (kPop32)
+            aa_gather32<Codec, kOdd>(f, a, c, x, z, wm, v);
+        else
+            aa_gather<Codec, kOdd>(f, a, c, x, z, wm, v);
         wall_function<Codec, false>(a, c, x, row - a.ny * z, fl, wm, v);
         collide_cell<Codec>(a, c, fl, wm, v);
     }
     if (boundary_wave(a, xw)) boundary_rows<Codec>(a, v, x, row, xw, active);
-    if (active) aa_store<Codec, kOdd>(f, a, c, x, z, wm, v);
+    if (active) {
+        if constexpr (kPop32)
+            aa_store32<Codec, kOdd>(f, a, c, x, z, wm, v);
+        else
+            aa_store<Codec, kOdd>(f, a, c, x, z, wm, v);
+    }
 }
 
 // The step with the learned closure (float arithmetic, pair-form collision).

@@ -213,8 +213,9 @@ struct Solver final : wfs_solver {
     int alloc() override {
         const size_t n = (size_t)Np;
         int rc;
-        for (int k = 0; k < 2; ++k)
-            if ((rc = alloc_buf(f[k], n * Q * sizeof(T)))) return rc;
+        // the second population buffer comes with the upload, unless the
+        // solver streams in place by then
+        if ((rc = alloc_buf(f[0], n * Q * sizeof(T)))) return rc;
         if ((rc = alloc_buf(flag, n))) return rc;
         if ((rc = alloc_buf(wmask, n * sizeof(uint32_t)))) return rc;
         for (DevBuf* b : {&Fx, &Fy, &Fz, &rho, &ux, &uy, &uz, &cfw, &uref})
@@ -297,6 +298,8 @@ struct Solver final : wfs_solver {
 
     int upload(const uint8_t* solid, const float* cf, const float* uin_h) override {
         int rc;
+        for (int k = 0; k < 2; ++k)
+            if (!inplace && !f[k].p && (rc = alloc_buf(f[k], (size_t)Np * Q * sizeof(T)))) return rc;
         if (cf) {
             if ((rc = put_cells(cfw, cf))) return rc;
         } else if ((rc = zero(cfw))) {
@@ -567,10 +570,9 @@ struct Solver final : wfs_solver {
                 if (rc) return rc;
             }
             if (inplace) {
-                if (aa_phase == 0)
-                    hipLaunchKernelGGL((k_step_aa<Codec, true>), grid, dim3(bx), 0, stream, fp(cur), a);
-                else
-                    hipLaunchKernelGGL((k_step_aa<Codec, false>), grid, dim3(bx), 0, stream, fp(cur), a);
+                auto* aa = aa_phase == 0 ? (a.pop32 ? k_step_aa<Codec, true, true> : k_step_aa<Codec, true, false>)
+                                         : (a.pop32 ? k_step_aa<Codec, false, true> : k_step_aa<Codec, false, false>);
+                hipLaunchKernelGGL(aa, grid, dim3(bx), 0, stream, fp(cur), a);
             } else {
                 launch_step(grid, a);
             }

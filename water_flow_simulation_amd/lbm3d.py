@@ -38,7 +38,7 @@ class LBM3D:
     engine = "hip-lbm"
 
     def __init__(self, be, solid, Fx, Fy, Fz, uin, tau0, cs=0.17, cf_wall=None, *, storage="f32", block_x=0,
-                 device=0, layout="dense"):
+                 device=0, layout="dense", streaming="ab"):
         if be is not None and getattr(be, "name", None) == "torch":
             raise RuntimeError("LBM3D: the 3-D model runs on the HIP library, not on a torch backend")
         if storage not in nat.STORAGE:
@@ -55,6 +55,8 @@ class LBM3D:
         self.storage = storage
         self.nstep = 0
         self.closure = "smagorinsky"
+        if streaming not in ("ab", "inplace"):
+            raise ValueError("LBM3D: streaming must be 'ab' or 'inplace'")
         self.inplace = False
         if layout not in ("dense", "tiles"):
             raise ValueError("LBM3D: layout must be 'dense' or 'tiles'")
@@ -76,6 +78,8 @@ class LBM3D:
         self._uin = A(uin)
         if self._uin.size != self.ny * self.nz:
             raise ValueError("LBM3D: uin must be (nz, ny)")
+        if streaming == "inplace":    # before the upload: the second buffer is never allocated
+            self.set_inplace(True)
         nat.check(L.wfs_upload(h, nat.ptr(sol), nat.ptr(cfw), nat.ptr(self._uin)), "wfs_upload")
         if Fx is None:        # forces come later as a cell list (set_force_cells)
             self._Fx = self._Fy = self._Fz = None
@@ -288,7 +292,9 @@ class LBM3D:
         two steps equal two two-buffer steps bit for bit); False: back to two
         buffers. populations, set_populations, forces and switching off need
         an even number of steps since switching on. Dense layout and the
-        constant-Cs closure only; nx - 1 must not be a multiple of 64."""
+        constant-Cs closure only; nx - 1 must not be a multiple of 64.
+        LBM3D(streaming="inplace") switches it on before the upload, so the
+        second buffer is never allocated."""
         nat.check(nat.lib().wfs_set_inplace(self.handle, int(bool(on))))
         self.inplace = bool(on)
 

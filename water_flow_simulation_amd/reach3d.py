@@ -479,11 +479,9 @@ class Run3D:
         uin_lb = (reach.uin_phys / reach.vel).astype(np.float32)
         self.lbm = LBM3D(be, reach.solid, reach.Fx, reach.Fy, reach.Fz, uin_lb, tau0=0.5 + 3 * nu_lb,
                          cs=reach.cfg.cs, cf_wall=reach.cf, storage=storage, block_x=block_x, device=device,
-                         layout=reach.layout)
+                         layout=reach.layout, streaming=streaming)
         if reach.force_cells is not None:
             self.lbm.set_force_cells(*reach.force_cells)
-        if streaming == "inplace":
-            self.lbm.set_inplace(True)
         if closure is not None:
             from .closure import geometry
             geo, hcol = geometry(reach.solid)
