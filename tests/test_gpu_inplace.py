@@ -7,8 +7,9 @@ push to where the next step reads) and an even step (own slots only). The
 two-buffer path is pinned to the oracle bit for bit elsewhere; here the
 in-place path must give the same macros after every step count, the same
 populations and force budget after every even count, on the parity channel
-(hull, propellers, wall model), uneven builder rows, raw reference fixtures
-and in every storage format."""
+(hull, propellers, wall model), uneven builder rows, raw reference fixtures,
+the tile-sparse meander and with the learned closure, in every storage
+format."""
 import numpy as np
 import pytest
 
@@ -217,20 +218,76 @@ def test_inplace_beyond_32_bit_offsets():
         assert np.array_equal(u[fluid], v[fluid])
 
 
+def _runs(reach, **kw):
+    return [R.Run3D(reach, streaming=s, **kw) for s in ("ab", "inplace")]
+
+
+def _same_runs(runs, fluid, closure=False):
+    (a, b) = runs
+    sa, na = a.lbm.sums()
+    sb, nb = b.lbm.sums()
+    assert na == nb
+    for k in sa:
+        assert np.array_equal(sa[k][fluid], sb[k][fluid]), k
+    _same_macros(a.lbm, b.lbm, fluid)
+    if closure:
+        assert np.array_equal(a.lbm.closure_field()[fluid], b.lbm.closure_field()[fluid])
+    fa, fb = a.forces(), b.forces()
+    for k in fa:
+        assert np.array_equal(fa[k], fb[k]), k
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16dev"])
+def test_inplace_on_tiles_matches_two_buffers(storage):
+    print(f"case: meander at scale 0.05 (625x317x50 box), tiles, {storage}, in-place vs two buffers", flush=True)
+    from dataclasses import replace
+    from water_flow_simulation_amd.meander import meander_case
+    r = replace(meander_case(scale=0.05), layout="tiles")
+    runs = _runs(r, storage=storage)
+    assert runs[1].lbm.memory_mb() < runs[0].lbm.memory_mb()
+    fluid = ~r.solid.ravel()
+    for m in (40, 20):
+        for run in runs:
+            run.advance(m)
+    _same_runs(runs, fluid)
+    kept = _kept(r.solid.shape) & np.tile(fluid, 19)
+    assert np.array_equal(runs[0].lbm.populations()[kept], runs[1].lbm.populations()[kept])
+    for run in runs:                  # an odd count: macros still agree
+        run.lbm.step(7)
+    _same_macros(runs[0].lbm, runs[1].lbm, fluid)
+    for run in runs:
+        run.close()
+
+
+@pytest.mark.parametrize("layout,storage", [("dense", "f32"), ("dense", "f16dev"), ("tiles", "f16dev")])
+def test_inplace_with_learned_closure(layout, storage):
+    print(f"case: meander at scale 0.05, learned closure, {layout}, {storage}, in-place vs two buffers", flush=True)
+    from dataclasses import replace
+    from water_flow_simulation_amd import closure as CL
+    from water_flow_simulation_amd.meander import meander_case
+    r = replace(meander_case(scale=0.05), layout=layout)
+    runs = _runs(r, storage=storage, closure=CL.init_params(3))
+    fluid = ~r.solid.ravel()
+    for run in runs:
+        run.advance(40)
+    _same_runs(runs, fluid, closure=True)
+    for run in runs:
+        run.lbm.step(3)
+    _same_macros(runs[0].lbm, runs[1].lbm, fluid)
+    for run in runs:
+        run.close()
+
+
 def test_inplace_rejections():
-    print("case: in-place streaming on unsupported set-ups", flush=True)
-    reach = channel128()
-    a = solver_inputs(reach)
-    g = LBM3D(None, a["solid"], a["Fx"], a["Fy"], a["Fz"], a["uin"], a["tau0"], a["cs"], a["cf_wall"],
-              layout="tiles")
-    with pytest.raises(ValueError, match="dense layout"):
-        g.set_inplace(True)
-    g.close()
+    print("case: in-place streaming with the outlet row opening a wavefront", flush=True)
     nz, ny = 6, 8
     solid = np.zeros((nz, ny, 129), bool)
     solid[0] = True
     z3 = np.zeros(solid.shape, np.float32)
+    with pytest.raises(ValueError, match="multiple of 64"):
+        LBM3D(None, solid, z3, z3, z3, np.full((nz, ny), 0.02, np.float32), 0.51, streaming="inplace")
     g = LBM3D(None, solid, z3, z3, z3, np.full((nz, ny), 0.02, np.float32), 0.51)
     with pytest.raises(ValueError, match="multiple of 64"):
         g.set_inplace(True)
+    g.step(2)                         # the two-buffer path is unaffected
     g.close()

@@ -47,6 +47,36 @@ __device__ __forceinline__ void dense_of(const CellMap& m, long long c, int& x, 
     j = m.tcoord[3 * t + 1] + (long long)m.ny * m.tcoord[3 * t + 2];
 }
 
+// In-place streaming: converts between the A-B layout (slot k holds f_k)
+// and state S, both ways. The slots of each opposite pair are exchanged,
+// except that a slot whose link enters from outside the box at this x-end row
+// keeps its own value. S -> A-B -> S is exact. A-B -> S drops only the
+// populations that leave the box through an x-end row (no step reads them);
+// A-B -> S -> A-B then gives the entering population of the pair there.
+template <typename T>
+__global__ void k_aa_exchange(T* __restrict__ f, long long N, CellMap m) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    int x;
+    long long j;
+    dense_of(m, c, x, j);
+    const bool xlo = x == 0, xhi = x == m.nx - 1;
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        if constexpr (i % 2 == 1) {
+            constexpr int o = kOPP[i];
+            static_assert(o == i + 1, "opposite directions are adjacent pairs");
+            const bool keep_i = (kCX[i] == 1 && xlo) || (kCX[i] == -1 && xhi);
+            const bool keep_o = (kCX[o] == 1 && xlo) || (kCX[o] == -1 && xhi);
+            T* pi = f + (long long)i * N + c;
+            T* po = f + (long long)o * N + c;
+            const T vi = *pi, vo = *po;
+            *pi = keep_i ? vi : vo;
+            *po = keep_o ? vo : vi;
+        }
+    });
+}
+
 // dense -> device cells; padding cells get `fill`.
 template <typename S, typename D>
 __global__ void k_pad(const S* __restrict__ dense, D* __restrict__ dev, CellMap m, long long Ndev, D fill) {

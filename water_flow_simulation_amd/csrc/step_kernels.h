@@ -959,43 +959,18 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
 // after an odd step); the results go back to the cell's own slots, giving S.
 // Two steps equal two A-B steps, bit for bit.
 
-// Converts between the A-B layout (slot k holds f_k) and state S, both ways:
-// the slots of each opposite pair are exchanged, except that a slot whose
-// link enters from outside the box at this x-end row keeps its own value.
-// S -> A-B -> S is exact; A-B -> S drops only populations that leave the box
-// through an x-end row (no step reads them), which A-B -> S -> A-B then gives
-// as the entering population of the pair.
-template <typename T>
-__global__ void k_aa_exchange(T* __restrict__ f, long long N, int nx, int px) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
-    const int x = int(c % px);
-    const bool xlo = x == 0, xhi = x == nx - 1;
-    static_for<Q>([&](auto I) {
-        constexpr int i = I;
-        if constexpr (i % 2 == 1) {
-            constexpr int o = kOPP[i];
-            static_assert(o == i + 1, "opposite directions are adjacent pairs");
-            const bool keep_i = (kCX[i] == 1 && xlo) || (kCX[i] == -1 && xhi);
-            const bool keep_o = (kCX[o] == 1 && xlo) || (kCX[o] == -1 && xhi);
-            T* pi = f + (long long)i * N + c;
-            T* po = f + (long long)o * N + c;
-            const T vi = *pi, vo = *po;
-            *pi = keep_i ? vi : vo;
-            *po = keep_o ? vo : vi;
-        }
-    });
-}
-
-// Slot k of cell c (dense layout).
+// Slot k of cell c.
 template <typename T>
 __device__ __forceinline__ T* aa_slot(T* f, long long N, int k, long long c) {
     return f + (long long)k * N + c;
 }
 
-template <class Codec, bool kOdd>
+// Either layout, 64-bit addresses. Neighbour cells come from pull_source: the
+// cell at x - c for a pull, at x + c (the offset negated) for a push.
+template <class Codec, bool kOdd, bool kSparse>
 __device__ __forceinline__ void aa_gather(const typename Codec::T* f, const StepArgs<typename Codec::R>& a, int c,
-                                          int x, int z, uint32_t wm, typename Codec::R* out) {
+                                          int x, int z, uint32_t wm, typename Codec::R* out, const int* nb,
+                                          int lane) {
     const int px = a.px;
     const int plane = px * a.ny;
     const long long N = a.N;
@@ -1012,10 +987,10 @@ __device__ __forceinline__ void aa_gather(const typename Codec::T* f, const Step
         if constexpr (kOdd) {
             if (!xout && !wall) {
                 if (cz == -1 && top) {
-                    src = c - cx - (long long)px * cy;
+                    src = pull_source<kSparse, cx, cy, 0>(c, px, plane, nb, lane);
                     slot = kOPP[kMIRZ[i]];
                 } else {
-                    src = c - cx - (long long)px * cy - (long long)plane * cz;
+                    src = pull_source<kSparse, cx, cy, cz>(c, px, plane, nb, lane);
                     slot = kOPP[i];
                 }
             }
@@ -1024,9 +999,9 @@ __device__ __forceinline__ void aa_gather(const typename Codec::T* f, const Step
     });
 }
 
-template <class Codec, bool kOdd>
+template <class Codec, bool kOdd, bool kSparse>
 __device__ __forceinline__ void aa_store(typename Codec::T* f, const StepArgs<typename Codec::R>& a, int c, int x,
-                                         int z, uint32_t wm, const typename Codec::R* v) {
+                                         int z, uint32_t wm, const typename Codec::R* v, const int* nb, int lane) {
     const int px = a.px;
     const int plane = px * a.ny;
     const long long N = a.N;
@@ -1042,10 +1017,11 @@ __device__ __forceinline__ void aa_store(typename Codec::T* f, const StepArgs<ty
                 Codec::template st<i>(aa_slot(f, N, kOPP[i], c), v[i], key);   // bounce-back: own slot
             } else if (cz == 1 && top) {
                 if (x + cx >= 0 && x + cx < a.nx)                               // through the lid
-                    Codec::template st<i>(aa_slot(f, N, kMIRZ[i], c + cx + (long long)px * cy), v[i], key);
+                    Codec::template st<i>(
+                        aa_slot(f, N, kMIRZ[i], pull_source<kSparse, -cx, -cy, 0>(c, px, plane, nb, lane)), v[i], key);
             } else if (x + cx >= 0 && x + cx < a.nx) {
-                Codec::template st<i>(aa_slot(f, N, i, c + cx + (long long)px * cy + (long long)plane * cz), v[i],
-                                      key);
+                Codec::template st<i>(aa_slot(f, N, i, pull_source<kSparse, -cx, -cy, -cz>(c, px, plane, nb, lane)),
+                                      v[i], key);
             }
         }
     });
@@ -1182,16 +1158,21 @@ __device__ __forceinline__ void aa_store32(typename Codec::T* f, const StepArgs<
     }
 }
 
-template <class Codec, bool kOdd, bool kPop32>
-__global__ void __launch_bounds__(256) k_step_aa(typename Codec::T* __restrict__ f, StepArgs<typename Codec::R> a) {
+// One in-place step of one x-tile (dense: blockDim.x cells of a row; tile-
+// sparse: the 64 cells of stored tile t). kPop32: the dense layout's 32-bit
+// buffer addressing.
+template <class Codec, bool kOdd, bool kPop32, bool kSparse>
+__device__ __forceinline__ void step_tile_aa(typename Codec::T* f, const StepArgs<typename Codec::R>& a, int row,
+                                             int tile, int t = 0) {
     using R = typename Codec::R;
-    int row, tile;
-    if (!tile_of(blockIdx.x, a.tiles, a.rows, row, tile)) return;   // whole work-group
-    const int x = tile * int(blockDim.x) + int(threadIdx.x);
-    const int xw = __builtin_amdgcn_readfirstlane(tile * int(blockDim.x) + (int(threadIdx.x) & ~63));
+    static_assert(!(kPop32 && kSparse), "32-bit addressing is for the dense layout");
+    const int lane = int(threadIdx.x) & 63;
+    const int x = kSparse ? tile * 64 + lane : tile * int(blockDim.x) + int(threadIdx.x);
+    const int xw = __builtin_amdgcn_readfirstlane(kSparse ? tile * 64 : tile * int(blockDim.x) + (int(threadIdx.x) & ~63));
     const bool inrow = x < a.nx;
     const int z = row / a.ny;
-    const int c = x + a.px * row;
+    const int c = kSparse ? t * 64 + lane : x + a.px * row;
+    const int* nb = kSparse ? a.nbr + 27 * t : nullptr;
     const uint8_t fl = inrow ? at(a.flag, c) : uint8_t(FLAG_SOLID);
     const bool active = !(fl & FLAG_SOLID);
     const uint32_t wm = (active && (fl & FLAG_NEARWALL)) ? at(a.wmask, c) : 0u;
@@ -1200,8 +1181,8 @@ __global__ void __launch_bounds__(256) k_step_aa(typename Codec::T* __restrict__
         if constexpr (kPop32)
             aa_gather32<Codec, kOdd>(f, a, c, x, z, wm, v);
         else
-            aa_gather<Codec, kOdd>(f, a, c, x, z, wm, v);
-        wall_function<Codec, false>(a, c, x, row - a.ny * z, fl, wm, v);
+            aa_gather<Codec, kOdd, kSparse>(f, a, c, x, z, wm, v, nb, lane);
+        wall_function<Codec, kSparse>(a, c, x, row - a.ny * z, fl, wm, v);
         collide_cell<Codec>(a, c, fl, wm, v);
     }
     if (boundary_wave(a, xw)) boundary_rows<Codec>(a, v, x, row, xw, active);
@@ -1209,15 +1190,32 @@ __global__ void __launch_bounds__(256) k_step_aa(typename Codec::T* __restrict__
         if constexpr (kPop32)
             aa_store32<Codec, kOdd>(f, a, c, x, z, wm, v);
         else
-            aa_store<Codec, kOdd>(f, a, c, x, z, wm, v);
+            aa_store<Codec, kOdd, kSparse>(f, a, c, x, z, wm, v, nb, lane);
     }
+}
+
+template <class Codec, bool kOdd, bool kPop32>
+__global__ void __launch_bounds__(256) k_step_aa(typename Codec::T* __restrict__ f, StepArgs<typename Codec::R> a) {
+    int row, tile;
+    if (!tile_of(blockIdx.x, a.tiles, a.rows, row, tile)) return;   // whole work-group
+    step_tile_aa<Codec, kOdd, kPop32, false>(f, a, row, tile);
+}
+
+// Tile-sparse layout: one stored tile per work-group (blk2tile; -1 idle).
+template <class Codec, bool kOdd>
+__global__ void __launch_bounds__(64) k_step_aa_sparse(typename Codec::T* __restrict__ f,
+                                                       StepArgs<typename Codec::R> a) {
+    const int t = __builtin_amdgcn_readfirstlane(a.blk2tile[blockIdx.x]);
+    if (t < 0) return;
+    const int tx = a.tcoord[3 * t], y = a.tcoord[3 * t + 1], z = a.tcoord[3 * t + 2];
+    step_tile_aa<Codec, kOdd, false, true>(f, a, y + a.ny * z, tx, t);
 }
 
 // The step with the learned closure (float arithmetic, pair-form collision).
 // The network runs on the whole wavefront between the moments and the
 // relaxation, so every lane reaches it; inactive lanes feed zero features.
 // Tile-sparse layout as in step_tile (one 64-cell tile per work-group).
-template <class Codec, bool kSparse = false>
+template <class Codec, bool kSparse = false, int kStream = 0>
 __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __restrict__ fs,
                                                   typename Codec::T* __restrict__ fd, const StepArgs<float>& a,
                                                   int row, int tile, const MlpPack* P, int t = 0) {
@@ -1240,7 +1238,12 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     float feat[kNF] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (active) {
         wm = (fl & FLAG_NEARWALL) ? at(a.wmask, c) : 0u;
-        gather<Codec, kSparse>(fs, a, c, x, row - a.ny * z, z, fl, wm, f, nb, lane);
+        if constexpr (kStream == 0) {
+            gather<Codec, kSparse>(fs, a, c, x, row - a.ny * z, z, fl, wm, f, nb, lane);
+        } else {   // in place: fd is the one buffer
+            aa_gather<Codec, kStream == 1, kSparse>(fd, a, c, x, z, wm, f, nb, lane);
+            wall_function<Codec, kSparse>(a, c, x, row - a.ny * z, fl, wm, f);
+        }
         if (forced) { fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale; }
         if (any_forced) pair_moments<R, true>(f, fx, fy, fz, m);
         else pair_moments<R, false>(f, fx, fy, fz, m);
@@ -1263,7 +1266,12 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
         if (a.write_all && a.c2out) at(a.c2out, c) = c2;
         if (a.colu_out_x) { at(a.colu_out_x, c) = m.ux; at(a.colu_out_y, c) = m.uy; }
     }
-    finish<Codec>(fd, a, f, c, x, row, xw, active);
+    if constexpr (kStream == 0) {
+        finish<Codec>(fd, a, f, c, x, row, xw, active);
+    } else {
+        if (boundary_wave(a, xw)) boundary_rows<Codec>(a, f, x, row, xw, active);
+        if (active) aa_store<Codec, kStream == 1, kSparse>(fd, a, c, x, z, wm, f, nb, lane);
+    }
 }
 
 template <class Codec>
@@ -1274,14 +1282,16 @@ k_step(const typename Codec::T* __restrict__ fs, typename Codec::T* __restrict__
     step_tile<Codec>(fs, fd, a, row, tile);
 }
 
-template <class Codec>
+// kStream: 0 two buffers (fs -> fd); 1 / 2 the odd / even in-place step on
+// the one buffer fd (fs unused).
+template <class Codec, int kStream = 0>
 __global__ void __launch_bounds__(256) k_step_learned(const typename Codec::T* __restrict__ fs,
                                                       typename Codec::T* __restrict__ fd, StepArgs<float> a) {
     __shared__ MlpPack P;
     int row, tile;
     if (!tile_of(blockIdx.x, a.tiles, a.rows, row, tile)) return;   // whole work-group
     stage_pack(a.mlp, &P);
-    step_tile_learned<Codec>(fs, fd, a, row, tile, &P);
+    step_tile_learned<Codec, false, kStream>(fs, fd, a, row, tile, &P);
 }
 
 // Tile-sparse layout: G wavefronts per work-group, one stored tile each
@@ -1290,7 +1300,7 @@ __global__ void __launch_bounds__(256) k_step_learned(const typename Codec::T* _
 // step four, so that one LDS staging of the network serves four tiles.
 constexpr int kTilesPerGroupLearned = 4;
 
-template <class Codec, int G>
+template <class Codec, int G, int kStream = 0>
 __global__ void __launch_bounds__(64 * G) k_step_learned_sparse(const typename Codec::T* __restrict__ fs,
                                                                typename Codec::T* __restrict__ fd,
                                                                StepArgs<float> a) {
@@ -1300,7 +1310,7 @@ __global__ void __launch_bounds__(64 * G) k_step_learned_sparse(const typename C
     const int t = __builtin_amdgcn_readfirstlane(a.blk2tile[blockIdx.x * G + (threadIdx.x >> 6)]);
     if (t < 0) return;   // whole wavefront
     const int tx = a.tcoord[3 * t], y = a.tcoord[3 * t + 1], z = a.tcoord[3 * t + 2];
-    step_tile_learned<Codec, true>(fs, fd, a, y + a.ny * z, tx, &P, t);
+    step_tile_learned<Codec, true, kStream>(fs, fd, a, y + a.ny * z, tx, &P, t);
 }
 
 template <class Codec, int G>

@@ -484,7 +484,7 @@ struct Solver final : wfs_solver {
 
     // A-B layout <-> state S of the in-place buffer (the same exchange both ways).
     int aa_exchange() {
-        hipLaunchKernelGGL(k_aa_exchange<T>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, fp(cur), Np, d.nx, px);
+        hipLaunchKernelGGL(k_aa_exchange<T>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, fp(cur), Np, cmap());
         WFS_LAUNCH_CHECK();
         return WFS_OK;
     }
@@ -500,16 +500,12 @@ struct Solver final : wfs_solver {
     }
 
     // In-place streaming on: the population buffer not in use is freed and
-    // the state converted to S; off: the second buffer comes back. The
-    // constant-Cs closure on the dense layout, with the outlet row fused into
-    // the step (nx - 1 not a multiple of 64).
+    // the state converted to S; off: the second buffer comes back. Needs the
+    // outlet row fused into the step (nx - 1 not a multiple of 64).
     int set_inplace(int on) override {
         if (!on == !inplace) return WFS_OK;
         int rc;
         if (on) {
-            if (tiled) return fail(WFS_EINVAL, "wfs_set_inplace: in-place streaming runs on the dense layout");
-            if (closure != WFS_CLOSURE_SMAGORINSKY)
-                return fail(WFS_EINVAL, "wfs_set_inplace: in-place streaming runs the constant-Cs closure");
             if ((d.nx - 1) % 64 == 0)
                 return fail(WFS_EINVAL, "wfs_set_inplace: nx - 1 must not be a multiple of 64 (the outlet row is "
                                         "fused into the step)");
@@ -530,8 +526,44 @@ struct Solver final : wfs_solver {
         return WFS_OK;
     }
 
+    // One in-place step: odd (from state S) or even, by aa_phase.
+    template <bool kOdd>
+    void launch_step_aa(const dim3& grid, const StepArgs<R>& a) {
+        constexpr int kStream = kOdd ? 1 : 2;
+        if constexpr (std::is_same<R, float>::value) {
+            if (closure == WFS_CLOSURE_MLP) {
+                if (tiled) {
+                    constexpr int G = kTilesPerGroupLearned;
+                    StepArgs<R> b = a;
+                    b.blk2tile = static_cast<const int*>(blk2tile4_d.p);
+                    hipLaunchKernelGGL((k_step_learned_sparse<Codec, G, kStream>),
+                                       dim3((unsigned)(h_blk2tile4.size() / G)), dim3(64 * G), 0, stream, nullptr,
+                                       fp(cur), b);
+                } else {
+                    hipLaunchKernelGGL((k_step_learned<Codec, kStream>), grid, dim3(bx), 0, stream, nullptr, fp(cur),
+                                       a);
+                }
+                return;
+            }
+        }
+        if (tiled)
+            hipLaunchKernelGGL((k_step_aa_sparse<Codec, kOdd>), dim3((unsigned)h_blk2tile.size()), dim3(64), 0, stream,
+                               fp(cur), a);
+        else if (a.pop32)
+            hipLaunchKernelGGL((k_step_aa<Codec, kOdd, true>), grid, dim3(bx), 0, stream, fp(cur), a);
+        else
+            hipLaunchKernelGGL((k_step_aa<Codec, kOdd, false>), grid, dim3(bx), 0, stream, fp(cur), a);
+    }
+
     // One step launch: the constant-Cs kernel or, with the learned closure, k_step_learned.
     void launch_step(const dim3& grid, const StepArgs<R>& a) {
+        if (inplace) {
+            if (aa_phase == 0)
+                launch_step_aa<true>(grid, a);
+            else
+                launch_step_aa<false>(grid, a);
+            return;
+        }
         if (tiled) {
             if constexpr (std::is_same<R, float>::value) {
                 if (closure == WFS_CLOSURE_MLP) {
@@ -569,13 +601,7 @@ struct Solver final : wfs_solver {
                 int rc = mark();
                 if (rc) return rc;
             }
-            if (inplace) {
-                auto* aa = aa_phase == 0 ? (a.pop32 ? k_step_aa<Codec, true, true> : k_step_aa<Codec, true, false>)
-                                         : (a.pop32 ? k_step_aa<Codec, false, true> : k_step_aa<Codec, false, false>);
-                hipLaunchKernelGGL(aa, grid, dim3(bx), 0, stream, fp(cur), a);
-            } else {
-                launch_step(grid, a);
-            }
+            launch_step(grid, a);
             WFS_LAUNCH_CHECK();
             if (timing) {
                 int rc = mark();
@@ -758,7 +784,6 @@ struct Solver final : wfs_solver {
             return WFS_OK;
         }
         if (kind != WFS_CLOSURE_MLP) return fail(WFS_EINVAL, "wfs_set_closure: unknown closure kind");
-        if (inplace) return fail(WFS_EINVAL, "wfs_set_closure: in-place streaming runs the constant-Cs closure");
         if constexpr (!std::is_same<R, float>::value) {
             return fail(WFS_EINVAL, "wfs_set_closure: the learned closure runs in float arithmetic (not FP64 storage)");
         } else {

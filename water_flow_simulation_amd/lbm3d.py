@@ -291,8 +291,8 @@ class LBM3D:
         """on: stream in place in one population buffer (the second is freed;
         two steps equal two two-buffer steps bit for bit); False: back to two
         buffers. populations, set_populations, forces and switching off need
-        an even number of steps since switching on. Dense layout and the
-        constant-Cs closure only; nx - 1 must not be a multiple of 64.
+        an even number of steps since switching on. nx - 1 must not be a
+        multiple of 64.
         LBM3D(streaming="inplace") switches it on before the upload, so the
         second buffer is never allocated."""
         nat.check(nat.lib().wfs_set_inplace(self.handle, int(bool(on))))

@@ -463,7 +463,7 @@ class Run3D:
         refreshes it every step inside the step kernel.
         streaming: "ab" streams between two population buffers, as the
         reference does; "inplace" keeps one (half the population memory, the
-        same results bit for bit; constant-Cs closure, dense layout)."""
+        same results bit for bit)."""
         if column_u not in ("cadence", "per_step"):
             raise ValueError("column_u: 'cadence' or 'per_step'")
         if streaming not in ("ab", "inplace"):
