@@ -155,9 +155,11 @@ def time_meander(variants, steps, scale=1.0):
     from water_flow_simulation_amd import meander
     out = {}
     for name in variants:
-        st, learned = name.split("+")[0], name.endswith("+mlp")
+        st, opts = name.split("+")[0], name.split("+")[1:]
+        learned = "mlp" in opts
         log(f"meander {name}")
-        r = meander.run(scale, steps, warmup=20, storage=st, learned=learned, log=log)
+        r = meander.run(scale, steps, warmup=20, storage=st, learned=learned, log=log,
+                        streaming="inplace" if "inplace" in opts else "ab")
         k_s = r["kernel_ms"] / 1e3
         b_alg = B_ALG[st] + (2 if learned else 0)
         out[name] = {"kernel_ms_avg": r["kernel_ms"], "mlups_fluid": r["mlups_fluid"],

@@ -259,6 +259,23 @@ def test_inplace_on_tiles_matches_two_buffers(storage):
         run.close()
 
 
+def test_inplace_on_tiles_beyond_32_bit_offsets():
+    """A tile-sparse buffer over 4 GiB (62 M stored cells, FP32): one buffer
+    resource per access instead of one for the whole buffer."""
+    print("case: meander at scale 0.25 (3125x777x50 box), tiles, f32 (buffer > 4 GiB), in-place vs two buffers",
+          flush=True)
+    from dataclasses import replace
+    from water_flow_simulation_amd.meander import meander_case
+    r = replace(meander_case(scale=0.25), layout="tiles")
+    runs = _runs(r, storage="f32")
+    assert 19 * 4 * runs[1].lbm.stored_cells >= 2 ** 32
+    for run in runs:
+        run.advance(20)
+    _same_runs(runs, ~r.solid.ravel())
+    for run in runs:
+        run.close()
+
+
 @pytest.mark.parametrize("layout,storage", [("dense", "f32"), ("dense", "f16dev"), ("tiles", "f16dev")])
 def test_inplace_with_learned_closure(layout, storage):
     print(f"case: meander at scale 0.05, learned closure, {layout}, {storage}, in-place vs two buffers", flush=True)

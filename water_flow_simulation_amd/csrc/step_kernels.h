@@ -959,18 +959,15 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
 // after an odd step); the results go back to the cell's own slots, giving S.
 // Two steps equal two A-B steps, bit for bit.
 
-// Slot k of cell c.
-template <typename T>
-__device__ __forceinline__ T* aa_slot(T* f, long long N, int k, long long c) {
-    return f + (long long)k * N + c;
-}
-
-// Either layout, 64-bit addresses. Neighbour cells come from pull_source: the
-// cell at x - c for a pull, at x + c (the offset negated) for a push.
+// Either layout, link by link. Neighbour cells come from pull_source: the
+// cell at x - c for a pull, at x + c (the offset negated) for a push. A lane
+// selects between uniform slot pointers (f + k N) and adds its cell's 32-bit
+// byte offset (each direction is below 4 GiB).
 template <class Codec, bool kOdd, bool kSparse>
 __device__ __forceinline__ void aa_gather(const typename Codec::T* f, const StepArgs<typename Codec::R>& a, int c,
                                           int x, int z, uint32_t wm, typename Codec::R* out, const int* nb,
                                           int lane) {
+    using T = typename Codec::T;
     const int px = a.px;
     const int plane = px * a.ny;
     const long long N = a.N;
@@ -982,26 +979,29 @@ __device__ __forceinline__ void aa_gather(const typename Codec::T* f, const Step
         const bool wall = !xout && ((wm >> i) & 1u);
         constexpr int ki = center_class(i), ko = center_class(kOPP[i]);
         const float e = (ki != ko && wall) ? a.ctr.c[ko] : a.ctr.c[ki];
-        long long src = c;
-        int slot = i;
+        const T* own = f + (long long)i * N;   // x-end copy, bounce-back, or the even step
         if constexpr (kOdd) {
-            if (!xout && !wall) {
-                if (cz == -1 && top) {
-                    src = pull_source<kSparse, cx, cy, 0>(c, px, plane, nb, lane);
-                    slot = kOPP[kMIRZ[i]];
-                } else {
-                    src = pull_source<kSparse, cx, cy, cz>(c, px, plane, nb, lane);
-                    slot = kOPP[i];
-                }
+            const T* pn;
+            int s;
+            if (cz == -1 && top) {   // through the lid: slot opp(MIRZ(i)) of the in-plane source
+                pn = f + (long long)kOPP[kMIRZ[i]] * N;
+                s = pull_source<kSparse, cx, cy, 0>(c, px, plane, nb, lane);
+            } else {
+                pn = f + (long long)kOPP[i] * N;
+                s = pull_source<kSparse, cx, cy, cz>(c, px, plane, nb, lane);
             }
+            const bool mine = xout || wall;
+            out[i] = Codec::template ld<i>(elem(mine ? own : pn, (uint32_t)(mine ? c : s)), e);
+        } else {
+            out[i] = Codec::template ld<i>(elem(own, (uint32_t)c), e);
         }
-        out[i] = Codec::template ld<i>(aa_slot(f, N, slot, src), e);
     });
 }
 
 template <class Codec, bool kOdd, bool kSparse>
 __device__ __forceinline__ void aa_store(typename Codec::T* f, const StepArgs<typename Codec::R>& a, int c, int x,
                                          int z, uint32_t wm, const typename Codec::R* v, const int* nb, int lane) {
+    using T = typename Codec::T;
     const int px = a.px;
     const int plane = px * a.ny;
     const long long N = a.N;
@@ -1010,29 +1010,34 @@ __device__ __forceinline__ void aa_store(typename Codec::T* f, const StepArgs<ty
         constexpr int i = I;
         constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
         const float key = store_key<Codec>(a.ctr, i);
+        T* back = f + (long long)kOPP[i] * N;   // bounce-back, or the even step: own slot opp(i)
         if constexpr (!kOdd) {
-            Codec::template st<i>(aa_slot(f, N, kOPP[i], c), v[i], key);
+            Codec::template st<i>(elem(back, (uint32_t)c), v[i], key);
         } else {
-            if ((wm >> kOPP[i]) & 1u) {
-                Codec::template st<i>(aa_slot(f, N, kOPP[i], c), v[i], key);   // bounce-back: own slot
-            } else if (cz == 1 && top) {
-                if (x + cx >= 0 && x + cx < a.nx)                               // through the lid
-                    Codec::template st<i>(
-                        aa_slot(f, N, kMIRZ[i], pull_source<kSparse, -cx, -cy, 0>(c, px, plane, nb, lane)), v[i], key);
-            } else if (x + cx >= 0 && x + cx < a.nx) {
-                Codec::template st<i>(aa_slot(f, N, i, pull_source<kSparse, -cx, -cy, -cz>(c, px, plane, nb, lane)),
-                                      v[i], key);
+            const bool bounce = (wm >> kOPP[i]) & 1u;
+            const bool inx = !((cx == 1 && xhi) || (cx == -1 && xlo));
+            T* pt;
+            int s;
+            if (cz == 1 && top) {   // through the lid: slot MIRZ(i) of the in-plane target
+                pt = f + (long long)kMIRZ[i] * N;
+                s = pull_source<kSparse, -cx, -cy, 0>(c, px, plane, nb, lane);
+            } else {
+                pt = f + (long long)i * N;
+                s = pull_source<kSparse, -cx, -cy, -cz>(c, px, plane, nb, lane);
             }
+            if (bounce || inx) Codec::template st<i>(elem(bounce ? back : pt, (uint32_t)(bounce ? c : s)), v[i], key);
         }
     });
     // x-end rows: a link entering from outside the box keeps the cell's own population
-    static_for<Q>([&](auto I) {
-        constexpr int i = I;
-        if constexpr (kCX[i] != 0) {
-            if ((kCX[i] == 1 && xlo) || (kCX[i] == -1 && xhi))
-                Codec::template st<i>(aa_slot(f, N, i, c), v[i], store_key<Codec>(a.ctr, i));
-        }
-    });
+    if (xlo || xhi) {
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            if constexpr (kCX[i] != 0) {
+                if ((kCX[i] == 1 && xlo) || (kCX[i] == -1 && xhi))
+                    Codec::template st<i>(elem(f + (long long)i * N, (uint32_t)c), v[i], store_key<Codec>(a.ctr, i));
+            }
+        });
+    }
 }
 
 // The same two steps with 32-bit addressing (Q * N * sizeof(T) < 4 GiB): one
@@ -1160,12 +1165,12 @@ __device__ __forceinline__ void aa_store32(typename Codec::T* f, const StepArgs<
 
 // One in-place step of one x-tile (dense: blockDim.x cells of a row; tile-
 // sparse: the 64 cells of stored tile t). kPop32: the dense layout's 32-bit
-// buffer addressing.
+// buffer addressing (whole buffer below 4 GiB).
 template <class Codec, bool kOdd, bool kPop32, bool kSparse>
 __device__ __forceinline__ void step_tile_aa(typename Codec::T* f, const StepArgs<typename Codec::R>& a, int row,
                                              int tile, int t = 0) {
     using R = typename Codec::R;
-    static_assert(!(kPop32 && kSparse), "32-bit addressing is for the dense layout");
+    static_assert(!(kPop32 && kSparse), "32-bit buffer addressing is for the dense layout");
     const int lane = int(threadIdx.x) & 63;
     const int x = kSparse ? tile * 64 + lane : tile * int(blockDim.x) + int(threadIdx.x);
     const int xw = __builtin_amdgcn_readfirstlane(kSparse ? tile * 64 : tile * int(blockDim.x) + (int(threadIdx.x) & ~63));

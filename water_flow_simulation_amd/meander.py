@@ -208,14 +208,16 @@ def budgets(run_, reach):
             "forces_N": {k: [round(float(v), 1) for v in f] for k, f in run_.forces().items()}}
 
 
-def run(scale=1.0, steps=200, warmup=20, storage="f16dev", learned=False, log=print, report_every=0):
+def run(scale=1.0, steps=200, warmup=20, storage="f16dev", learned=False, log=print, report_every=0,
+        streaming="ab"):
     """Builds the case, steps it inside the run cadence and reports MLUPS over
     stored and fluid cells, device memory, and the FP64 budgets: mass,
     discharge at the inlet and outlet, thrust and the forces on the hulls and
     on bed and banks. learned: the committed learned closure instead of
     constant-Cs Smagorinsky. report_every > 0: every that many steps, check
     the state is finite and log one JSON line with the step, simulated time
-    and the budgets."""
+    and the budgets. streaming: "ab" (two population buffers) or "inplace"
+    (one)."""
     t0 = time.time()
     reach = meander_case(scale)
     t_build = time.time() - t0
@@ -228,7 +230,7 @@ def run(scale=1.0, steps=200, warmup=20, storage="f16dev", learned=False, log=pr
     if learned:
         from .closure import load_params
         closure = load_params()[0]
-    run_ = R.Run3D(reach, storage=storage, closure=closure)
+    run_ = R.Run3D(reach, storage=storage, closure=closure, streaming=streaming)
     t_up = time.time() - t1
     L = run_.lbm
     stored = L.stored_cells
@@ -256,7 +258,7 @@ def run(scale=1.0, steps=200, warmup=20, storage="f16dev", learned=False, log=pr
            "upload_s": round(t_up, 1), "wall_s": round(wall, 3), "kernel_ms": round(ms / max(n, 1), 4),
            "mlups_fluid": round(steps * fluid / wall / 1e6, 1), "mlups_stored": round(steps * stored / wall / 1e6, 1),
            "device_gb": round(L.memory_mb() / 1e3, 2), **budgets(run_, reach), "t_s": round(run_.t, 3),
-           "dt_s": reach.dt, "Q": reach.Q, "closure": "learned" if learned else "smagorinsky",
+           "dt_s": reach.dt, "Q": reach.Q, "closure": "learned" if learned else "smagorinsky", "streaming": streaming,
            "thrust_design_N": float(sum(p["T"] for p in reach.props))}
     run_.close()
     return res
@@ -269,9 +271,12 @@ def main():
     ap.add_argument("--storage", default="f16dev")
     ap.add_argument("--learned", action="store_true", help="the learned sub-grid closure")
     ap.add_argument("--report-every", type=int, default=0, help="steps between budget reports (0: none)")
+    ap.add_argument("--streaming", default="ab", choices=("ab", "inplace"),
+                    help="two population buffers or one streamed in place")
     a = ap.parse_args()
     print(json.dumps(run(a.scale, a.steps, storage=a.storage, learned=a.learned,
-                         log=lambda m: print(m, flush=True), report_every=a.report_every)), flush=True)
+                         log=lambda m: print(m, flush=True), report_every=a.report_every,
+                         streaming=a.streaming)), flush=True)
 
 
 if __name__ == "__main__":
