@@ -972,6 +972,20 @@ __device__ __forceinline__ void aa_gather(const typename Codec::T* f, const Step
     const int plane = px * a.ny;
     const long long N = a.N;
     const bool xlo = x == 0, xhi = x == a.nx - 1, top = z == a.nz - 1;
+    if (kOdd && wm == 0u && !xlo && !xhi && !top) {
+        // interior: slot opp(i) of x - c_i for every link, at a uniform pointer
+        // plus the cell's (dense) or lane's (tile-sparse) offset
+        const uint32_t off = (uint32_t)(kSparse ? lane : c);
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+            const T* pn = f + (long long)kOPP[i] * N +
+                          (kSparse ? 64LL * nb[1 + 3 * ((1 - cy) + 3 * (1 - cz))] - cx
+                                   : -(cx + (long long)px * cy + (long long)plane * cz));
+            out[i] = Codec::template ld<i>(elem(pn, off), a.ctr.c[center_class(i)]);
+        });
+        return;
+    }
     static_for<Q>([&](auto I) {
         constexpr int i = I;
         constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
@@ -1006,6 +1020,19 @@ __device__ __forceinline__ void aa_store(typename Codec::T* f, const StepArgs<ty
     const int plane = px * a.ny;
     const long long N = a.N;
     const bool xlo = x == 0, xhi = x == a.nx - 1, top = z == a.nz - 1;
+    if (kOdd && wm == 0u && !xlo && !xhi && !top) {
+        // interior: slot i of x + c_i for every link
+        const uint32_t off = (uint32_t)(kSparse ? lane : c);
+        static_for<Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+            T* pt = f + (long long)i * N +
+                    (kSparse ? 64LL * nb[1 + 3 * ((1 + cy) + 3 * (1 + cz))] + cx
+                             : cx + (long long)px * cy + (long long)plane * cz);
+            Codec::template st<i>(elem(pt, off), v[i], store_key<Codec>(a.ctr, i));
+        });
+        return;
+    }
     static_for<Q>([&](auto I) {
         constexpr int i = I;
         constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
