@@ -261,10 +261,10 @@ __device__ __forceinline__ void store_cell(typename Codec::T* __restrict__ fd, l
             buf_store<T>(r, off, Codec::template enc<i>(f[i], store_key<Codec>(ctr, i)), (uint32_t)i * bytes);
         });
     } else {
+        // direction i at the uniform pointer fd + i N, the cell's 32-bit offset
         static_for<Q>([&](auto I) {
             constexpr int i = I;
-            buf_store<T>(buf_rsrc(fd + (long long)i * N, bytes), off,
-                         Codec::template enc<i>(f[i], store_key<Codec>(ctr, i)));
+            Codec::template st<i>(elem(fd + (long long)i * N, (uint32_t)c), f[i], store_key<Codec>(ctr, i));
         });
     }
 }
@@ -637,7 +637,8 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
                 f[i] = Codec::template dec<i>(buf_load<T>(rs, voff, base * ts), a.ctr.c[center_class(i)]);
             });
         } else {
-            // one resource per direction at its pull base (i N + ... >= 0)
+            // a uniform pointer per direction at its pull base (i N + ... >= 0)
+            // and the lane's 32-bit offset
             static_for<Q>([&](auto I) {
                 constexpr int i = I;
                 long long base;
@@ -645,8 +646,8 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
                     base = (long long)i * N + (long long)row_of(I) * 64 - kCX[i];
                 else
                     base = (long long)i * N - (kCX[i] + (long long)px * kCY[i] + (long long)plane * kCZ[i]);
-                f[i] = Codec::template dec<i>(buf_load<T>(buf_rsrc(fs + base, (uint32_t)N * ts), voff),
-                                              a.ctr.c[center_class(i)]);
+                f[i] = Codec::template ld<i>(elem(fs + base, (uint32_t)(kSparse ? lane : c)),
+                                             a.ctr.c[center_class(i)]);
             });
         }
     } else {
