@@ -423,9 +423,9 @@ __global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __r
         const uint8_t fl = x < a.nx ? a.flag[c] : uint8_t(FLAG_SOLID);
         if (!(fl & FLAG_SOLID)) {
             if (fl & FLAG_FORCED) {
-                acc[0] = double(a.Fx[c] * a.fscale);
-                acc[1] = double(a.Fy[c] * a.fscale);
-                acc[2] = double(a.Fz[c] * a.fscale);
+                acc[0] = double(a.aux->Fx[c] * a.fscale);
+                acc[1] = double(a.aux->Fy[c] * a.fscale);
+                acc[2] = double(a.aux->Fz[c] * a.fscale);
             }
             if (fl & FLAG_NEARWALL) {
                 const uint32_t wm = a.wmask[c];

@@ -269,19 +269,41 @@ __device__ __forceinline__ void store_cell(typename Codec::T* __restrict__ fd, l
     }
 }
 
+// Arrays the step reads or writes in some cells only: forces (forced cells),
+// the macroscopic fields (wall-model cells, and every cell on the last step of
+// a call), the wall model's inputs and the per-step column buffers. They sit
+// in device memory behind one pointer, so a kernel loads each where it is
+// used; as kernel arguments the compiler loads them all at entry and spills
+// them to vector-register lanes.
+template <typename R>
+struct StepAux {
+    const R* Fx;
+    const R* Fy;
+    const R* Fz;
+    R* rho;
+    R* ux;
+    R* uy;
+    R* uz;
+    const R* cfw;
+    const R* uref;
+    const uint8_t* bedwall;   // per cell: bed or bank cell of the wall model
+    const uint8_t* colcount;  // per column (x + px y): fluid cells, at least 1
+    // per-step column velocity for the wall model (when StepArgs::colu_on):
+    // every step writes ux, uy of its fluid cells to colu_out_*, and a bed/bank
+    // wall-model cell takes uref from its column of colu_in_* (the previous
+    // step), as the run cadence does every 20 steps
+    const R* colu_in_x;
+    const R* colu_in_y;
+    R* colu_out_x;
+    R* colu_out_y;
+};
+
 template <typename R>
 struct StepArgs {
     const uint8_t* __restrict__ flag;
     const uint32_t* __restrict__ wmask;
-    const R* __restrict__ Fx;
-    const R* __restrict__ Fy;
-    const R* __restrict__ Fz;
-    R* __restrict__ rho;
-    R* __restrict__ ux;
-    R* __restrict__ uy;
-    R* __restrict__ uz;
-    const R* __restrict__ cfw;
-    const R* __restrict__ uref;
+    const StepAux<R>* __restrict__ aux;
+    int colu_on;   // per-step column velocity (StepAux::colu_*)
     R fscale;      // forces used are F * fscale (thrust ramp)
     R tau0;
     R tau0sq;      // tau0 * tau0
@@ -309,16 +331,6 @@ struct StepArgs {
     const int* __restrict__ blk2tile;   // work-group -> tile (-1: idle), rows kept on one XCD
     const int* __restrict__ tmap;       // tile-sparse layout: x-tile + ntx * row -> stored tile or -1
     int ntx;
-    // per-step column velocity for the wall model (off when colu_in_x is null):
-    // every step writes ux, uy of its fluid cells to colu_out_*, and a bed/bank
-    // wall-model cell takes uref from its column of colu_in_* (the previous
-    // step), as the run cadence does every 20 steps
-    const R* __restrict__ colu_in_x;
-    const R* __restrict__ colu_in_y;
-    R* __restrict__ colu_out_x;
-    R* __restrict__ colu_out_y;
-    const uint8_t* __restrict__ bedwall;   // per cell: bed or bank cell of the wall model
-    const uint8_t* __restrict__ colcount;  // per column (x + px y): fluid cells, at least 1
 };
 
 // Work-group -> (row, x-tile) map. Work-groups are dispatched round-robin over
@@ -534,10 +546,10 @@ __device__ __forceinline__ R column_u(const StepArgs<R>& a, int x, int y) {
         } else {
             cc = x + (long long)a.px * row;
         }
-        sx += at(a.colu_in_x, int(cc));
-        sy += at(a.colu_in_y, int(cc));
+        sx += at(a.aux->colu_in_x, int(cc));
+        sy += at(a.aux->colu_in_y, int(cc));
     }
-    const R ncol = R(a.colcount[x + (long long)a.px * y]);
+    const R ncol = R(a.aux->colcount[x + (long long)a.px * y]);
     return sqrt(sx * sx + sy * sy) / ncol;
 }
 
@@ -550,7 +562,8 @@ __device__ __forceinline__ void wall_function(const StepArgs<typename Codec::R>&
                                               uint32_t wm, typename Codec::R* f) {
     using R = typename Codec::R;
     if (wm != 0u && (fl & FLAG_CF)) {
-        const R cf = at(a.cfw, c);
+        const StepAux<R>& ax = *a.aux;
+        const R cf = at(ax.cfw, c);
         int inx = 0, iny = 0, inz = 0;
         static_for<Q>([&](auto I) {
             constexpr int i = I;
@@ -564,13 +577,13 @@ __device__ __forceinline__ void wall_function(const StepArgs<typename Codec::R>&
         const R area = sqrt(afx * afx + afy * afy + afz * afz);
         if (nn > R(0) && area > R(0)) {
             const R ex = nwx / nn, ey = nwy / nn, ez = nwz / nn;
-            const R ox = at(a.ux, c), oy = at(a.uy, c), oz = at(a.uz, c);   // previous step, own cell
+            const R ox = at(ax.ux, c), oy = at(ax.uy, c), oz = at(ax.uz, c);   // previous step, own cell
             const R un = ox * ex + oy * ey + oz * ez;
             R tx = ox - un * ex, ty = oy - un * ey, tz = oz - un * ez;
             const R ut = sqrt(tx * tx + ty * ty + tz * tz);
             if (ut > R(1e-7)) {
                 tx /= ut; ty /= ut; tz /= ut;
-                const R rw = at(a.rho, c);
+                const R rw = at(ax.rho, c);
                 R D = R(0), K = R(0);
                 static_for<Q>([&](auto I) {
                     constexpr int i = I;
@@ -581,8 +594,8 @@ __device__ __forceinline__ void wall_function(const StepArgs<typename Codec::R>&
                     }
                 });
                 if (K > R(0)) {
-                    const R ur = !a.colu_in_x ? at(a.uref, c)
-                                 : (at(a.bedwall, c) ? column_u<kSparse>(a, x, y) : R(0));
+                    const R ur = !a.colu_on ? at(ax.uref, c)
+                                 : (at(ax.bedwall, c) ? column_u<kSparse>(a, x, y) : R(0));
                     const R U = ur > R(0) ? ur : ut;
                     const R us0 = (R(-2) * D - rw * cf * U * U * area) / (R(6) * rw * K);
                     const R us = fmin(fmax(us0, -ut), ut);
@@ -800,7 +813,7 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename Codec::R>& 
         const bool forced = (fl & FLAG_FORCED) != 0;
         R fx = R(0), fy = R(0), fz = R(0);
         if (forced) {
-            fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale;
+            fx = at(a.aux->Fx, c) * a.fscale; fy = at(a.aux->Fy, c) * a.fscale; fz = at(a.aux->Fz, c) * a.fscale;
             ux = (mx + R(0.5) * fx) / r; uy = (my + R(0.5) * fy) / r; uz = (mz + R(0.5) * fz) / r;
         } else {
             ux = mx / r; uy = my / r; uz = mz / r;
@@ -871,16 +884,17 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename Codec::R>& 
         const bool forced = (fl & FLAG_FORCED) != 0;
         if (__any(forced)) {   // uniform over the wavefront; forced cells are few
             R fx = R(0), fy = R(0), fz = R(0);
-            if (forced) { fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale; }
+            if (forced) { fx = at(a.aux->Fx, c) * a.fscale; fy = at(a.aux->Fy, c) * a.fscale; fz = at(a.aux->Fz, c) * a.fscale; }
             collide_pairs<R, true>(f, fx, fy, fz, a.tau0, a.tau0sq, a.ksmag, r, ux, uy, uz);
         } else {
             collide_pairs<R, false>(f, R(0), R(0), R(0), a.tau0, a.tau0sq, a.ksmag, r, ux, uy, uz);
         }
     }
     if (a.write_all || (wm != 0u && (fl & FLAG_CF))) {
-        at(a.rho, c) = r; at(a.ux, c) = ux; at(a.uy, c) = uy; at(a.uz, c) = uz;
+        const StepAux<R>& ax = *a.aux;
+        at(ax.rho, c) = r; at(ax.ux, c) = ux; at(ax.uy, c) = uy; at(ax.uz, c) = uz;
     }
-    if (a.colu_out_x) { at(a.colu_out_x, c) = ux; at(a.colu_out_y, c) = uy; }
+    if (a.colu_on) { at(a.aux->colu_out_x, c) = ux; at(a.aux->colu_out_y, c) = uy; }
 }
 
 // The update of one active cell: pull, wall function, collision; f gets the
@@ -1277,7 +1291,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
             aa_gather<Codec, kStream == 1, kSparse>(fd, a, c, x, z, wm, f, nb, lane);
             wall_function<Codec, kSparse>(a, c, x, row - a.ny * z, fl, wm, f);
         }
-        if (forced) { fx = at(a.Fx, c) * a.fscale; fy = at(a.Fy, c) * a.fscale; fz = at(a.Fz, c) * a.fscale; }
+        if (forced) { fx = at(a.aux->Fx, c) * a.fscale; fy = at(a.aux->Fy, c) * a.fscale; fz = at(a.aux->Fz, c) * a.fscale; }
         if (any_forced) pair_moments<R, true>(f, fx, fy, fz, m);
         else pair_moments<R, false>(f, fx, fy, fz, m);
         // resolved strain rate S = -3 Pi_dev / (2 rho tau), tau from the constant-Cs model
@@ -1294,10 +1308,11 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
         if (any_forced) pair_relax<R, true>(f, fx, fy, fz, m, a.tau0, a.tau0sq, ks);
         else pair_relax<R, false>(f, fx, fy, fz, m, a.tau0, a.tau0sq, ks);
         if (a.write_all || (wm != 0u && (fl & FLAG_CF))) {
-            at(a.rho, c) = m.r; at(a.ux, c) = m.ux; at(a.uy, c) = m.uy; at(a.uz, c) = m.uz;
+            const StepAux<R>& ax = *a.aux;
+            at(ax.rho, c) = m.r; at(ax.ux, c) = m.ux; at(ax.uy, c) = m.uy; at(ax.uz, c) = m.uz;
         }
         if (a.write_all && a.c2out) at(a.c2out, c) = c2;
-        if (a.colu_out_x) { at(a.colu_out_x, c) = m.ux; at(a.colu_out_y, c) = m.uy; }
+        if (a.colu_on) { at(a.aux->colu_out_x, c) = m.ux; at(a.aux->colu_out_y, c) = m.uy; }
     }
     if constexpr (kStream == 0) {
         finish<Codec>(fd, a, f, c, x, row, xw, active);

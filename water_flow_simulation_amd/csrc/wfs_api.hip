@@ -190,6 +190,9 @@ struct Solver final : wfs_solver {
     long long part_n = 0, part_ninlet = 0, part_step = 0;
     uint64_t part_seed = 0;
     DevBuf tmap_d, tcoord_d, nbr_d, blk2tile_d, blk2tile4_d;   // tile-sparse tables (when tiled)
+    DevBuf aux_d;                                     // StepAux for column-buffer parity 0 and 1
+    StepAux<R> aux_h[2]{};                            // what aux_d holds
+    bool aux_up = false;
 
     CellMap cmap() const {
         CellMap m;
@@ -422,18 +425,45 @@ struct Solver final : wfs_solver {
         return WFS_OK;
     }
 
-    void fill_args(StepArgs<R>& a) {
+    // The StepAux pair (column-buffer parity 0 and 1) in device memory,
+    // uploaded again when one of its pointers changed (set-up,
+    // wfs_set_column_u); running steps read it, so the stream drains first.
+    int sync_aux() {
+        StepAux<R> v[2]{};
+        for (int p = 0; p < 2; ++p) {
+            v[p].Fx = rp(Fx);
+            v[p].Fy = rp(Fy);
+            v[p].Fz = rp(Fz);
+            v[p].rho = rp(rho);
+            v[p].ux = rp(ux);
+            v[p].uy = rp(uy);
+            v[p].uz = rp(uz);
+            v[p].cfw = rp(cfw);
+            v[p].uref = rp(uref);
+            v[p].bedwall = static_cast<const uint8_t*>(bedwall.p);
+            v[p].colcount = static_cast<const uint8_t*>(colcount.p);
+            if (colu_on) {
+                v[p].colu_in_x = rp(colu[p][0]);
+                v[p].colu_in_y = rp(colu[p][1]);
+                v[p].colu_out_x = rp(colu[p ^ 1][0]);
+                v[p].colu_out_y = rp(colu[p ^ 1][1]);
+            }
+        }
+        if (aux_up && std::memcmp(v, aux_h, sizeof v) == 0) return WFS_OK;
+        int rc;
+        if (!aux_d.p && (rc = alloc_buf(aux_d, sizeof v))) return rc;
+        WFS_HIP(hipStreamSynchronize(stream));
+        WFS_HIP(hipMemcpy(aux_d.p, v, sizeof v, hipMemcpyHostToDevice));
+        std::memcpy(aux_h, v, sizeof v);
+        aux_up = true;
+        return WFS_OK;
+    }
+
+    int fill_args(StepArgs<R>& a) {
+        int rc = sync_aux();
+        if (rc) return rc;
         a.flag = flags();
         a.wmask = static_cast<const uint32_t*>(wmask.p);
-        a.Fx = rp(Fx);
-        a.Fy = rp(Fy);
-        a.Fz = rp(Fz);
-        a.rho = rp(rho);
-        a.ux = rp(ux);
-        a.uy = rp(uy);
-        a.uz = rp(uz);
-        a.cfw = rp(cfw);
-        a.uref = rp(uref);
         a.fscale = fscale;
         // FP32 solvers see tau0 and Cs^2 rounded to float, like the float kernel arguments of
         // lbm3d.py:403; the FP64 solver keeps them in double.
@@ -463,23 +493,16 @@ struct Solver final : wfs_solver {
         a.blk2tile = static_cast<const int*>(blk2tile_d.p);
         a.tmap = static_cast<const int*>(tmap_d.p);
         a.ntx = ntx;
-        a.bedwall = static_cast<const uint8_t*>(bedwall.p);
-        a.colcount = static_cast<const uint8_t*>(colcount.p);
         set_colu(a);
+        return WFS_OK;
     }
 
     // The column velocity this step reads (written by the previous step) and
-    // the one it writes; the pair alternates every step.
+    // the one it writes: the StepAux of this step's parity; the pair
+    // alternates every step.
     void set_colu(StepArgs<R>& a) {
-        if (!colu_on) {
-            a.colu_in_x = a.colu_in_y = nullptr;
-            a.colu_out_x = a.colu_out_y = nullptr;
-            return;
-        }
-        a.colu_in_x = rp(colu[colu_par][0]);
-        a.colu_in_y = rp(colu[colu_par][1]);
-        a.colu_out_x = rp(colu[colu_par ^ 1][0]);
-        a.colu_out_y = rp(colu[colu_par ^ 1][1]);
+        a.colu_on = colu_on ? 1 : 0;
+        a.aux = static_cast<const StepAux<R>*>(aux_d.p) + (colu_on ? colu_par : 0);
     }
 
     // A-B layout <-> state S of the in-place buffer (the same exchange both ways).
@@ -590,7 +613,8 @@ struct Solver final : wfs_solver {
 
     int step(int n) override {
         StepArgs<R> a;
-        fill_args(a);
+        int rc0 = fill_args(a);
+        if (rc0) return rc0;
         const long long groups = (long long)((a.rows + 7) / 8) * 8 * a.tiles;
         const dim3 grid((unsigned)groups);
         for (int k = 0; k < n; ++k) {
@@ -939,10 +963,10 @@ struct Solver final : wfs_solver {
 
     int forces_ab(double* out) {
         StepArgs<R> a;
-        fill_args(a);
+        int rc = fill_args(a);
+        if (rc) return rc;
         const long long blocks = (Np + 255) / 256;
         DevBuf part, sum;
-        int rc;
         if ((rc = alloc_buf(part, (size_t)(blocks + kSumChunks) * 9 * sizeof(double))) ||
             (rc = alloc_buf(sum, 9 * sizeof(double))))
             return rc;
@@ -1076,7 +1100,8 @@ struct Batch final : BatchBase {
             WFS_HIP(hipStreamSynchronize(c->stream));
             cases.push_back(c);
             CaseArgs<Codec> ca{};
-            c->fill_args(ca.a);
+            int rc = c->fill_args(ca.a);
+            if (rc) return rc;
             ca.f[0] = c->fp(c->cur);
             ca.f[1] = c->fp(c->cur ^ 1);
             ca.fscale = nullptr;   // set below, once the schedule array exists
