@@ -152,7 +152,8 @@ int wfs_set_column_u(wfs_solver* h, int32_t per_step);
  * through an x-end row (x = 0 or nx - 1), which no step reads, reads as its
  * opposite. Called between wfs_create and wfs_upload, the second buffer is
  * never allocated. Both layouts and both closures; nx - 1 not a multiple of
- * 64; no batches. No reference counterpart: the reference streams between two
+ * 64; not for a case of a batch (WFS_EINVAL while a batch holds the solver).
+ * No reference counterpart: the reference streams between two
  * arrays (floodsim/lbm3d.py:50-60); this halves the population memory. */
 int wfs_set_inplace(wfs_solver* h, int32_t on);
 

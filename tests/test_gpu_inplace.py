@@ -295,6 +295,24 @@ def test_inplace_with_learned_closure(layout, storage):
         run.close()
 
 
+def test_inplace_refused_for_a_batch_case():
+    """A batch holds pointers to both population buffers of its cases, so a
+    case cannot switch to one buffer while the batch exists."""
+    print("case: two small seeded reaches in a batch, set_inplace on a case", flush=True)
+    from water_flow_simulation_amd.batch import BatchRun
+    from water_flow_simulation_amd.sweep import ensemble
+    reaches, _ = ensemble(2, seed=3, scale=0.2)
+    b = BatchRun(reaches)
+    b.advance(20)
+    b.synchronize()
+    with pytest.raises(ValueError, match="case of a batch"):
+        b.runs[0].lbm.set_inplace(True)
+    b.set_group(1)                    # new batches, same refusal
+    with pytest.raises(ValueError, match="case of a batch"):
+        b.runs[1].lbm.set_inplace(True)
+    b.close()
+
+
 def test_inplace_rejections():
     print("case: in-place streaming with the outlet row opening a wavefront", flush=True)
     nz, ny = 6, 8

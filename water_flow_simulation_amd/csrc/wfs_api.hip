@@ -153,6 +153,7 @@ struct wfs_solver {
     // aa_phase 0 = state S (an even number of in-place steps taken)
     bool inplace = false;
     int aa_phase = 0;
+    int batch_refs = 0;     // batches that hold this solver's buffers
     int bx = 128;   // step-kernel work-group width (64..256)
     // tile-sparse layout: only the 64-cell x-runs of a row that hold fluid are stored
     bool tiled = false;
@@ -527,6 +528,7 @@ struct Solver final : wfs_solver {
     // outlet row fused into the step (nx - 1 not a multiple of 64).
     int set_inplace(int on) override {
         if (!on == !inplace) return WFS_OK;
+        if (batch_refs) return fail(WFS_EINVAL, "wfs_set_inplace: the solver is a case of a batch");
         int rc;
         if (on) {
             if ((d.nx - 1) % 64 == 0)
@@ -1080,6 +1082,7 @@ struct Batch final : BatchBase {
     ~Batch() override {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
+        for (auto* c : cases) --c->batch_refs;
     }
 
     int init(wfs_solver* const* hs, int n) {
@@ -1099,6 +1102,7 @@ struct Batch final : BatchBase {
                 return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the constant-Cs closure");
             WFS_HIP(hipStreamSynchronize(c->stream));
             cases.push_back(c);
+            ++c->batch_refs;
             CaseArgs<Codec> ca{};
             int rc = c->fill_args(ca.a);
             if (rc) return rc;
