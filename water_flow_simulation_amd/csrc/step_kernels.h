@@ -220,15 +220,17 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uin
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
 }
 // soff: a uniform byte offset added to the lane offset (scalar register)
-template <typename T>
+// kAux: cache policy of the load (0 default, kPopLoadNT non-temporal)
+constexpr int kPopLoadNT = 2;
+template <typename T, int kAux = 0>
 __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff = 0) {
     if constexpr (sizeof(T) == 2) {
-        return __builtin_bit_cast(T, (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, off, soff, 0));
+        return __builtin_bit_cast(T, (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(r, off, soff, kAux));
     } else if constexpr (sizeof(T) == 4) {
-        return __builtin_bit_cast(T, (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+        return __builtin_bit_cast(T, (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, off, soff, kAux));
     } else {
         static_assert(sizeof(T) == 8, "2, 4 or 8 bytes");
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, kAux));
     }
 }
 // cache policy of population stores: non-temporal (nt) when WFS_NT_STORE
@@ -612,7 +614,8 @@ __device__ __forceinline__ void wall_function(const StepArgs<typename Codec::R>&
     }
 }
 
-template <class Codec, bool kSparse = false>
+// kLoadAux: cache policy of the population loads through the buffer resource.
+template <class Codec, bool kSparse = false, int kLoadAux = 0>
 __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs, const StepArgs<typename Codec::R>& a,
                                        int c, int x, int y, int z, uint8_t fl, uint32_t wm, typename Codec::R* f,
                                        const int* nb = nullptr, int lane = 0) {
@@ -647,7 +650,7 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
                 else
                     base = (uint32_t)i * (uint32_t)N - (uint32_t)kCX[i] - (uint32_t)px * kCY[i] -
                            (uint32_t)plane * kCZ[i];
-                f[i] = Codec::template dec<i>(buf_load<T>(rs, voff, base * ts), a.ctr.c[center_class(i)]);
+                f[i] = Codec::template dec<i>(buf_load<T, kLoadAux>(rs, voff, base * ts), a.ctr.c[center_class(i)]);
             });
         } else {
             // a uniform pointer per direction at its pull base (i N + ... >= 0)
@@ -727,7 +730,7 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
                         base = (uint32_t)lo1 * nn;
                     }
                 }
-                f[i] = Codec::template dec<i>(buf_load<T>(rs, ((uint32_t)s + adj) * ts, base * ts), e);
+                f[i] = Codec::template dec<i>(buf_load<T, kLoadAux>(rs, ((uint32_t)s + adj) * ts, base * ts), e);
             });
         } else {
             static_for<Q>([&](auto I) {
@@ -1286,7 +1289,9 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     if (active) {
         wm = (fl & FLAG_NEARWALL) ? at(a.wmask, c) : 0u;
         if constexpr (kStream == 0) {
-            gather<Codec, kSparse>(fs, a, c, x, row - a.ny * z, z, fl, wm, f, nb, lane);
+            // non-temporal population loads: faster here (FP32 0.79 -> 0.75 ms on
+            // 23.5 M cells), slower in the plain step (0.617 -> 0.637 ms)
+            gather<Codec, kSparse, kPopLoadNT>(fs, a, c, x, row - a.ny * z, z, fl, wm, f, nb, lane);
         } else {   // in place: fd is the one buffer
             aa_gather<Codec, kStream == 1, kSparse>(fd, a, c, x, z, wm, f, nb, lane);
             wall_function<Codec, kSparse>(a, c, x, row - a.ny * z, fl, wm, f);
