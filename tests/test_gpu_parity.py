@@ -56,10 +56,10 @@ def test_step_matches_oracle_bitwise_on_fixture(case):
     L.close()
 
 
-def _run_pair(reach, nsteps, storage="f32", precision="f32", block_x=0):
+def _run_pair(reach, nsteps, storage="f32", precision="f32", block_x=0, streaming="ab"):
     args = solver_inputs(reach)
     gpu = LBM3D(None, args["solid"], args["Fx"], args["Fy"], args["Fz"], args["uin"], args["tau0"], args["cs"],
-                args["cf_wall"], storage=storage, block_x=block_x)
+                args["cf_wall"], storage=storage, block_x=block_x, streaming=streaming)
     gpu.set_bedwall(reach.bedwall)
     orc = OracleLBM(args["solid"], args["Fx"], args["Fy"], args["Fz"], args["uin"], args["tau0"], args["cs"],
                     args["cf_wall"], precision=precision)
@@ -82,9 +82,11 @@ def _level_profile(reach, rho):
     return ex - ex[-1]
 
 
-def test_channel128_1000_steps_matches_oracle():
+@pytest.mark.parametrize("streaming", ["ab", "inplace"])
+def test_channel128_1000_steps_matches_oracle(streaming):
+    print(f"case: 128x32x16 channel, f32, 1000 steps, streaming {streaming}", flush=True)
     reach = channel128()
-    gpu, orc = _run_pair(reach, 1000)
+    gpu, orc = _run_pair(reach, 1000, streaming=streaming)
     rho, ux, uy, uz = gpu.macro()
     fl = ~reach.solid.ravel()
     vel = np.concatenate([ux[fl], uy[fl], uz[fl]])
