@@ -157,6 +157,21 @@ int wfs_set_column_u(wfs_solver* h, int32_t per_step);
  * arrays (floodsim/lbm3d.py:50-60); this halves the population memory. */
 int wfs_set_inplace(wfs_solver* h, int32_t on);
 
+/* Checkpoints: the raw solver state as one host buffer of wfs_state_bytes
+ * bytes. It holds the population buffer in use (device layout and storage
+ * format as they are, the in-place phase included), the macroscopic fields,
+ * uref, the six time-mean sums, the per-step column buffers when on, the step
+ * count, the sample count and the force scale. wfs_state_load restores it
+ * into a solver created and uploaded the same way (dimensions, storage,
+ * layout, streaming mode, column-speed mode; WFS_EINVAL otherwise), and the
+ * run then continues bit for bit. Inputs (geometry, forces, wall cf, inlet
+ * profile, closure) come from the solver's own set-up; particles are not
+ * part of the state. No reference counterpart: the reference never
+ * checkpoints a run (it caches results, web3d.py:187-206). */
+int wfs_state_bytes(wfs_solver* h, int64_t* bytes);
+int wfs_state_save(wfs_solver* h, void* out, int64_t bytes);
+int wfs_state_load(wfs_solver* h, const void* in, int64_t bytes);
+
 /* Particle tracer (Run3D.init_particles / move_particles / particles_xyz,
  * reach3d.py:500-541). wfs_particles_set uploads n positions (f64, x y z per
  * particle, metres from the domain corner) and the rows y + ny*z of the inlet

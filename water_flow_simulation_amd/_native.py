@@ -88,6 +88,9 @@ _SIGS = {
     "wfs_set_body_mask": ([_P, _P], ctypes.c_int),
     "wfs_set_column_u": ([_P, ctypes.c_int32], ctypes.c_int),
     "wfs_set_inplace": ([_P, ctypes.c_int32], ctypes.c_int),
+    "wfs_state_bytes": ([_P, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "wfs_state_save": ([_P, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "wfs_state_load": ([_P, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
     "wfs_particles_set": ([_P, ctypes.c_int64, _P, ctypes.c_int64, _P, ctypes.c_uint64], ctypes.c_int),
     "wfs_particles_move": ([_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P], ctypes.c_int),
     "wfs_particles_get": ([_P, _P], ctypes.c_int),

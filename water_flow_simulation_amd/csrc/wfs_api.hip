@@ -138,6 +138,9 @@ struct wfs_solver {
     virtual int set_body_mask(const uint8_t* body) = 0;
     virtual int set_column_u(int per_step) = 0;
     virtual int set_inplace(int on) = 0;
+    virtual int state_bytes(long long* out) = 0;
+    virtual int state_save(void* out, long long bytes) = 0;
+    virtual int state_load(const void* in, long long bytes) = 0;
     virtual int particles_set(long long n, const double* xyz, long long ninlet, const int32_t* inlet,
                               uint64_t seed) = 0;
     virtual int particles_move(double dt, double vel, double dx, long long* respawned) = 0;
@@ -723,6 +726,83 @@ struct Solver final : wfs_solver {
                            static_cast<const float*>(tmp.p), cmap(), Np, N, ctr);
         WFS_LAUNCH_CHECK();
         WFS_HIP(hipStreamSynchronize(stream));
+        return WFS_OK;
+    }
+
+    // ---- checkpoints: the raw solver state. The population buffer in use (its
+    // device layout and storage format as they are, in-place phase
+    // included), the macroscopic fields (the next step's wall function reads
+    // them), uref, the six time-mean sums, the column buffers when on, and
+    // the counters. Loaded into a solver built the same way (dimensions,
+    // storage, layout, streaming mode, column-speed mode), a run continues
+    // bit for bit.
+    static constexpr uint64_t kStateMagic = 0x3144335374534657ull;   // the bytes "WFStS3D1"
+    struct StateHead {
+        uint64_t magic;
+        int32_t version, storage, tiled, inplace, aa_phase, colu_on, colu_par, navg;
+        int64_t Np, nstep;
+        double fscale;
+    };
+    std::vector<std::pair<DevBuf*, size_t>> state_parts() {
+        std::vector<std::pair<DevBuf*, size_t>> v;
+        v.push_back({&f[cur], (size_t)Np * Q * sizeof(T)});
+        for (DevBuf* b : {&rho, &ux, &uy, &uz, &uref}) v.push_back({b, (size_t)Np * sizeof(R)});
+        for (auto& sb : sums) v.push_back({&sb, (size_t)Np * sizeof(R)});
+        if (colu_on)
+            for (auto& pr : colu)
+                for (auto& b : pr) v.push_back({&b, (size_t)Np * sizeof(R)});
+        return v;
+    }
+    int state_bytes(long long* out) override {
+        size_t n = sizeof(StateHead);
+        for (auto& pr : state_parts()) n += pr.second;
+        *out = (long long)n;
+        return WFS_OK;
+    }
+    int state_save(void* out, long long bytes) override {
+        if (!uploaded) return fail(WFS_EINVAL, "wfs_state_save: no geometry uploaded");
+        long long need;
+        state_bytes(&need);
+        if (bytes != need) return fail(WFS_EINVAL, "wfs_state_save: buffer size " + std::to_string(bytes) +
+                                                       " != wfs_state_bytes " + std::to_string(need));
+        StateHead h{kStateMagic, 1, d.storage, tiled ? 1 : 0, inplace ? 1 : 0, aa_phase, colu_on ? 1 : 0,
+                    colu_par, navg, Np, (int64_t)nstep, (double)fscale};
+        std::memcpy(out, &h, sizeof h);
+        WFS_HIP(hipStreamSynchronize(stream));
+        char* o = static_cast<char*>(out) + sizeof h;
+        for (auto& pr : state_parts()) {
+            WFS_HIP(hipMemcpy(o, pr.first->p, pr.second, hipMemcpyDeviceToHost));
+            o += pr.second;
+        }
+        return WFS_OK;
+    }
+    int state_load(const void* in, long long bytes) override {
+        if (!uploaded) return fail(WFS_EINVAL, "wfs_state_load: upload the geometry first");
+        StateHead h;
+        if (bytes < (long long)sizeof h) return fail(WFS_EINVAL, "wfs_state_load: not a solver state (too short)");
+        std::memcpy(&h, in, sizeof h);
+        if (h.magic != kStateMagic || h.version != 1) return fail(WFS_EINVAL, "wfs_state_load: not a solver state");
+        if (h.storage != d.storage || h.tiled != (tiled ? 1 : 0) || h.Np != Np)
+            return fail(WFS_EINVAL, "wfs_state_load: the state is of another storage format, layout or size");
+        if (h.inplace != (inplace ? 1 : 0))
+            return fail(WFS_EINVAL, "wfs_state_load: the state is of the other streaming mode");
+        if (h.colu_on != (colu_on ? 1 : 0))
+            return fail(WFS_EINVAL, "wfs_state_load: per-step column speed on in one and off in the other");
+        long long need;
+        state_bytes(&need);
+        if (bytes != need) return fail(WFS_EINVAL, "wfs_state_load: size " + std::to_string(bytes) + " != " +
+                                                       std::to_string(need));
+        WFS_HIP(hipStreamSynchronize(stream));
+        const char* p = static_cast<const char*>(in) + sizeof h;
+        for (auto& pr : state_parts()) {
+            WFS_HIP(hipMemcpy(pr.first->p, p, pr.second, hipMemcpyHostToDevice));
+            p += pr.second;
+        }
+        nstep = h.nstep;
+        navg = h.navg;
+        aa_phase = h.aa_phase;
+        colu_par = h.colu_par;
+        fscale = R(h.fscale);
         return WFS_OK;
     }
 
@@ -1848,6 +1928,26 @@ int wfs_set_inplace(wfs_solver* h, int32_t on) {
     if (!h) return fail(WFS_EINVAL, "wfs_set_inplace: null handle");
     DeviceGuard g(h->device);
     return h->set_inplace(on);
+}
+
+int wfs_state_bytes(wfs_solver* h, int64_t* bytes) {
+    if (!h || !bytes) return fail(WFS_EINVAL, "wfs_state_bytes: null argument");
+    long long n = 0;
+    int rc = h->state_bytes(&n);
+    *bytes = n;
+    return rc;
+}
+
+int wfs_state_save(wfs_solver* h, void* out, int64_t bytes) {
+    if (!h || !out) return fail(WFS_EINVAL, "wfs_state_save: null argument");
+    DeviceGuard g(h->device);
+    return h->state_save(out, bytes);
+}
+
+int wfs_state_load(wfs_solver* h, const void* in, int64_t bytes) {
+    if (!h || !in) return fail(WFS_EINVAL, "wfs_state_load: null argument");
+    DeviceGuard g(h->device);
+    return h->state_load(in, bytes);
 }
 
 int wfs_set_body_mask(wfs_solver* h, const uint8_t* body) {

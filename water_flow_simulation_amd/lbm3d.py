@@ -298,6 +298,24 @@ class LBM3D:
         nat.check(nat.lib().wfs_set_inplace(self.handle, int(bool(on))))
         self.inplace = bool(on)
 
+    # ------------------------------------------------------------ checkpoints
+    def state(self):
+        """The raw solver state (populations as stored, macroscopic fields,
+        uref, time-mean sums, column buffers, counters) as a uint8 array."""
+        n = ctypes.c_int64()
+        nat.check(nat.lib().wfs_state_bytes(self.handle, ctypes.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        nat.check(nat.lib().wfs_state_save(self.handle, nat.ptr(buf), n.value))
+        return buf
+
+    def load_state(self, buf):
+        """Restore a state() of a solver built the same way; the run then
+        continues bit for bit."""
+        buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        nat.check(nat.lib().wfs_state_load(self.handle, nat.ptr(buf), buf.size))
+        self.nstep = int(nat.lib().wfs_nstep(self.handle))
+        self._uref = None
+
     # ------------------------------------------------------------ particle tracer
     def particles_set(self, xyz, inlet_rows, seed=0):
         """Upload particle positions (n x 3, metres from the domain corner) and

@@ -643,6 +643,21 @@ class Run3D:
         k = RHO * self.r.dx ** 4 / self.r.dt ** 2
         return {name: v * k for name, v in f.items()}
 
+    def checkpoint(self, path):
+        """Save the run (solver state, simulated time, ramp and averaging
+        counters) to an .npz file; restore() into a Run3D of the same reach
+        and options continues it bit for bit. Particles are not saved."""
+        np.savez(path, state=self.lbm.state(), t=np.float64(self.t), ramp_done=np.bool_(self._ramp_done),
+                 navg=np.int64(self.navg), steps_wall=np.float64(self.steps_wall))
+
+    def restore(self, path):
+        with np.load(path) as d:
+            self.lbm.load_state(d["state"])
+            self.t = float(d["t"])
+            self._ramp_done = bool(d["ramp_done"])
+            self.navg = int(d["navg"])
+            self.steps_wall = float(d["steps_wall"])
+
     def close(self):
         self.lbm.close()
 
