@@ -222,6 +222,18 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uin
 // soff: a uniform byte offset added to the lane offset (scalar register)
 // kAux: cache policy of the load (0 default, kPopLoadNT non-temporal)
 constexpr int kPopLoadNT = 2;
+// A non-temporal load through a pointer.
+template <typename T>
+__device__ __forceinline__ T load_nt(const T* p) {
+    if constexpr (sizeof(T) == 2) {
+        return __builtin_bit_cast(T, __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p)));
+    } else if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_nontemporal_load(reinterpret_cast<const unsigned int*>(p)));
+    } else {
+        static_assert(sizeof(T) == 8, "2, 4 or 8 bytes");
+        return __builtin_bit_cast(T, __builtin_nontemporal_load(reinterpret_cast<const unsigned long long*>(p)));
+    }
+}
 template <typename T, int kAux = 0>
 __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff = 0) {
     if constexpr (sizeof(T) == 2) {
@@ -1024,8 +1036,8 @@ __device__ __forceinline__ void aa_gather(const typename Codec::T* f, const Step
             }
             const bool mine = xout || wall;
             out[i] = Codec::template ld<i>(elem(mine ? own : pn, (uint32_t)(mine ? c : s)), e);
-        } else {
-            out[i] = Codec::template ld<i>(elem(own, (uint32_t)c), e);
+        } else {   // even step: own lines, non-temporal as in aa_gather32
+            out[i] = Codec::template dec<i>(load_nt(elem(own, (uint32_t)c)), e);
         }
     });
 }
@@ -1113,7 +1125,11 @@ __device__ __forceinline__ void aa_gather32(const typename Codec::T* f, const St
                 const int off = kCX[i] + px * kCY[i] + plane * kCZ[i];
                 out[i] = Codec::template dec<i>(buf_load<T>(rs, vc, ((uint32_t)kOPP[i] * nn - (uint32_t)off) * ts), e);
             } else {
-                out[i] = Codec::template dec<i>(buf_load<T>(rs, vc, (uint32_t)i * nn * ts), e);
+                // even step: the cell's own lines, read once and written back;
+                // non-temporal loads measured 4 % (FP32) and 6 % (FP16) faster
+                // here and 5 % on the tile-sparse meander; on the odd step's
+                // neighbour loads they cost 13 %
+                out[i] = Codec::template dec<i>(buf_load<T, kPopLoadNT>(rs, vc, (uint32_t)i * nn * ts), e);
             }
         });
         return;
