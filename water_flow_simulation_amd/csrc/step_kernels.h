@@ -1343,17 +1343,22 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     }
 }
 
-// Resident waves per SIMD asked of the register allocator for the step. BF16
-// asks for 7 and gets 8 (0.349 -> 0.340 ms on 23.5 M cells). FP16 would get 8
-// too, but with more scalar spills: 0.349 -> 0.347 ms, and 0.443 -> 0.469 ms
-// with the per-step column speed; FP32 gets 6 and does not change.
-template <class Codec>
-constexpr int kStepWaves =
-    (std::is_same<Codec, CodecBF16Dev>::value && WFS_STEP_WAVES_PER_EU < 7) ? 7 : WFS_STEP_WAVES_PER_EU;
+// Resident waves per SIMD asked of the register allocator for the step. The
+// 16-bit formats ask for 7 and get 8 (BF16 0.349 -> 0.340 ms on 23.5 M cells);
+// FP32 gets 6 and does not change when asked for 7. The per-step column
+// speed is a separate instance (kColu): at 8 waves its FP16 step lost 6 %.
+template <class Codec, bool kColu>
+constexpr int kStepWaves = (sizeof(typename Codec::T) == 2 && !(kColu && std::is_same<Codec, CodecF16Dev>::value) &&
+                            WFS_STEP_WAVES_PER_EU < 7)
+                               ? 7
+                               : WFS_STEP_WAVES_PER_EU;
 
-template <class Codec>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStepWaves<Codec>)))
+// kColu: the per-step column speed (StepArgs::colu_on); without it the
+// column code drops out of the instance.
+template <class Codec, bool kColu>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kStepWaves<Codec, kColu>)))
 k_step(const typename Codec::T* __restrict__ fs, typename Codec::T* __restrict__ fd, StepArgs<typename Codec::R> a) {
+    if constexpr (!kColu) a.colu_on = 0;
     int row, tile;
     if (!tile_of(blockIdx.x, a.tiles, a.rows, row, tile)) return;   // whole work-group
     step_tile<Codec>(fs, fd, a, row, tile);

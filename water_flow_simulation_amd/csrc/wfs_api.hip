@@ -613,7 +613,10 @@ struct Solver final : wfs_solver {
                 return;
             }
         }
-        hipLaunchKernelGGL(k_step<Codec>, grid, dim3(bx), 0, stream, fp(cur), fp(cur ^ 1), a);
+        if (a.colu_on)
+            hipLaunchKernelGGL((k_step<Codec, true>), grid, dim3(bx), 0, stream, fp(cur), fp(cur ^ 1), a);
+        else
+            hipLaunchKernelGGL((k_step<Codec, false>), grid, dim3(bx), 0, stream, fp(cur), fp(cur ^ 1), a);
     }
 
     int step(int n) override {
