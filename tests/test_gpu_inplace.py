@@ -164,6 +164,37 @@ def test_inplace_on_raw_fixture_state(case):
     aa.close()
 
 
+@pytest.mark.parametrize("nx,ny,nz,storage", [(3, 2, 2, "f32"), (4, 3, 2, "f16dev"), (5, 2, 3, "f64"), (67, 3, 3, "f32")])
+def test_inplace_on_tiny_grids(nx, ny, nz, storage):
+    """The x-end rows side by side (nx = 3), a lid over one layer of fluid,
+    a solid cell, random non-equilibrium populations everywhere; nx = 67
+    puts the outlet row in the second wavefront."""
+    print(f"case: {nx}x{ny}x{nz} random state, {storage}, in-place vs two buffers", flush=True)
+    rng = np.random.default_rng(nx * 100 + ny * 10 + nz)
+    solid = np.zeros((nz, ny, nx), bool)
+    solid[0, 0, nx // 2] = True
+    sh = solid.shape
+    F = [(1e-4 * rng.standard_normal(sh)).astype(np.float32) for _ in range(3)]
+    cf = np.where(solid, 0, 2e-3 * rng.random(sh)).astype(np.float32)
+    uin = np.full((nz, ny), 0.02, np.float32)
+    f0 = (np.tile(np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, np.float32)[:, None], (1, solid.size))
+          * (1 + 0.05 * rng.standard_normal((19, solid.size)))).astype(np.float32)
+    runs = []
+    for streaming in ("ab", "inplace"):
+        g = LBM3D(None, solid, *F, uin, 0.52, cf_wall=cf, storage=storage, streaming=streaming)
+        g.set_populations(f0)
+        runs.append(g)
+    fluid = ~solid.ravel()
+    kept = _kept(sh) & np.tile(fluid, 19)
+    for n in (1, 3, 4):
+        for g in runs:
+            g.step(n)
+        _same_macros(runs[0], runs[1], fluid)
+    assert np.array_equal(runs[0].populations(np.float64)[kept], runs[1].populations(np.float64)[kept])
+    for g in runs:
+        g.close()
+
+
 def test_inplace_with_per_step_column_u():
     print("case: 128x32x16 channel, f16dev, per-step column speed, in-place vs two buffers", flush=True)
     reach = channel128()
