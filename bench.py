@@ -13,10 +13,12 @@ Prints one JSON line: MLUPS (all cells, the reference's convention), the
 step kernel's HBM roofline fraction from HIP events around every launch on
 the solver's stream, and a CPU baseline (the OpenMP oracle in its
 Numba-like precision, on a bounded sample of the same reach). Beside it:
-the FP32-fast, 16-bit and learned-closure variants on the same reach, the
-batched sweep against the same cases run one after another, and the
-meandering 1 km reach (1.56 G-cell box, 0.67 G fluid cells) on the
-tile-sparse layout, FP16 storage, with and without the learned closure.
+the FP32-fast, 16-bit, learned-closure, per-step column speed and in-place
+(one population buffer) variants on the same reach, the batched sweep
+against the same cases run one after another, and the meandering 1 km reach
+(1.56 G-cell box, 0.67 G fluid cells) on the tile-sparse layout, FP16
+storage, with and without the learned closure (``--meander`` also takes
+``+inplace`` variants).
 """
 from __future__ import annotations
 
