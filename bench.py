@@ -135,17 +135,20 @@ def time_batched(cases, steps, seed=0, scale=0.5):
     from water_flow_simulation_amd import sweep
     reaches, _ = sweep.ensemble(cases, seed, scale=scale)
     cells = sum(r.cells for r in reaches)
-    runs, _ = sweep.run_sequential(reaches, 40)          # warm-up (allocation, first launches)
+    runs, _, _ = sweep.run_sequential(reaches, 40)       # warm-up (allocation, first launches)
     for run in runs:
         run.close()
-    runs, seq_s = sweep.run_sequential(reaches, steps)
+    runs, seq_all_s, seq_s = sweep.run_sequential(reaches, steps)
     for run in runs:
         run.close()
-    b, _, bat_s = sweep.run_batched(reaches, steps)
+    b, bat_all_s, bat_s = sweep.run_batched(reaches, steps)
     b.close()
+    # speedup: the stepping alone on both sides; speedup_with_setup: the
+    # whole sweep, the solvers' set-up included on both sides
     return {"cases": cases, "seed": seed, "cells_total": cells, "steps": steps,
             "sequential_mlups": round(steps * cells / seq_s / 1e6, 1),
             "batched_mlups": round(steps * cells / bat_s / 1e6, 1), "speedup": round(seq_s / bat_s, 3),
+            "speedup_with_setup": round(seq_all_s / bat_all_s, 3),
             "ensemble": f"seeded boat-test reaches at dx 0.2 m, lengths and widths x{scale}"}
 
 
