@@ -989,10 +989,13 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
 // after an odd step); the results go back to the cell's own slots, giving S.
 // Two steps equal two A-B steps, bit for bit.
 
-// Either layout, link by link. Neighbour cells come from pull_source: the
-// cell at x - c for a pull, at x + c (the offset negated) for a push. A lane
-// selects between uniform slot pointers (f + k N) and adds its cell's 32-bit
-// byte offset (each direction is below 4 GiB).
+// Either layout. Neighbour cells come from pull_source: the cell at x - c
+// for a pull, at x + c (the offset negated) for a push. Interior cells of an
+// odd step (no wall link, away from the x-ends and the lid) address every
+// direction as a uniform pointer plus the cell's (dense) or lane's
+// (tile-sparse) offset; other cells go link by link, a lane selecting
+// between uniform slot pointers (f + k N) and adding its cell's 32-bit byte
+// offset (each direction is below 4 GiB).
 template <class Codec, bool kOdd, bool kSparse>
 __device__ __forceinline__ void aa_gather(const typename Codec::T* f, const StepArgs<typename Codec::R>& a, int c,
                                           int x, int z, uint32_t wm, typename Codec::R* out, const int* nb,
