@@ -261,7 +261,8 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off
 
 // Stores the post-collision populations of cell c (all lanes that call it).
 // pop32: one resource for the whole buffer, direction i at the uniform offset
-// i N (fewer scalar instructions); otherwise one resource per direction.
+// i N (fewer scalar instructions); otherwise the uniform pointer fd + i N and
+// the cell's 32-bit offset.
 template <class Codec>
 __device__ __forceinline__ void store_cell(typename Codec::T* __restrict__ fd, long long N, const Center& ctr,
                                            const typename Codec::R* f, int c, bool pop32) {
