@@ -128,10 +128,11 @@ def cpu_baseline(reach, target_s=15.0):
                 sample=f"{n} steps of the full {reach.nx}x{reach.ny}x{reach.nz} reach (all cells counted)")
 
 
-def time_batched(cases, steps, seed=0, scale=0.5):
-    """The boat-test sweep: the same seeded ensemble run one case after
-    another and as one batch (one step launch over all cases, device-side
-    cadence, HIP-graph replay)."""
+def time_batched(cases, steps, seed=0, scale=1.0):
+    """The boat-test sweep: the same seeded ensemble (at scale 1 the
+    32-reach boat-test ensemble of widths 20-60 m, depths 1.5-5 m, lengths
+    100-200 m at dx 0.2 m) run one case after another and as one batch (one
+    step launch over all cases, device-side cadence, HIP-graph replay)."""
     from water_flow_simulation_amd import sweep
     reaches, _ = sweep.ensemble(cases, seed, scale=scale)
     cells = sum(r.cells for r in reaches)
@@ -196,8 +197,9 @@ def main():
                                           "f16dev+inplace",
                     help="extra variants timed (comma list; storage format, '+mlp' for the learned closure, "
                          "'+colu' for the per-step column speed, '+inplace' for one population buffer)")
-    ap.add_argument("--batch-cases", type=int, default=8, help="cases of the batched-sweep line (0: skip)")
+    ap.add_argument("--batch-cases", type=int, default=32, help="cases of the batched-sweep line (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=400)
+    ap.add_argument("--batch-scale", type=float, default=1.0, help="length and width factor of the ensemble")
     ap.add_argument("--meander", default="f16dev,f16dev+mlp",
                     help="variants timed on the meandering reach, tile-sparse layout ('' to skip)")
     ap.add_argument("--meander-steps", type=int, default=200)
@@ -255,7 +257,7 @@ def main():
         out["variants"] = variants
     if a.batch_cases > 0:
         log(f"timing the batched sweep ({a.batch_cases} cases)")
-        out["batched"] = time_batched(a.batch_cases, a.batch_steps)
+        out["batched"] = time_batched(a.batch_cases, a.batch_steps, scale=a.batch_scale)
     if a.meander:
         out["meander"] = time_meander([v for v in a.meander.split(",") if v], a.meander_steps, a.meander_scale)
     if not a.no_cpu_baseline:

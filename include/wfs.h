@@ -163,8 +163,11 @@ int wfs_set_inplace(wfs_solver* h, int32_t on);
  * uref, the six time-mean sums, the per-step column buffers when on, the step
  * count, the sample count and the force scale. wfs_state_load restores it
  * into a solver created and uploaded the same way (dimensions, storage,
- * layout, streaming mode, column-speed mode; WFS_EINVAL otherwise), and the
- * run then continues bit for bit. Inputs (geometry, forces, wall cf, inlet
+ * layout, streaming mode, column-speed mode, and the same geometry: the
+ * state carries a fingerprint of the solid mask and the wall coefficients
+ * given to wfs_upload; WFS_EINVAL otherwise), and the run then continues
+ * bit for bit. wfs_state_load, like wfs_step, wfs_set_macro and
+ * wfs_set_populations, is refused while a batch holds the solver. Inputs (geometry, forces, wall cf, inlet
  * profile, closure) come from the solver's own set-up; particles are not
  * part of the state. No reference counterpart: the reference never
  * checkpoints a run (it caches results, web3d.py:187-206). */
@@ -249,6 +252,7 @@ int wfs_total(wfs_solver* h, int32_t field, double* out);
 int wfs_set_body_mask(wfs_solver* h, const uint8_t* body);
 int wfs_forces(wfs_solver* h, double* out);
 
+
 /* Watchdog: WFS_ENONFINITE if any fluid cell's rho or u is NaN/Inf. */
 int wfs_check_finite(wfs_solver* h);
 
@@ -275,6 +279,17 @@ int wfs_batch_destroy(wfs_batch* b);
 /* Case k: time step dt (s), ramp length t_ramp (s; <= 0: no ramp), start of
  * the averaging window t_avg (s), and the case's current time t0 (s). */
 int wfs_batch_set_schedule(wfs_batch* b, int32_t k, double dt, double t_ramp, double t_avg, double t0);
+/* Step budget of case k, counted from the next wfs_batch_run (steps < 0: no
+ * limit, the default). A case takes steps until its budget is spent and then
+ * sits out the rest of the batch's launches: inside a chunk it stops after
+ * its last step with the cadence of a sequential chunk of that length (the
+ * time, the averaging sample and the fields of a run of exactly that many
+ * steps), so cases with different end times (floodsim/reach3d.py:455-467,
+ * run_to_end of each case of floodsim/boat_tests.py:97-103) share one
+ * batched run. wfs_batch_budget reads the budget left as of the last
+ * synchronize. */
+int wfs_batch_set_budget(wfs_batch* b, int32_t k, int64_t steps);
+int wfs_batch_budget(wfs_batch* b, int32_t k, int64_t* steps_left);
 int wfs_batch_run(wfs_batch* b, int32_t nsteps, int32_t use_graph);
 int wfs_batch_synchronize(wfs_batch* b);
 /* Case k's time and sample count as of the last synchronize. */

@@ -91,3 +91,78 @@ def test_block_width_switch_mid_run_and_autotuner(tmp_path, monkeypatch):
     batch = BatchRun(reaches, group="auto")
     assert batch.group in (1, 2, 3) and len(batch.tuning) >= 2
     batch.close()
+
+
+def test_batched_case_refuses_direct_stepping_and_state_changes():
+    """While a batch holds a solver, stepping it, loading a state or
+    replacing its populations or macroscopic fields directly is refused (the
+    batch's next run would overwrite it); once the batch is gone it works."""
+    print("case: batch of 2 reaches, direct calls on a held case", flush=True)
+    reaches, _ = ensemble(2, seed=3, scale=0.2)
+    b = BatchRun(reaches)
+    b.advance(20)
+    L = b.runs[0].lbm
+    st = L.state()
+    rho, ux, uy, uz = L.macro()
+    for call in (lambda: L.step(1), lambda: L.load_state(st), lambda: L.set_populations(L.populations()),
+                 lambda: L.set_macro(rho, ux, uy, uz)):
+        with pytest.raises(ValueError, match="case of a batch"):
+            call()
+    assert L.nstep == 20
+    b._destroy()
+    L.step(1)
+    assert L.nstep == 21
+    b.close()
+
+
+def _nx65_reach():
+    """A reach whose outlet row opens a wavefront ((nx - 1) % 64 == 0), so the
+    batch runs its outlet row as a separate pass."""
+    from water_flow_simulation_amd.geometry import Section
+    sec = Section(B=6.0, Hb=2.5, z=1.0, Bf=0.0, n=0.03)
+    fleet = R.Fleet(R.boat_type("longtail"), rows=1, per_row=1, x_first=4.0)
+    r = R.build_reach(sec, 2.0, 9.0, 2.2e-4, fleet, R.ReachConfig(length=16.25, dx=0.25))
+    assert (r.nx - 1) % 64 == 0
+    return r
+
+
+def test_batch_budgets_match_sequential_bitwise():
+    """Cases with uneven step budgets (some ending inside a 20-step chunk,
+    one with none) share launches; each ends exactly like a sequential run
+    of its own length, and after the budgets are lifted all continue."""
+    print("case: batch of 6 reaches with budgets 60/35/100/0/47/81, then 40 more steps", flush=True)
+    reaches = _cases() + [_nx65_reach()]
+    budgets = [60, 35, 100, 0, 47, 81]
+    seq = []
+    for r, n in zip(reaches, budgets):
+        run = R.Run3D(r)
+        run.t_avg = 0.0
+        if n:
+            run.advance(n)
+        seq.append(run)
+    b = BatchRun(reaches)
+    for run in b.runs:
+        run.t_avg = 0.0
+    b.set_group(len(reaches))          # pushes the schedules again
+    b.set_budgets(budgets)
+    b.advance(100)
+    assert b.budgets_left() == [0] * len(reaches)
+    b.set_budgets([None] * len(reaches))
+    b.advance(40)
+    for run in seq:
+        run.advance(40)
+    for k, (rs, rb) in enumerate(zip(seq, b.runs)):
+        s_sums, s_navg, s_mac, s_pop = _state(rs)
+        b_sums, b_navg, b_mac, b_pop = _state(rb)
+        assert rb.lbm.nstep == rs.lbm.nstep == budgets[k] + 40, k
+        assert s_navg == b_navg and rs.t == rb.t, k
+        fl = ~reaches[k].solid.ravel()
+        for name in s_sums:
+            assert np.array_equal(s_sums[name][fl], b_sums[name][fl]), (k, name)
+        for a, c in zip(s_mac, b_mac):
+            assert np.array_equal(a[fl], c[fl]), k
+        m19 = np.tile(fl, 19)
+        assert np.array_equal(s_pop[m19], b_pop[m19]), k
+    for run in seq:
+        run.close()
+    b.close()

@@ -1,0 +1,168 @@
+"""The solver object driven only through the surface the reference's run
+driver uses (floodsim/reach3d.py:381-453): the constructor with a numpy
+backend, ``xp``, in-place scaling and reassignment of ``Fx/Fy/Fz``,
+reassignment of ``uref`` computed on the host from ``ux/uy``, reads of
+``ux/uy/uz/rho`` after every chunk, ``step(m)`` and ``be.sync()``. The
+result must equal the oracle's restatement of the same run (OracleRun) bit
+for bit, and one chunk's field reads must cost one device fetch."""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleRun
+from tests.helpers import channel128
+from water_flow_simulation_amd.lbm3d import LBM3D
+
+pytestmark = pytest.mark.gpu
+
+
+class NumpyBackend:
+    """The backend object the reference passes around (name, xp, sync)."""
+    name = "numpy"
+    xp = np
+
+    def sync(self):
+        pass
+
+
+class CallerPatternRun:
+    """A run driven the way the reference's Run3D drives its solver: every
+    operation below is one the reference performs on its ``LBM3D`` (the
+    order of reach3d.py:385-407 at set-up and 424-453 per call)."""
+
+    def __init__(self, reach, be, sched):
+        self.r, self.be = reach, be
+        nu = max(1e-6 * reach.dt / reach.dx ** 2, 2e-4)
+        self.L = LBM3D(be, reach.solid, reach.Fx, reach.Fy, reach.Fz, (reach.uin_phys / reach.vel).astype(np.float32),
+                       tau0=0.5 + 3 * nu, cs=reach.cfg.cs, cf_wall=reach.cf)
+        xp = self.L.xp
+        fluid = ~reach.solid
+        self.mask = xp.asarray(fluid.astype(np.float32))
+        self.cells_per_col = xp.asarray(np.maximum(fluid.sum(0), 1).astype(np.float32))
+        self.bw = xp.asarray(reach.bedwall)
+        self.refresh_uref()
+        self.acc = {k: xp.zeros(reach.cells, np.float32) for k in ("ux", "uy", "uz", "rho", "sp", "uxx")}
+        self.samples = 0
+        self.t = 0.0
+        self.t_avg, self.t_ramp = sched["t_avg"], sched["t_ramp"]
+        self.F_full = (self.L.Fx.copy(), self.L.Fy.copy(), self.L.Fz.copy())
+        self.ramping = self.t_ramp > 0
+        if self.ramping:
+            self.L.Fx *= 0          # augmented assignment: the setter sees the scaled array
+            self.L.Fy *= 0
+            self.L.Fz *= 0
+
+    def refresh_uref(self):
+        r = self.r
+        ux = self.L.ux.reshape(r.nz, r.ny, r.nx)
+        uy = self.L.uy.reshape(r.nz, r.ny, r.nx)
+        colx = (ux * self.mask).sum(0)
+        coly = (uy * self.mask).sum(0)
+        speed = self.L.xp.sqrt(colx ** 2 + coly ** 2) / self.cells_per_col
+        self.L.uref = self.L.xp.where(self.bw, speed[None], 0).astype(np.float32).ravel()
+
+    def advance(self, n):
+        L = self.L
+        done = 0
+        while done < n:
+            m = min(20, n - done)
+            if self.ramping:
+                s = min(self.t / self.t_ramp, 1.0)
+                s = np.float32(s * s * (3 - 2 * s))
+                L.Fx = self.F_full[0] * s
+                L.Fy = self.F_full[1] * s
+                L.Fz = self.F_full[2] * s
+                self.ramping = s < 1.0
+            self.refresh_uref()
+            L.step(m)
+            done += m
+            self.t += m * self.r.dt
+            if self.t >= self.t_avg:
+                a = self.acc
+                a["ux"] += L.ux
+                a["uy"] += L.uy
+                a["uz"] += L.uz
+                a["rho"] += L.rho
+                a["sp"] += L.xp.sqrt(L.ux * L.ux + L.uy * L.uy + L.uz * L.uz)
+                a["uxx"] += L.ux * L.ux
+                self.samples += 1
+        self.be.sync()
+
+
+def _schedule(r):
+    # the ramp spans the first 3.5 chunks, the averaging window opens after 2
+    return dict(t_ramp=70 * r.dt, t_avg=40 * r.dt)
+
+
+def test_reference_caller_pattern_matches_oracle_run_bitwise():
+    print("case: 128x32x16 channel with boat, reference caller pattern, 40+30+30+60 steps", flush=True)
+    r = channel128()
+    sched = _schedule(r)
+    run = CallerPatternRun(r, NumpyBackend(), sched)
+    t_total = r.cfg.run_flows * r.cfg.length / max(r.V, 0.05)
+    orc = OracleRun(dict(solid=r.solid, Fx=r.Fx, Fy=r.Fy, Fz=r.Fz, uin_lb=(r.uin_phys / r.vel).astype(np.float32),
+                         tau0=0.5 + 3 * max(1e-6 * r.dt / r.dx ** 2, 2e-4), cs=r.cfg.cs, cf=r.cf, bedwall=r.bedwall,
+                         dt=r.dt, t_total=t_total, **sched), precision="f32")
+    for n in (40, 30, 30, 60):      # chunks of 20, and 20 + 10 as the reference splits them
+        run.advance(n)
+        orc.advance(n)
+    fl = ~r.solid.ravel()
+    assert run.samples == orc.navg > 0 and run.t == orc.t
+    assert run.L.nstep == orc.lbm.nstep == 160
+    for k in run.acc:
+        assert np.array_equal(run.acc[k][fl], orc.sum[k][fl]), k
+    for name in ("rho", "ux", "uy", "uz", "uref"):
+        assert np.array_equal(getattr(run.L, name)[fl], getattr(orc.lbm, name)[fl]), name
+    run.L.close()
+
+
+def test_field_reads_share_one_fetch_per_step(monkeypatch):
+    print("case: 128x32x16 channel, field reads between steps", flush=True)
+    r = channel128()
+    L = CallerPatternRun(r, NumpyBackend(), _schedule(r)).L
+    calls = []
+    real = L.macro
+    monkeypatch.setattr(L, "macro", lambda *a, **k: calls.append(1) or real(*a, **k))
+    L.step(5)
+    a = (L.ux, L.uy, L.uz, L.rho, L.ux * L.ux)
+    assert len(calls) == 1
+    assert not a[0].flags.writeable
+    with pytest.raises(ValueError):
+        a[0][0] = 1.0                  # the shared copy is read-only
+    L.step(1)
+    b = L.ux
+    assert len(calls) == 2 and not np.array_equal(a[0], b)
+    L.set_macro(L.rho, L.ux, L.uy, L.uz)   # a write invalidates the copy
+    _ = L.rho
+    assert len(calls) == 3
+    L.close()
+
+
+def test_uref_assigned_then_state_loaded_then_step():
+    """A uref assigned before load_state is dropped with the restore (the
+    state carries its own uref), and the next step runs."""
+    print("case: 128x32x16 channel, uref assignment then restore then step", flush=True)
+    r = channel128()
+    a = CallerPatternRun(r, NumpyBackend(), _schedule(r))
+    a.advance(20)
+    st = a.L.state()
+    b = CallerPatternRun(r, NumpyBackend(), _schedule(r))
+    b.L.uref = np.ones(r.cells, np.float32)
+    b.L.load_state(st)
+    b.L.step(3)
+    a.L.step(3)
+    fl = ~r.solid.ravel()
+    assert np.array_equal(a.L.ux[fl], b.L.ux[fl])
+    a.L.close()
+    b.L.close()
+
+
+def test_solver_holds_copies_of_the_callers_arrays():
+    """Scaling L.Fx in place (as the reference's driver does before its ramp)
+    leaves the caller's own force arrays untouched."""
+    print("case: 128x32x16 channel, in-place scaling of the force attribute", flush=True)
+    r = channel128()
+    before = r.Fx.copy()
+    L = CallerPatternRun(r, NumpyBackend(), _schedule(r)).L
+    assert not np.any(L.Fx) and np.any(r.Fx)
+    assert np.array_equal(r.Fx, before)
+    L.close()

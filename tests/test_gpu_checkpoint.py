@@ -82,3 +82,35 @@ def test_checkpoint_refuses_another_solver(tmp_path):
     with pytest.raises(ValueError, match="size"):
         a.lbm.load_state(st[:-8])
     a.close()
+
+
+def test_checkpoint_refuses_another_geometry_of_the_same_size():
+    """Two reaches with the same stored-cell count but a different lattice
+    shape or solid mask: the state of one is refused by the other."""
+    print("case: 128x32x16 channel vs 128x16x32 box and a shifted hull", flush=True)
+    from water_flow_simulation_amd.lbm3d import LBM3D
+    def solver(solid):
+        nz, ny, nx = solid.shape
+        z = np.zeros(solid.shape, np.float32)
+        return LBM3D(None, solid, z, z, z, np.full((nz, ny), 0.05, np.float32), 0.5006)
+    s1 = np.zeros((16, 32, 128), bool)
+    s1[:, 0] = s1[:, -1] = True
+    s2 = np.zeros((32, 16, 128), bool)
+    s2[:, 0] = s2[:, -1] = True
+    s3 = s1.copy()
+    s3[8:, 14:18, 40:60] = True
+    a = solver(s1)
+    a.step(4)
+    st = a.state()
+    for other in (s2, s3):
+        b = solver(other)
+        with pytest.raises(ValueError, match="another geometry"):
+            b.load_state(st)
+        b.close()
+    c = solver(s1.copy())
+    c.load_state(st)
+    c.step(2)
+    a.step(2)
+    assert np.array_equal(a.ux, c.ux)
+    for L in (a, c):
+        L.close()

@@ -77,6 +77,8 @@ _SIGS = {
     "wfs_batch_destroy": ([_P], ctypes.c_int),
     "wfs_batch_set_schedule": ([_P, _I32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double],
                                ctypes.c_int),
+    "wfs_batch_set_budget": ([_P, _I32, ctypes.c_int64], ctypes.c_int),
+    "wfs_batch_budget": ([_P, _I32, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "wfs_batch_run": ([_P, _I32, _I32], ctypes.c_int),
     "wfs_batch_synchronize": ([_P], ctypes.c_int),
     "wfs_batch_time": ([_P, _I32, _P, _P], ctypes.c_int),

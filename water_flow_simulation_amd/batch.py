@@ -9,8 +9,10 @@ graph. With ``group`` cases per batch the cases are split into several
 batches, each on its own stream, which the GPU runs side by side; the
 autotuner picks ``group`` and the work-group width. Results are read per case
 with the ``Run3D`` readouts (``field``, ``profiles``). This replaces the
-one-case-after-another loop of ``run_test`` (floodsim/boat_tests.py:79-110)
-for cases that run the same number of steps.
+one-case-after-another loop of ``run_test`` (floodsim/boat_tests.py:79-110):
+``run_to_end`` gives every case a step budget equal to the steps its own
+``Run3D.run_to_end`` would take (each case stops at its own end time), and
+the library stops each case when its budget is spent while the others go on.
 """
 from __future__ import annotations
 
@@ -34,7 +36,8 @@ class BatchRun:
         self._hs = []
         self.group = 0
         self.steps_wall = 0.0
-        self.nstep = 0
+        self.nstep = 0          # batch steps run (each case takes at most this many)
+        self.updates = 0        # cell updates over all cases (all cells counted, as MLUPS counts them)
         if group == "auto":
             from .autotune import tune_group
             tune_group(self)
@@ -60,13 +63,69 @@ class BatchRun:
         self.group = k
 
     def advance(self, nsteps, use_graph=True):
-        """Advance every case by nsteps; returns when all batches are done."""
+        """Advance every case by nsteps (fewer for a case whose budget runs
+        out first); returns when all batches are done."""
         t0 = time.time()
+        before = [run.lbm.nstep for run in self.runs]
         for h, _, _ in self._hs:          # queued on each batch's own stream: the batches overlap
             nat.check(nat.lib().wfs_batch_run(h, int(nsteps), 1 if use_graph else 0), "wfs_batch_run")
         self.nstep += int(nsteps)
         self.synchronize()
         self.steps_wall += time.time() - t0
+        self.updates += sum((run.lbm.nstep - b) * run.r.cells for run, b in zip(self.runs, before))
+
+    def _each(self):
+        """(batch handle, index in that batch, index in self.runs) per case."""
+        for h, start, n in self._hs:
+            for j in range(n):
+                yield h, j, start + j
+
+    def set_budgets(self, steps):
+        """Step budget per case from the next advance (None: no limit)."""
+        if len(steps) != len(self.runs):
+            raise ValueError("set_budgets: one budget per case")
+        for h, j, k in self._each():
+            b = steps[k]
+            nat.check(nat.lib().wfs_batch_set_budget(h, j, -1 if b is None else int(b)), "wfs_batch_set_budget")
+
+    def budgets_left(self):
+        out = []
+        v = ctypes.c_int64(0)
+        for h, j, _ in self._each():
+            nat.check(nat.lib().wfs_batch_budget(h, j, ctypes.byref(v)))
+            out.append(None if v.value < 0 else int(v.value))
+        return out
+
+    def run_to_end(self, progress=None, cancel=None, chunk=400, use_graph=True):
+        """Every case runs to its own end time exactly as Run3D.run_to_end
+        would run it alone (floodsim/reach3d.py:455-467): the same number of
+        steps in the same 20-step chunks, so the results are bit-identical,
+        but all cases share each launch until their budgets are spent.
+        ``progress(f)`` gets the fraction of the longest case done after
+        every ``chunk`` steps; ``cancel()`` stops all cases there. Returns
+        the per-case step counts."""
+        from .reach3d import GPU_DUTY
+        budgets = [run.steps_to_end(chunk) for run in self.runs]
+        seg = self.runs[0].run_chunk(chunk) if self.runs else chunk
+        duty = min(max(GPU_DUTY, 0.05), 1.0)
+        self.set_budgets(budgets)
+        total = max(budgets, default=0)
+        done = 0
+        try:
+            while done < total:
+                t0 = time.time()
+                m = min(seg, total - done)
+                self.advance(m, use_graph=use_graph)
+                done += m
+                if duty < 1:
+                    time.sleep((time.time() - t0) * (1 / duty - 1))
+                if progress:
+                    progress(done / total)
+                if cancel and cancel():
+                    break
+        finally:
+            self.set_budgets([None] * len(self.runs))
+        return budgets
 
     def synchronize(self):
         L = nat.lib()
@@ -82,8 +141,8 @@ class BatchRun:
                 run._ramp_done = run.t_ramp <= 0 or run.t >= run.t_ramp
 
     def mlups(self):
-        cells = sum(run.r.cells for run in self.runs)
-        return self.nstep * cells / max(self.steps_wall, 1e-9) / 1e6
+        """Cell updates of all cases per second of batched stepping (millions)."""
+        return self.updates / max(self.steps_wall, 1e-9) / 1e6
 
     def _destroy(self):
         for h, _, _ in getattr(self, "_hs", []):

@@ -1,14 +1,21 @@
-"""The recommended boat tests and their sequential runner.
+"""The recommended boat tests and their runner.
 
 ``BOAT_TESTS`` lists the same six test definitions as the reference
 (floodsim/boat_tests.py:9-60): boat type, rows x boats per row, optional row
-spacing, water level ('low'/'mid'/'high') and per-boat overrides. ``run_test``
-runs every variant plus one no-boat base per water level and compares them
-(floodsim/boat_tests.py:73-113), one GPU solve after another.
+spacing, water level ('low'/'mid'/'high') and per-boat overrides.
+
+``run_test`` gives what the reference's runner gives (floodsim/boat_tests.py
+:73-113): every variant compared with a no-boat base at its water level, the
+profiles of both attached. It is organised differently. The whole test is
+planned first (``plan_test``: one reach per variant, one base per level the
+variants use), then every reach is simulated at once as one ``BatchRun`` in
+which each case stops at its own end time (``simulate``), and the readouts
+come last. ``batched=False`` simulates the same plan one case after another,
+the reference's order of work; both give the same numbers bit for bit.
 """
 from __future__ import annotations
 
-from dataclasses import replace
+from dataclasses import dataclass, field, replace
 
 from .reach3d import Fleet, ReachConfig, Run3D, boat_type, build_reach, compare, profiles
 
@@ -74,42 +81,93 @@ def variant_fleet(v, x_first=40.0, boat_edits=None):
     return Fleet(b, rows=v["rows"], per_row=v["per_row"], row_spacing=v.get("row_spacing", 30.0), x_first=x_first)
 
 
-def run_test(test, river, sec, levels, h_sel, cfg: ReachConfig, be, flow_fn, x_first=40.0,
-             progress=None, cancel=None, on_result=None, boat_edits=None, storage="f32"):
-    """Run every variant of a test and its no-boat base per level; returns
-    the comparison dicts with the profiles attached. ``flow_fn(h)`` gives
-    (Q, S) for a level; ``river`` is accepted for signature compatibility."""
-    base_cache = {}
-    results = []
-    n = len(test["variants"])
-    for k, v in enumerate(test["variants"]):
-        if cancel and cancel():
-            break
+@dataclass
+class TestPlan:
+    """The reaches of one boat test: ``cases[k]`` is (label, level, Reach,
+    index of its base in ``cases`` or None for a base itself)."""
+    cases: list = field(default_factory=list)
+
+    @property
+    def reaches(self) -> list:
+        return [c[2] for c in self.cases]
+
+    def variants(self):
+        return [(k, c) for k, c in enumerate(self.cases) if c[3] is not None]
+
+
+def plan_test(test, sec, levels, h_sel, cfg: ReachConfig, flow_fn, x_first=40.0, boat_edits=None) -> TestPlan:
+    """Build every reach the test needs. A variant's level is its 'level'
+    entry resolved by ``pick_level`` (else ``h_sel``); ``flow_fn(h)`` gives
+    (Q, S) once per level. The base of a level is built with the first
+    variant's fleet switched off, so it carries no hull, disk or force."""
+    plan = TestPlan()
+    flows, base_at = {}, {}
+    for v in test["variants"]:
         h = pick_level(levels, v["level"], sec.Hb) if "level" in v else h_sel
-        Q, S = flow_fn(h)
+        if h not in flows:
+            flows[h] = flow_fn(h)
+        Q, S = flows[h]
         fleet = variant_fleet(v, x_first, boat_edits)
+        if h not in base_at:
+            base_at[h] = len(plan.cases)
+            plan.cases.append((f"no boats, h = {h:g} m", h, build_reach(sec, h, Q, S, replace(fleet, enabled=False),
+                                                                        cfg), None))
+        plan.cases.append((v["label"], h, build_reach(sec, h, Q, S, fleet, cfg), base_at[h]))
+    return plan
 
-        def prog(f, k=k, part=0):
-            if progress:
-                progress((k + 0.5 * part + 0.5 * f) / n, v["label"])
 
-        if h not in base_cache:
-            rb = build_reach(sec, h, Q, S, replace(fleet, enabled=False), cfg)
-            run_b = Run3D(rb, be, storage=storage)
-            run_b.run_to_end(progress=lambda f: prog(f, part=0), cancel=cancel)
-            base_cache[h] = profiles(run_b)
-            run_b.close()
-        rr = build_reach(sec, h, Q, S, fleet, cfg)
-        run = Run3D(rr, be, storage=storage)
-        run.run_to_end(progress=lambda f: prog(f, part=1), cancel=cancel)
-        res = profiles(run)
-        c = compare(res, base_cache[h], rr, cfg)
-        c["label"] = v["label"]
-        c["profiles"] = res
-        c["base_profiles"] = base_cache[h]
-        c["notes"] = rr.notes
-        results.append(c)
-        if on_result:
-            on_result(c, run)
-        run.close()
-    return results
+def simulate(reaches, be=None, storage="f32", batched=True, progress=None, cancel=None, group=0):
+    """Run every reach to its own end time (Run3D.run_to_end); returns the
+    runs, open for readout. Batched, all reaches share each launch until
+    their step budgets are spent; otherwise one after another. ``progress``
+    gets the fraction done (of the longest case's steps when batched, of all
+    cell updates otherwise)."""
+    if batched:
+        from .batch import BatchRun
+        b = BatchRun(reaches, storage=storage, be=be, group=group)
+        b.run_to_end(progress=progress, cancel=cancel)
+        b._destroy()           # the cases stay open for readout
+        runs, b.runs = b.runs, []
+    else:
+        runs = [Run3D(r, be, storage=storage) for r in reaches]
+        cost = [run.steps_to_end() * run.r.cells for run in runs]
+        total = max(sum(cost), 1)
+        done = 0
+        for run, c in zip(runs, cost):
+            if cancel and cancel():
+                break
+            run.run_to_end(progress=(lambda f, d=done, c=c: progress((d + f * c) / total)) if progress else None,
+                           cancel=cancel)
+            done += c
+    return runs
+
+
+def run_test(test, river, sec, levels, h_sel, cfg: ReachConfig, be, flow_fn, x_first=40.0,
+             progress=None, cancel=None, on_result=None, boat_edits=None, storage="f32", batched=True, group=0):
+    """One boat test, with the reference's signature and result records: a
+    list with one dict per variant, ``compare`` of the variant against the
+    base at its level plus 'label', 'profiles', 'base_profiles' and 'notes'
+    (floodsim/boat_tests.py:73-113). ``progress(f, label)`` reports the
+    fraction done; after ``cancel()`` turns true the runs stop where they
+    are and the results are read from that state. ``on_result(result,
+    run)`` is called per variant while its run is still open. ``river`` is
+    accepted for signature compatibility."""
+    if cancel and cancel():
+        return []
+    plan = plan_test(test, sec, levels, h_sel, cfg, flow_fn, x_first, boat_edits)
+    tag = f"{len(plan.cases)} runs" + (" together" if batched else "")
+    prog = (lambda f: progress(f, tag)) if progress else None
+    runs = simulate(plan.reaches, be, storage, batched, prog, cancel, group)
+    try:
+        prof = [profiles(run) for run in runs]
+        results = []
+        for k, (label, h, reach, kb) in plan.variants():
+            c = compare(prof[k], prof[kb], reach, cfg)
+            c.update(label=label, profiles=prof[k], base_profiles=prof[kb], notes=reach.notes)
+            results.append(c)
+            if on_result:
+                on_result(c, runs[k])
+        return results
+    finally:
+        for run in runs:
+            run.close()

@@ -187,8 +187,9 @@ __global__ void k_forced(uint8_t* __restrict__ flag, const R* __restrict__ Fx, c
 // (floodsim/reach3d.py:412-418): one work-item per (x, y) column, sums over
 // z = 0..nz-1 in order.
 template <typename R>
-__global__ void k_uref(const R* __restrict__ ux, const R* __restrict__ uy, const uint8_t* __restrict__ flag,
-                       const uint8_t* __restrict__ bedwall, R* __restrict__ uref, CellMap m, int nz) {
+__device__ __forceinline__ void uref_column(const R* __restrict__ ux, const R* __restrict__ uy,
+                                            const uint8_t* __restrict__ flag, const uint8_t* __restrict__ bedwall,
+                                            R* __restrict__ uref, CellMap m, int nz) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= m.nx) return;
@@ -207,6 +208,12 @@ __global__ void k_uref(const R* __restrict__ ux, const R* __restrict__ uy, const
         const long long c = cell_of(m, x, y + (long long)m.ny * z);
         if (c >= 0) uref[c] = bedwall[c] ? U : R(0);
     }
+}
+
+template <typename R>
+__global__ void k_uref(const R* __restrict__ ux, const R* __restrict__ uy, const uint8_t* __restrict__ flag,
+                       const uint8_t* __restrict__ bedwall, R* __restrict__ uref, CellMap m, int nz) {
+    uref_column(ux, uy, flag, bedwall, uref, m, nz);
 }
 
 // Fluid cells per column (at least 1), for the per-step column velocity.
@@ -253,24 +260,36 @@ __global__ void k_accumulate(const R* __restrict__ rho, const R* __restrict__ ux
 
 // ---------------------------------------------------------------- batched cadence
 // Per-case run scalars kept on the device by a batch (the state of
-// floodsim/reach3d.py Run3D.advance between chunks: time, ramp, averaging).
+// floodsim/reach3d.py Run3D.advance between chunks: time, ramp, averaging),
+// and the case's step budget: each case stops when its own budget is spent,
+// so cases of uneven length (each run to its own end time) share launches.
 template <typename R>
 struct CaseSched {
     double t, dt, t_ramp, t_avg;
+    long long steps_left;   // step budget; negative: no limit
+    long long steps_taken;  // steps taken since the host last read them
     R fscale;          // force scale used by the step
     int ramp_done;
     int navg;
     int accumulate;    // this chunk adds to the time-mean sums
+    int active;        // steps this case takes in the current chunk (0: stopped)
+    int flip;          // the case reads buffer f[parity ^ flip] (steps skipped while stopped)
 };
 
-// Before a chunk: thrust ramp F = F0 * s, s = smoothstep(min(t / t_ramp, 1))
-// evaluated in double and rounded to float (reach3d.py:431-437).
+// Before a chunk of m steps: how many of them the case takes (its budget),
+// and for a running case the thrust ramp F = F0 * s, s = smoothstep(min(t /
+// t_ramp, 1)) evaluated in double and rounded to float (reach3d.py:431-437).
+// af[2k], af[2k+1] mirror case k's active and flip for the step kernels (a
+// flat array they read beside the case's arguments).
 template <typename R>
-__global__ void k_batch_pre(CaseSched<R>* __restrict__ sch, int n) {
+__global__ void k_batch_pre(CaseSched<R>* __restrict__ sch, int* __restrict__ af, int n, int m) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     CaseSched<R> c = sch[k];
-    if (!c.ramp_done) {
+    c.active = c.steps_left < 0 ? m : (int)(c.steps_left < m ? c.steps_left : m);
+    af[2 * k] = c.active;
+    af[2 * k + 1] = c.flip;
+    if (c.active > 0 && !c.ramp_done) {
         double f = c.t / c.t_ramp;
         f = f < 1.0 ? f : 1.0;
         const float s = (float)(f * f * (3.0 - 2.0 * f));
@@ -280,17 +299,38 @@ __global__ void k_batch_pre(CaseSched<R>* __restrict__ sch, int n) {
     sch[k] = c;
 }
 
-// After a chunk of m steps: t += m dt; the time-mean sums take this state
-// once t reaches the averaging window (reach3d.py:441-451).
+// After a chunk of m steps, for a case that took a > 0 of them: t += a dt,
+// and the time-mean sums take this state once t reaches the averaging window
+// (reach3d.py:441-451). The m - a launches a stopping case sat out flipped
+// the batch parity without it, so its buffer parity is corrected.
 template <typename R>
-__global__ void k_batch_post(CaseSched<R>* __restrict__ sch, int n, int m) {
+__global__ void k_batch_post(CaseSched<R>* __restrict__ sch, int* __restrict__ af, int n, int m) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     CaseSched<R> c = sch[k];
-    c.t += m * c.dt;
-    c.accumulate = c.t >= c.t_avg;
-    if (c.accumulate) c.navg += 1;
+    const int a = c.active;
+    c.flip ^= (m - a) & 1;
+    af[2 * k + 1] = c.flip;
+    c.accumulate = 0;
+    if (a > 0) {
+        c.t += a * c.dt;
+        c.accumulate = c.t >= c.t_avg;
+        if (c.accumulate) c.navg += 1;
+        c.steps_taken += a;
+        if (c.steps_left > 0) c.steps_left -= a;
+    }
     sch[k] = c;
+}
+
+// k_uref for case k of a batch, skipped when the case takes no step in
+// this chunk (a stopped case keeps the uref of its last chunk, as a
+// sequential run would).
+template <typename R>
+__global__ void k_uref_if(const CaseSched<R>* __restrict__ sch, int k, const R* __restrict__ ux,
+                          const R* __restrict__ uy, const uint8_t* __restrict__ flag,
+                          const uint8_t* __restrict__ bedwall, R* __restrict__ uref, CellMap m, int nz) {
+    if (sch[k].active <= 0) return;
+    uref_column(ux, uy, flag, bedwall, uref, m, nz);
 }
 
 // k_accumulate for one case, skipped when its schedule says so.
@@ -407,13 +447,10 @@ namespace wfs {
 // the wall receives the momentum c_opp(i) (g + f_i). Each block writes its
 // nine partial sums; k_sum_parts adds the blocks in a fixed order.
 template <class Codec, bool kSparse>
-__global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __restrict__ fs,
-                                                    StepArgs<typename Codec::R> a, CellMap cm,
-                                                    const uint32_t* __restrict__ body, double* __restrict__ part) {
+__device__ __forceinline__ void wall_force_cell(const typename Codec::T* __restrict__ fs,
+                                                const StepArgs<typename Codec::R>& a, CellMap cm,
+                                                const uint32_t* __restrict__ body, long long c, double* acc) {
     using R = typename Codec::R;
-    __shared__ double red[9][256];
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c < a.N) {
         int x;
         long long j;
@@ -423,9 +460,9 @@ __global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __r
         const uint8_t fl = x < a.nx ? a.flag[c] : uint8_t(FLAG_SOLID);
         if (!(fl & FLAG_SOLID)) {
             if (fl & FLAG_FORCED) {
-                acc[0] = double(a.aux->Fx[c] * a.fscale);
-                acc[1] = double(a.aux->Fy[c] * a.fscale);
-                acc[2] = double(a.aux->Fz[c] * a.fscale);
+                acc[0] += double(a.aux->Fx[c] * a.fscale);
+                acc[1] += double(a.aux->Fy[c] * a.fscale);
+                acc[2] += double(a.aux->Fz[c] * a.fscale);
             }
             if (fl & FLAG_NEARWALL) {
                 const uint32_t wm = a.wmask[c];
@@ -452,6 +489,11 @@ __global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __r
             }
         }
     }
+}
+
+// The nine sums of a 256-thread work-group in a fixed LDS tree; thread 0..8 write part[g * 9 + k].
+__device__ __forceinline__ void block_sum9(const double* acc, double* __restrict__ part, long long g) {
+    __shared__ double red[9][256];
 #pragma unroll
     for (int k = 0; k < 9; ++k) red[k][threadIdx.x] = acc[k];
     __syncthreads();
@@ -461,7 +503,17 @@ __global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __r
             for (int k = 0; k < 9; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x < 9) part[(long long)blockIdx.x * 9 + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 9) part[g * 9 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// One 256-cell block per work-group, partials per block.
+template <class Codec, bool kSparse>
+__global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __restrict__ fs,
+                                                    StepArgs<typename Codec::R> a, CellMap cm,
+                                                    const uint32_t* __restrict__ body, double* __restrict__ part) {
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    wall_force_cell<Codec, kSparse>(fs, a, cm, body, (long long)blockIdx.x * blockDim.x + threadIdx.x, acc);
+    block_sum9(acc, part, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- particle tracer

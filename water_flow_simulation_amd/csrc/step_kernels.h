@@ -1422,20 +1422,37 @@ struct CaseArgs {
     unsigned group_begin;
 };
 
-// All cases advance one step: work-group g belongs to case group_case[g]
-// and covers one (row, x-tile) of it. parity selects the source buffer.
+// Step s of a chunk for all cases: work-group g belongs to case
+// group_case[g] and covers one (row, x-tile) of it. af[2k] is how many
+// steps of the chunk case k takes (its budget; the cadence kernel writes
+// it) and af[2k+1] its buffer-parity correction. A case whose budget ends
+// before step s sits the launch out; the batch parity, corrected by the
+// case's flip, selects its source buffer; the case's last step of the
+// chunk writes the macroscopic fields of every cell (the chunk end of a
+// sequential run).
+// Resident waves per SIMD asked for the batched step. FP32 asks for 6,
+// which k_step reaches by itself: the per-case schedule reads would
+// otherwise take the batched FP32 step to 82 VGPRs and 5 waves. The 16-bit
+// instances reach 7 unasked (65-66 VGPRs); asked for 7 they spill scalars.
 template <class Codec>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WFS_STEP_WAVES_PER_EU)))
-k_step_batch(const CaseArgs<Codec>* __restrict__ cases, const unsigned short* __restrict__ group_case, int parity,
-             int write_all) {
+constexpr int kBatchWaves = (sizeof(typename Codec::T) == 4 && WFS_STEP_WAVES_PER_EU < 6) ? 6 : WFS_STEP_WAVES_PER_EU;
+
+template <class Codec>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kBatchWaves<Codec>)))
+k_step_batch(const CaseArgs<Codec>* __restrict__ cases, const unsigned short* __restrict__ group_case,
+             const int* __restrict__ af, int parity, int s) {
     const unsigned g = blockIdx.x;
-    const CaseArgs<Codec>& cs = cases[group_case[g]];
+    const int k = group_case[g];
+    const int act = af[2 * k];
+    if (s >= act) return;   // whole work-group: the case has stopped
+    const CaseArgs<Codec>& cs = cases[k];
+    const int par = parity ^ af[2 * k + 1];
     StepArgs<typename Codec::R> a = cs.a;
     a.fscale = *cs.fscale;
-    a.write_all = write_all;
+    a.write_all = s == act - 1 ? 1 : 0;
     int row, tile;
     if (!tile_of(g - cs.group_begin, a.tiles, a.rows, row, tile)) return;   // whole work-group
-    step_tile<Codec>(cs.f[parity], cs.f[parity ^ 1], a, row, tile);
+    step_tile<Codec>(cs.f[par], cs.f[par ^ 1], a, row, tile);
 }
 
 // ---------------------------------------------------------------- inlet/outlet rows
@@ -1452,12 +1469,10 @@ __device__ __forceinline__ typename Codec::R feq_generic(int i, typename Codec::
 }
 
 template <class Codec>
-__global__ void __launch_bounds__(256) k_bc(typename Codec::T* __restrict__ fd, const uint8_t* __restrict__ flag,
-                                            const typename Codec::R* __restrict__ uin, int nx, int px, int rows,
-                                            long long N, typename Codec::R rho_out, int sides, Center ctr) {
+__device__ __forceinline__ void bc_row(typename Codec::T* __restrict__ fd, const uint8_t* __restrict__ flag,
+                                       const typename Codec::R* __restrict__ uin, int nx, int px, long long N,
+                                       typename Codec::R rho_out, int sides, Center ctr, int j) {
     using R = typename Codec::R;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= rows) return;
     const long long row = (long long)px * j;  // j = y + ny*z
     for (int side = 0; side < 2; ++side) {
         if (!((sides >> side) & 1)) continue;
@@ -1484,6 +1499,29 @@ __global__ void __launch_bounds__(256) k_bc(typename Codec::T* __restrict__ fd, 
                                   store_key<Codec>(ctr, i));
         });
     }
+}
+
+template <class Codec>
+__global__ void __launch_bounds__(256) k_bc(typename Codec::T* __restrict__ fd, const uint8_t* __restrict__ flag,
+                                            const typename Codec::R* __restrict__ uin, int nx, int px, int rows,
+                                            long long N, typename Codec::R rho_out, int sides, Center ctr) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= rows) return;
+    bc_row<Codec>(fd, flag, uin, nx, px, N, rho_out, sides, ctr, j);
+}
+
+// The outlet row of case k of a batch after step s of a chunk (cases whose
+// outlet row opens a wavefront), on the buffer that step wrote; skipped
+// when the case has stopped.
+template <class Codec>
+__global__ void __launch_bounds__(256) k_bc_batch(const CaseArgs<Codec>* __restrict__ cases,
+                                                  const int* __restrict__ af, int k, int parity, int s) {
+    if (s >= af[2 * k]) return;
+    const CaseArgs<Codec>& cs = cases[k];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cs.a.rows) return;
+    const int par = parity ^ af[2 * k + 1];
+    bc_row<Codec>(cs.f[par ^ 1], cs.a.flag, cs.a.uin, cs.a.nx, cs.a.px, cs.a.N, typename Codec::R(1), 2, cs.a.ctr, j);
 }
 
 }  // namespace wfs

@@ -53,6 +53,28 @@ __global__ void k_fill_zero_u32(uint32_t* p, long long n) {
     if (k < n) p[k] = 0u;
 }
 
+// 64-bit fingerprint of a byte range: 8-byte words through a multiply-rotate
+// mix, the tail zero-padded, the length folded in last. Identifies the
+// geometry a checkpoint was taken on; not a cryptographic hash.
+uint64_t fingerprint(const void* p, size_t n, uint64_t h) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    auto mix = [&h](uint64_t w) {
+        h ^= w * 0x9E3779B97F4A7C15ull;
+        h = ((h << 31) | (h >> 33)) * 0xBF58476D1CE4E5B9ull;
+    };
+    size_t k = 0;
+    for (; k + 8 <= n; k += 8) {
+        uint64_t w;
+        std::memcpy(&w, b + k, 8);
+        mix(w);
+    }
+    uint64_t w = 0;
+    if (k < n) std::memcpy(&w, b + k, n - k);
+    mix(w);
+    mix((uint64_t)n);
+    return h;
+}
+
 // Device buffer owned by a solver.
 struct DevBuf {
     void* p = nullptr;
@@ -157,6 +179,7 @@ struct wfs_solver {
     bool inplace = false;
     int aa_phase = 0;
     int batch_refs = 0;     // batches that hold this solver's buffers
+    uint64_t geo_hash = 0;  // fingerprint of the uploaded geometry (dims, solid mask, wall cf)
     int bx = 128;   // step-kernel work-group width (64..256)
     // tile-sparse layout: only the 64-cell x-runs of a row that hold fluid are stored
     bool tiled = false;
@@ -235,6 +258,7 @@ struct Solver final : wfs_solver {
         if ((rc = alloc_buf(partial, (size_t)d.nx * 256 * sizeof(double)))) return rc;
         if ((rc = alloc_buf(counter, 256))) return rc;
         if ((rc = alloc_buf(stage, (size_t)N * sizeof(double)))) return rc;  // dense host <-> device staging
+        if ((rc = alloc_buf(aux_d, 2 * sizeof(StepAux<R>)))) return rc;     // StepAux pair (sync_aux fills it)
         if (tiled) {
             for (auto pr : {std::make_pair(&tmap_d, &h_tmap), std::make_pair(&tcoord_d, &h_tcoord),
                             std::make_pair(&nbr_d, &h_nbr), std::make_pair(&blk2tile_d, &h_blk2tile),
@@ -305,6 +329,10 @@ struct Solver final : wfs_solver {
 
     int upload(const uint8_t* solid, const float* cf, const float* uin_h) override {
         int rc;
+        const int32_t dims[4] = {d.nx, d.ny, d.nz, tiled ? 1 : 0};
+        geo_hash = fingerprint(dims, sizeof dims, 0x57465347656f6dull);
+        geo_hash = fingerprint(solid, (size_t)N, geo_hash);
+        if (cf) geo_hash = fingerprint(cf, (size_t)N * sizeof(float), geo_hash);
         for (int k = 0; k < 2; ++k)
             if (!inplace && !f[k].p && (rc = alloc_buf(f[k], (size_t)Np * Q * sizeof(T)))) return rc;
         if (cf) {
@@ -740,11 +768,16 @@ struct Solver final : wfs_solver {
     // storage, layout, streaming mode, column-speed mode), a run continues
     // bit for bit.
     static constexpr uint64_t kStateMagic = 0x3144335374534657ull;   // the bytes "WFStS3D1"
+    // Version 2 adds the lattice size and the geometry fingerprint: a state of
+    // another reach with the same stored-cell count is refused.
+    static constexpr int32_t kStateVersion = 2;
     struct StateHead {
         uint64_t magic;
         int32_t version, storage, tiled, inplace, aa_phase, colu_on, colu_par, navg;
         int64_t Np, nstep;
         double fscale;
+        int32_t nx, ny, nz, pad;
+        uint64_t geo_hash;
     };
     std::vector<std::pair<DevBuf*, size_t>> state_parts() {
         std::vector<std::pair<DevBuf*, size_t>> v;
@@ -768,8 +801,9 @@ struct Solver final : wfs_solver {
         state_bytes(&need);
         if (bytes != need) return fail(WFS_EINVAL, "wfs_state_save: buffer size " + std::to_string(bytes) +
                                                        " != wfs_state_bytes " + std::to_string(need));
-        StateHead h{kStateMagic, 1, d.storage, tiled ? 1 : 0, inplace ? 1 : 0, aa_phase, colu_on ? 1 : 0,
-                    colu_par, navg, Np, (int64_t)nstep, (double)fscale};
+        StateHead h{kStateMagic, kStateVersion, d.storage, tiled ? 1 : 0, inplace ? 1 : 0, aa_phase,
+                    colu_on ? 1 : 0, colu_par, navg, Np, (int64_t)nstep, (double)fscale, d.nx, d.ny, d.nz, 0,
+                    geo_hash};
         std::memcpy(out, &h, sizeof h);
         WFS_HIP(hipStreamSynchronize(stream));
         char* o = static_cast<char*>(out) + sizeof h;
@@ -784,9 +818,13 @@ struct Solver final : wfs_solver {
         StateHead h;
         if (bytes < (long long)sizeof h) return fail(WFS_EINVAL, "wfs_state_load: not a solver state (too short)");
         std::memcpy(&h, in, sizeof h);
-        if (h.magic != kStateMagic || h.version != 1) return fail(WFS_EINVAL, "wfs_state_load: not a solver state");
+        if (h.magic != kStateMagic || h.version != kStateVersion)
+            return fail(WFS_EINVAL, "wfs_state_load: not a solver state");
         if (h.storage != d.storage || h.tiled != (tiled ? 1 : 0) || h.Np != Np)
             return fail(WFS_EINVAL, "wfs_state_load: the state is of another storage format, layout or size");
+        if (h.nx != d.nx || h.ny != d.ny || h.nz != d.nz || h.geo_hash != geo_hash)
+            return fail(WFS_EINVAL, "wfs_state_load: the state is of another geometry (lattice size, solid mask or "
+                                    "wall coefficients differ)");
         if (h.inplace != (inplace ? 1 : 0))
             return fail(WFS_EINVAL, "wfs_state_load: the state is of the other streaming mode");
         if (h.colu_on != (colu_on ? 1 : 0))
@@ -1142,6 +1180,8 @@ struct BatchBase {
         if (stream) (void)hipStreamDestroy(stream);
     }
     virtual int set_schedule(int k, double dt, double t_ramp, double t_avg, double t0) = 0;
+    virtual int set_budget(int k, long long steps) = 0;
+    virtual int budget_of(int k, long long* steps_left) = 0;
     virtual int run(int nsteps, int use_graph) = 0;
     virtual int synchronize() = 0;
     virtual int time_of(int k, double* t, int32_t* navg) = 0;
@@ -1152,11 +1192,10 @@ struct Batch final : BatchBase {
     using R = typename Codec::R;
     std::vector<Solver<Codec>*> cases;
     std::vector<CaseArgs<Codec>> args;   // host copy of the per-case step views
-    DevBuf d_args, d_group_case, d_sched;
+    DevBuf d_args, d_group_case, d_sched, d_af;   // d_af: per case (active, flip) for the step kernels
     std::vector<CaseSched<R>> sched;     // host mirror, valid after synchronize()
     unsigned groups = 0;
-    int parity = 0;                      // 0: every case reads its buffer f[0]
-    int64_t pending_steps = 0;
+    int parity = 0;                      // batch parity: case k reads f[parity ^ flip_k]
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool dirty_sched = true;
@@ -1206,6 +1245,10 @@ struct Batch final : BatchBase {
             sc.ramp_done = 1;
             sc.navg = c->navg;
             sc.accumulate = 0;
+            sc.steps_left = -1;
+            sc.steps_taken = 0;
+            sc.active = 0;
+            sc.flip = 0;
             sched.push_back(sc);
         }
         groups = (unsigned)gcase.size();
@@ -1214,13 +1257,19 @@ struct Batch final : BatchBase {
         if ((rc = cases[0]->alloc_buf(d_group_case, gcase.size() * sizeof(unsigned short)))) return rc;
         if ((rc = cases[0]->alloc_buf(d_sched, sched.size() * sizeof(CaseSched<R>)))) return rc;
         if ((rc = cases[0]->alloc_buf(d_args, args.size() * sizeof(CaseArgs<Codec>)))) return rc;
+        if ((rc = cases[0]->alloc_buf(d_af, args.size() * 2 * sizeof(int)))) return rc;
         for (size_t k = 0; k < args.size(); ++k)
             args[k].fscale = &static_cast<CaseSched<R>*>(d_sched.p)[k].fscale;
         WFS_HIP(hipMemcpyAsync(d_group_case.p, gcase.data(), gcase.size() * sizeof(unsigned short),
                                hipMemcpyHostToDevice, stream));
         WFS_HIP(hipMemcpyAsync(d_args.p, args.data(), args.size() * sizeof(CaseArgs<Codec>), hipMemcpyHostToDevice,
                                stream));
+        // the device schedule is valid from the start: a synchronize before
+        // the first run reads back what the cases hold now
+        WFS_HIP(hipMemcpyAsync(d_sched.p, sched.data(), sched.size() * sizeof(CaseSched<R>), hipMemcpyHostToDevice,
+                               stream));
         WFS_HIP(hipStreamSynchronize(stream));
+        dirty_sched = false;
         return WFS_OK;
     }
 
@@ -1238,36 +1287,55 @@ struct Batch final : BatchBase {
         return WFS_OK;
     }
 
-    // The kernels of one chunk of m steps, queued on the batch stream.
+    // Budget of case k in steps from now (negative: no limit). A case whose
+    // budget is spent sits out the launches until it gets a new one.
+    int set_budget(int k, long long steps) override {
+        if (k < 0 || k >= (int)cases.size()) return fail(WFS_EINVAL, "wfs_batch_set_budget: case out of range");
+        sched[(size_t)k].steps_left = steps < 0 ? -1 : steps;
+        dirty_sched = true;
+        return WFS_OK;
+    }
+
+    int budget_of(int k, long long* steps_left) override {
+        if (k < 0 || k >= (int)cases.size()) return fail(WFS_EINVAL, "wfs_batch_budget: case out of range");
+        *steps_left = sched[(size_t)k].steps_left;
+        return WFS_OK;
+    }
+
+    // The kernels of one chunk of m steps, queued on the batch stream. Each
+    // case takes min(m, its budget) of them (k_batch_pre), with the cadence
+    // of a sequential chunk of that length.
     int enqueue_chunk(int m, int parity0) {
         const int n = (int)cases.size();
         CaseSched<R>* ds = static_cast<CaseSched<R>*>(d_sched.p);
-        hipLaunchKernelGGL(k_batch_pre<R>, dim3(blocks_for(n, 64)), dim3(64), 0, stream, ds, n);
+        int* af = static_cast<int*>(d_af.p);
+        hipLaunchKernelGGL(k_batch_pre<R>, dim3(blocks_for(n, 64)), dim3(64), 0, stream, ds, af, n, m);
         WFS_LAUNCH_CHECK();
-        for (auto* c : cases) {
+        for (size_t k = 0; k < cases.size(); ++k) {
+            Solver<Codec>* c = cases[k];
             if (!c->have_bedwall) continue;
-            hipLaunchKernelGGL(k_uref<R>, dim3(blocks_for(c->d.nx, 64), c->d.ny), dim3(64), 0, stream,
-                               c->rp(c->ux), c->rp(c->uy), c->flags(), static_cast<const uint8_t*>(c->bedwall.p),
-                               c->rp(c->uref), c->cmap(), c->d.nz);
+            hipLaunchKernelGGL(k_uref_if<R>, dim3(blocks_for(c->d.nx, 64), c->d.ny), dim3(64), 0, stream,
+                               static_cast<const CaseSched<R>*>(ds), (int)k, c->rp(c->ux), c->rp(c->uy), c->flags(),
+                               static_cast<const uint8_t*>(c->bedwall.p), c->rp(c->uref), c->cmap(), c->d.nz);
             WFS_LAUNCH_CHECK();
         }
         const CaseArgs<Codec>* da = static_cast<const CaseArgs<Codec>*>(d_args.p);
         int par = parity0;
         for (int s = 0; s < m; ++s) {
             hipLaunchKernelGGL(k_step_batch<Codec>, dim3(groups), dim3(bx), 0, stream, da,
-                               static_cast<const unsigned short*>(d_group_case.p), par, s == m - 1 ? 1 : 0);
+                               static_cast<const unsigned short*>(d_group_case.p), static_cast<const int*>(af), par,
+                               s);
             WFS_LAUNCH_CHECK();
             for (size_t k = 0; k < cases.size(); ++k) {
                 Solver<Codec>* c = cases[k];
                 if ((c->d.nx - 1) % 64 != 0) continue;   // outlet row fused into the step
-                hipLaunchKernelGGL(k_bc<Codec>, dim3(blocks_for(c->rows(), 64)), dim3(64), 0, stream,
-                                   args[k].f[par ^ 1], c->flags(), c->rp(c->uin), c->d.nx, c->px, c->rows(), c->Np,
-                                   R(1), 2, c->ctr);
+                hipLaunchKernelGGL(k_bc_batch<Codec>, dim3(blocks_for(c->rows(), 64)), dim3(64), 0, stream, da,
+                                   static_cast<const int*>(af), (int)k, par, s);
                 WFS_LAUNCH_CHECK();
             }
             par ^= 1;
         }
-        hipLaunchKernelGGL(k_batch_post<R>, dim3(blocks_for(n, 64)), dim3(64), 0, stream, ds, n, m);
+        hipLaunchKernelGGL(k_batch_post<R>, dim3(blocks_for(n, 64)), dim3(64), 0, stream, ds, af, n, m);
         WFS_LAUNCH_CHECK();
         for (size_t k = 0; k < cases.size(); ++k) {
             Solver<Codec>* c = cases[k];
@@ -1317,7 +1385,6 @@ struct Batch final : BatchBase {
                 graph = nullptr;
             }
         }
-        pending_steps += nsteps;
         return WFS_OK;
     }
 
@@ -1326,12 +1393,13 @@ struct Batch final : BatchBase {
         WFS_HIP(hipMemcpy(sched.data(), d_sched.p, sched.size() * sizeof(CaseSched<R>), hipMemcpyDeviceToHost));
         for (size_t k = 0; k < cases.size(); ++k) {
             Solver<Codec>* c = cases[k];
-            c->nstep += pending_steps;
+            c->nstep += sched[k].steps_taken;
+            sched[k].steps_taken = 0;
             c->navg = sched[k].navg;
             c->fscale = sched[k].fscale;
-            c->cur = (args[k].f[parity] == c->fp(0)) ? 0 : 1;
+            c->cur = (args[k].f[(parity ^ sched[k].flip) & 1] == c->fp(0)) ? 0 : 1;
         }
-        pending_steps = 0;
+        dirty_sched = true;   // steps_taken cleared on the host copy
         return WFS_OK;
     }
 
@@ -1357,6 +1425,15 @@ wfs_solver* make_solver(int storage) {
 int check_handle(const wfs_solver* h, bool need_upload = true) {
     if (!h) return fail(WFS_EINVAL, "null solver handle");
     if (need_upload && !h->uploaded) return fail(WFS_EINVAL, "solver has no geometry yet (call wfs_upload first)");
+    return WFS_OK;
+}
+
+// A batch keeps its own pointers to a case's population buffers and its
+// parity: while a batch holds a solver, calls that step it or replace its
+// state directly would be overwritten by the batch's next run (or race its
+// stream), so they are refused.
+int check_not_batched(const wfs_solver* h, const char* what) {
+    if (h->batch_refs) return fail(WFS_EINVAL, std::string(what) + ": the solver is a case of a batch (destroy the batch first)");
     return WFS_OK;
 }
 
@@ -1620,6 +1697,7 @@ int wfs_step(wfs_solver* h, int32_t n) {
     int rc = check_handle(h);
     if (rc) return rc;
     if (n < 0) return fail(WFS_EINVAL, "wfs_step: n must be >= 0");
+    if ((rc = check_not_batched(h, "wfs_step"))) return rc;
     if (n == 0) return WFS_OK;
     DeviceGuard g(h->device);
     return h->step(n);
@@ -1653,6 +1731,7 @@ int wfs_set_macro(wfs_solver* h, const float* rho, const float* ux, const float*
     int rc = check_handle(h);
     if (rc) return rc;
     if (!rho || !ux || !uy || !uz) return fail(WFS_EINVAL, "wfs_set_macro: null array");
+    if ((rc = check_not_batched(h, "wfs_set_macro"))) return rc;
     DeviceGuard g(h->device);
     return h->set_macro(rho, ux, uy, uz);
 }
@@ -1685,6 +1764,7 @@ int wfs_set_populations(wfs_solver* h, const float* f) {
     int rc = check_handle(h);
     if (rc) return rc;
     if (!f) return fail(WFS_EINVAL, "wfs_set_populations: null array");
+    if ((rc = check_not_batched(h, "wfs_set_populations"))) return rc;
     DeviceGuard g(h->device);
     return h->set_pops(f);
 }
@@ -1824,6 +1904,19 @@ int wfs_batch_set_schedule(wfs_batch* b, int32_t k, double dt, double t_ramp, do
     return b->impl->set_schedule(k, dt, t_ramp, t_avg, t0);
 }
 
+int wfs_batch_set_budget(wfs_batch* b, int32_t k, int64_t steps) {
+    if (!b) return fail(WFS_EINVAL, "null batch");
+    return b->impl->set_budget(k, (long long)steps);
+}
+
+int wfs_batch_budget(wfs_batch* b, int32_t k, int64_t* steps_left) {
+    if (!b || !steps_left) return fail(WFS_EINVAL, "wfs_batch_budget: null argument");
+    long long v = 0;
+    int rc = b->impl->budget_of(k, &v);
+    *steps_left = v;
+    return rc;
+}
+
 int wfs_batch_run(wfs_batch* b, int32_t nsteps, int32_t use_graph) {
     if (!b) return fail(WFS_EINVAL, "null batch");
     DeviceGuard g(b->impl->device);
@@ -1949,6 +2042,8 @@ int wfs_state_save(wfs_solver* h, void* out, int64_t bytes) {
 
 int wfs_state_load(wfs_solver* h, const void* in, int64_t bytes) {
     if (!h || !in) return fail(WFS_EINVAL, "wfs_state_load: null argument");
+    int rc = check_not_batched(h, "wfs_state_load");
+    if (rc) return rc;
     DeviceGuard g(h->device);
     return h->state_load(in, bytes);
 }

@@ -14,11 +14,17 @@ uref`` may be replaced by float32 arrays of the right size, checked at the
 next ``step`` with ``TypeError`` like the reference (lbm3d.py:392-394).
 
 Differences by design: the state lives in GPU memory, so the field
-attributes return host copies (a fetch per access) instead of live device
-views; ``be`` is accepted for signature compatibility and only a torch
-backend is rejected, as in the reference (lbm3d.py:349-350). Keyword-only
-extensions choose the population storage, the work-group width and the
-device, and give the device-side cadence helpers used by ``Run3D``.
+attributes return host copies instead of live device views. One fetch
+brings all four fields of the current step; later reads of any of them
+before the next step reuse it, so a caller that reads ``ux``, ``uy``,
+``uz`` and ``rho`` after a chunk (reach3d.py:412-418, 441-450) pays one
+device-to-host copy of the fields, not six. The returned arrays are
+read-only. ``xp`` is numpy: the arrays a caller gets and gives are host
+arrays (the reference's ``xp`` is the array module of its backend,
+lbm3d.py:348-351). ``be`` is accepted for signature compatibility and only a
+torch backend is rejected, as in the reference (lbm3d.py:349-350).
+Keyword-only extensions choose the population storage, the work-group width
+and the device, and give the device-side cadence helpers used by ``Run3D``.
 """
 from __future__ import annotations
 
@@ -36,6 +42,7 @@ W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
 
 class LBM3D:
     engine = "hip-lbm"
+    xp = np
 
     def __init__(self, be, solid, Fx, Fy, Fz, uin, tau0, cs=0.17, cf_wall=None, *, storage="f32", block_x=0,
                  device=0, layout="dense", streaming="ab"):
@@ -71,7 +78,9 @@ class LBM3D:
         else:
             nat.check(L.wfs_create(ctypes.byref(desc), ctypes.byref(h)), "wfs_create")
         self._h = h
-        A = lambda a: np.ascontiguousarray(np.asarray(a).ravel(), dtype=np.float32)  # noqa: E731
+        # own copies, as the reference's device backend takes them (lbm3d.py:352-355): a caller
+        # scaling L.Fx in place must not change its own arrays
+        A = lambda a: np.array(a, dtype=np.float32, copy=True).ravel()  # noqa: E731
         cfw = A(cf_wall) if cf_wall is not None else None
         if cfw is not None and cfw.size != self.N:
             raise ValueError("LBM3D: cf_wall must have the shape of solid")
@@ -91,6 +100,8 @@ class LBM3D:
                       "wfs_set_force")
         self._uref = None          # host value assigned by a caller, uploaded at the next step
         self._dirty = set()
+        self._gen = 0              # bumped by every call that may change the macroscopic fields
+        self._macro_cache = None   # ((nstep, gen), (rho, ux, uy, uz)) of the last fetch
 
     # ------------------------------------------------------------ lifetime
     def close(self):
@@ -181,6 +192,7 @@ class LBM3D:
     def step(self, n=1):
         """Advance n steps (lbm3d.py:390-412). Asynchronous on the GPU."""
         self._flush()
+        self._gen += 1
         nat.check(nat.lib().wfs_step(self.handle, int(n)), "wfs_step")
         self.nstep += int(n)
 
@@ -195,10 +207,21 @@ class LBM3D:
         nat.check(fn(self.handle, *(nat.ptr(a) for a in out)))
         return tuple(out)
 
-    rho = property(lambda s: s.macro()[0])
-    ux = property(lambda s: s.macro()[1])
-    uy = property(lambda s: s.macro()[2])
-    uz = property(lambda s: s.macro()[3])
+    def _fields(self):
+        """(rho, ux, uy, uz) of the current step, fetched once per step and
+        shared by the four field attributes (read-only arrays)."""
+        key = (int(nat.lib().wfs_nstep(self.handle)), self._gen)
+        if self._macro_cache is None or self._macro_cache[0] != key:
+            arrs = self.macro()
+            for a in arrs:
+                a.flags.writeable = False
+            self._macro_cache = (key, arrs)
+        return self._macro_cache[1]
+
+    rho = property(lambda s: s._fields()[0])
+    ux = property(lambda s: s._fields()[1])
+    uy = property(lambda s: s._fields()[2])
+    uz = property(lambda s: s._fields()[3])
 
     def set_macro(self, rho, ux, uy, uz):
         """Overwrite the macroscopic fields the next step's wall model reads."""
@@ -206,6 +229,7 @@ class LBM3D:
         arrs = [A(a) for a in (rho, ux, uy, uz)]
         if any(a.size != self.N for a in arrs):
             raise ValueError("LBM3D: macroscopic fields must hold N values")
+        self._gen += 1
         nat.check(nat.lib().wfs_set_macro(self.handle, *(nat.ptr(a) for a in arrs)))
 
     def populations(self, dtype=np.float32):
@@ -312,9 +336,12 @@ class LBM3D:
         """Restore a state() of a solver built the same way; the run then
         continues bit for bit."""
         buf = np.ascontiguousarray(buf, dtype=np.uint8)
+        self._gen += 1
         nat.check(nat.lib().wfs_state_load(self.handle, nat.ptr(buf), buf.size))
         self.nstep = int(nat.lib().wfs_nstep(self.handle))
+        # uref comes with the state: a value assigned before the restore is dropped
         self._uref = None
+        self._dirty.discard("uref")
 
     # ------------------------------------------------------------ particle tracer
     def particles_set(self, xyz, inlet_rows, seed=0):

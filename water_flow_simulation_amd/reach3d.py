@@ -531,10 +531,30 @@ class Run3D:
         L.synchronize()
         self.steps_wall += time.time() - t0
 
+    @staticmethod
+    def run_chunk(chunk=400):
+        """Steps per advance() call of run_to_end: ``chunk``, or 100 when the
+        GPU duty cycle is limited (reach3d.py:456-458)."""
+        return 100 if min(max(GPU_DUTY, 0.05), 1.0) < 1 else chunk
+
+    def steps_to_end(self, chunk=400):
+        """How many steps run_to_end(chunk=chunk) takes from the current time:
+        whole advance() calls until t >= t_total, with t accumulated chunk by
+        chunk in the same float64 operations as advance()."""
+        chunk = self.run_chunk(chunk)
+        t, n = self.t, 0
+        while not t >= self.t_total:
+            k = 0
+            while k < chunk:
+                m = min(self.cadence, chunk - k)
+                t += m * self.r.dt
+                k += m
+            n += chunk
+        return n
+
     def run_to_end(self, progress=None, cancel=None, chunk=400):
         duty = min(max(GPU_DUTY, 0.05), 1.0)
-        if duty < 1:
-            chunk = 100
+        chunk = self.run_chunk(chunk)
         while not self.done:
             t0 = time.time()
             self.advance(chunk)
