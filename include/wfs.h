@@ -252,6 +252,19 @@ int wfs_total(wfs_solver* h, int32_t field, double* out);
 int wfs_set_body_mask(wfs_solver* h, const uint8_t* body);
 int wfs_forces(wfs_solver* h, double* out);
 
+/* wfs_forces without any device allocation: the same nine sums, through a
+ * fixed scratch array in the library (147,528 bytes per GPU, held until the
+ * process ends) instead of partial-sum buffers allocated per call. No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The result equals wfs_forces up to the
+ * order of the FP64 summation (a relative difference of order 1e-15 of the
+ * largest component). The scratch does not grow with the lattice: any solver
+ * up to the library's 2^31 - 1 stored cells is accepted (WFS_EINVAL
+ * above). Every call on one GPU uses that GPU's one scratch array: calls
+ * that overlap on one GPU (from two threads) race and must be serialised by
+ * the caller. Blocks until the sums are on the host. */
+int wfs_forces_noalloc(wfs_solver* h, double* out);
 
 /* Watchdog: WFS_ENONFINITE if any fluid cell's rho or u is NaN/Inf. */
 int wfs_check_finite(wfs_solver* h);

@@ -97,6 +97,7 @@ _SIGS = {
     "wfs_particles_move": ([_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P], ctypes.c_int),
     "wfs_particles_get": ([_P, _P], ctypes.c_int),
     "wfs_forces": ([_P, _P], ctypes.c_int),
+    "wfs_forces_noalloc": ([_P, _P], ctypes.c_int),
     "wfs_set_block_x": ([_P, _I32], ctypes.c_int),
     "wfs_block_x": ([_P], _I32),
     "wfs_set_closure": ([_P, _I32, _P, _P, _P, ctypes.c_double], ctypes.c_int),

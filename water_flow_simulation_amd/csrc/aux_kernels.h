@@ -516,6 +516,21 @@ __global__ void __launch_bounds__(256) k_wall_force(const typename Codec::T* __r
     block_sum9(acc, part, blockIdx.x);
 }
 
+// The same sums with a fixed number of work-groups: group g takes the
+// 256-cell blocks g, g + gridDim.x, ..., so the partials (gridDim.x x 9)
+// do not grow with the lattice.
+template <class Codec, bool kSparse>
+__global__ void __launch_bounds__(256) k_wall_force_strided(const typename Codec::T* __restrict__ fs,
+                                                            StepArgs<typename Codec::R> a, CellMap cm,
+                                                            const uint32_t* __restrict__ body,
+                                                            double* __restrict__ part) {
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const long long nblk = (a.N + 255) / 256;
+    for (long long b = blockIdx.x; b < nblk; b += gridDim.x)
+        wall_force_cell<Codec, kSparse>(fs, a, cm, body, b * 256 + threadIdx.x, acc);
+    block_sum9(acc, part, blockIdx.x);
+}
+
 // ---------------------------------------------------------------- particle tracer
 // Forward Euler on the cell velocity, as the reference's tracer
 // (floodsim/reach3d.py:500-536): P (metres from the domain corner, x y z per

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -73,6 +74,38 @@ uint64_t fingerprint(const void* p, size_t n, uint64_t h) {
     mix(w);
     mix((uint64_t)n);
     return h;
+}
+
+// Scratch of wfs_forces_noalloc, the allocation-free variant of wfs_forces:
+// kNoAllocGroups work-groups each write nine partial sums, and one more
+// work-group writes their nine totals after them. Fixed size (147,528
+// bytes), one copy per GPU for the life of the process, shared by every
+// call on that GPU: the partials are overwritten by each call, nothing needs
+// clearing, and two calls that overlap on one GPU race.
+constexpr int kNoAllocGroups = 2048;
+constexpr long long kScratchDoubles = (long long)(kNoAllocGroups + 1) * 9;
+constexpr long long kNoAllocMaxCells = (1ll << 31) - 1;   // stored cells per call (the solver's own limit)
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+// The device address of `scratch` on each GPU, taken once per device (with
+// that device current) by hipGetSymbolAddress; host code uses only these
+// addresses, never the array's name.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+int scratch_of(int device, double** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(WFS_ENODEV, "scratch: device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(WFS_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[device] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch[device];
+    return WFS_OK;
 }
 
 // Device buffer owned by a solver.
@@ -168,6 +201,7 @@ struct wfs_solver {
     virtual int particles_move(double dt, double vel, double dx, long long* respawned) = 0;
     virtual int particles_get(double* xyz) = 0;
     virtual int forces(double* out) = 0;
+    virtual int forces_noalloc(double* out) = 0;
     virtual int get_closure_field(float* c2) = 0;
     virtual int closure_samples(const uint16_t* geo, const uint8_t* hcol, int ncx, int ncy, int ncz, double kv,
                                 double u0, float* X, float* ABP, uint8_t* valid) = 0;
@@ -1081,6 +1115,40 @@ struct Solver final : wfs_solver {
             if (!rc) rc = rc2;
             if (hipStreamSynchronize(stream) != hipSuccess && !rc) rc = fail(WFS_EHIP, "wfs_forces: synchronize");
         }
+        return rc;
+    }
+
+    // wfs_forces through the fixed scratch array: no device allocation.
+    int forces_noalloc(double* out) override {
+        int rc = aa_readable("wfs_forces_noalloc");
+        if (rc) return rc;
+        double* sc = nullptr;
+        if ((rc = scratch_of(device, &sc))) return rc;
+        if (inplace && (rc = aa_exchange())) return rc;
+        StepArgs<R> a;
+        rc = fill_args(a);
+        if (!rc) {
+            if (tiled)
+                hipLaunchKernelGGL((k_wall_force_strided<Codec, true>), dim3(kNoAllocGroups), dim3(256), 0, stream,
+                                   fp(cur), a, cmap(), static_cast<const uint32_t*>(body.p), sc);
+            else
+                hipLaunchKernelGGL((k_wall_force_strided<Codec, false>), dim3(kNoAllocGroups), dim3(256), 0, stream,
+                                   fp(cur), a, cmap(), static_cast<const uint32_t*>(body.p), sc);
+            if (hipGetLastError() != hipSuccess) rc = fail(WFS_EHIP, "wfs_forces_noalloc: launch");
+        }
+        if (!rc) {
+            hipLaunchKernelGGL(k_sum_parts<9>, dim3(1), dim3(256), 0, stream, static_cast<const double*>(sc),
+                               (long long)kNoAllocGroups, (long long)kNoAllocGroups, sc + (long long)kNoAllocGroups * 9);
+            if (hipGetLastError() != hipSuccess) rc = fail(WFS_EHIP, "wfs_forces_noalloc: launch");
+        }
+        if (!rc && hipMemcpyAsync(out, sc + (long long)kNoAllocGroups * 9, 9 * sizeof(double), hipMemcpyDeviceToHost,
+                                  stream) != hipSuccess)
+            rc = fail(WFS_EHIP, "wfs_forces_noalloc: copy");
+        if (inplace) {
+            int rc2 = aa_exchange();
+            if (!rc) rc = rc2;
+        }
+        if (hipStreamSynchronize(stream) != hipSuccess && !rc) rc = fail(WFS_EHIP, "wfs_forces_noalloc: synchronize");
         return rc;
     }
 
@@ -2046,6 +2114,16 @@ int wfs_state_load(wfs_solver* h, const void* in, int64_t bytes) {
     if (rc) return rc;
     DeviceGuard g(h->device);
     return h->state_load(in, bytes);
+}
+
+int wfs_forces_noalloc(wfs_solver* h, double* out) {
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!out) return fail(WFS_EINVAL, "wfs_forces_noalloc: null output");
+    if (h->Np >= kNoAllocMaxCells)
+        return fail(WFS_EINVAL, "wfs_forces_noalloc: more than 2^31 - 1 stored cells (" + std::to_string(h->Np) + ")");
+    DeviceGuard g(h->device);
+    return h->forces_noalloc(out);
 }
 
 int wfs_set_body_mask(wfs_solver* h, const uint8_t* body) {
