@@ -218,10 +218,16 @@ int wfs_get_populations_f64(wfs_solver* h, double* f);
 int wfs_set_populations(wfs_solver* h, const float* f);
 
 /* Run3D time-mean accumulation (reach3d.py:443-451): adds ux, uy, uz, rho,
- * |u| and ux^2 of the current state to running sums (storage precision). */
+ * |u| and ux^2 of the current state to running sums (storage precision).
+ * Unlike the reference, the density enters as rho - 1: a float sum of
+ * rho ~ 1 loses the level signal (rho - 1 ~ 1e-5..1e-4) once its ulp
+ * exceeds it (about 40 % of it lost after 3000 samples, 90 % after 11000),
+ * while a float sum of rho - 1 keeps it to its own relative precision. */
 int wfs_accumulate(wfs_solver* h);
 int wfs_reset_sums(wfs_solver* h);
-/* Copies the six running sums (f32, N each; NULL skips) and the sample count. */
+/* Copies the six running sums (f32, N each; NULL skips) and the sample count.
+ * The rho sum is returned as the sum of rho, n + sum(rho - 1) formed in
+ * double and rounded once. */
 int wfs_get_sums(wfs_solver* h, float* ux, float* uy, float* uz, float* rho, float* sp, float* uxx,
                  int32_t* navg);
 
@@ -229,7 +235,8 @@ int wfs_get_sums_f64(wfs_solver* h, double* ux, double* uy, double* uz, double* 
                      int32_t* navg);
 
 /* FP64 per-x reductions over fluid cells with z0 <= z < z1:
- * out[x] = sum_{y, z} field(x, y, z) for field in enum wfs_field. */
+ * out[x] = sum_{y, z} field(x, y, z) for field in enum wfs_field
+ * (WFS_F_SUM_RHO is the device sum of rho - 1). */
 enum wfs_field {
     WFS_F_RHO = 0, WFS_F_UX = 1, WFS_F_UY = 2, WFS_F_UZ = 3,
     WFS_F_SUM_UX = 4, WFS_F_SUM_UY = 5, WFS_F_SUM_UZ = 6, WFS_F_SUM_RHO = 7, WFS_F_SUM_SP = 8, WFS_F_SUM_UXX = 9,

@@ -163,11 +163,19 @@ class OracleRun:
     ``inputs`` holds the case arrays produced by a case builder: solid, Fx, Fy,
     Fz (lattice units), uin_lb, tau0, cs, cf, bedwall, and the scalars dt,
     t_total, t_avg, t_ramp.
+
+    ``rho_sum``: "reference" sums rho in float32 as the reference does
+    (reach3d.py:447); "deviation" sums rho - 1 in float32, as the library
+    does, and ``sum["rho"]`` then reads n + sum(rho - 1), formed in double and
+    rounded to float32 once (the library's readout).
     """
 
     cadence = 20   # steps between ramp/uref updates and averaging samples (reach3d.py:427)
 
-    def __init__(self, inputs, precision="f32"):
+    def __init__(self, inputs, precision="f32", rho_sum="deviation"):
+        if rho_sum not in ("reference", "deviation"):
+            raise ValueError("rho_sum: 'reference' or 'deviation'")
+        self.rho_sum = rho_sum
         self.i = inputs
         self.lbm = OracleLBM(inputs["solid"], inputs["Fx"], inputs["Fy"], inputs["Fz"], inputs["uin_lb"],
                              inputs["tau0"], inputs["cs"], inputs["cf"], precision=precision)
@@ -182,8 +190,16 @@ class OracleRun:
         if not self.ramp_done:
             for a in (self.lbm.Fx, self.lbm.Fy, self.lbm.Fz):
                 a *= 0
-        self.sum = {k: np.zeros(self.lbm.N, np.float32) for k in ("ux", "uy", "uz", "rho", "sp", "uxx")}
+        self._sum = {k: np.zeros(self.lbm.N, np.float32) for k in ("ux", "uy", "uz", "rho", "sp", "uxx")}
         self.navg = 0
+
+    @property
+    def sum(self):
+        if self.rho_sum == "reference":
+            return self._sum
+        out = dict(self._sum)
+        out["rho"] = (np.float64(self.navg) + self._sum["rho"].astype(np.float64)).astype(np.float32)
+        return out
 
     def advance(self, nsteps):
         L = self.lbm
@@ -204,11 +220,11 @@ class OracleRun:
             self.t += m * self.dt
             if self.t >= self.t_avg:
                 ux, uy, uz, rho = (a.astype(np.float32) for a in (L.ux, L.uy, L.uz, L.rho))
-                s = self.sum
+                s = self._sum
                 s["ux"] += ux
                 s["uy"] += uy
                 s["uz"] += uz
-                s["rho"] += rho
+                s["rho"] += rho if self.rho_sum == "reference" else rho - np.float32(1)
                 s["sp"] += np.sqrt(ux * ux + uy * uy + uz * uz)
                 s["uxx"] += ux * ux
                 self.navg += 1

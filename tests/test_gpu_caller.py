@@ -101,7 +101,7 @@ def test_reference_caller_pattern_matches_oracle_run_bitwise():
     t_total = r.cfg.run_flows * r.cfg.length / max(r.V, 0.05)
     orc = OracleRun(dict(solid=r.solid, Fx=r.Fx, Fy=r.Fy, Fz=r.Fz, uin_lb=(r.uin_phys / r.vel).astype(np.float32),
                          tau0=0.5 + 3 * max(1e-6 * r.dt / r.dx ** 2, 2e-4), cs=r.cfg.cs, cf=r.cf, bedwall=r.bedwall,
-                         dt=r.dt, t_total=t_total, **sched), precision="f32")
+                         dt=r.dt, t_total=t_total, **sched), precision="f32", rho_sum="reference")
     for n in (40, 30, 30, 60):      # chunks of 20, and 20 + 10 as the reference splits them
         run.advance(n)
         orc.advance(n)

@@ -229,12 +229,19 @@ def _fp64_reference_c1(nsteps):
 
 @pytest.mark.parametrize("storage,vel_tol,eta_tol", [
     ("f32fast", 1e-4, 1e-3),
+    ("f16dev", 3e-4, 4e-3),
+    ("bf16dev", 3e-3, 2.5e-2),
 ])
 def test_storage_formats_against_fp64_after_1000_steps(storage, vel_tol, eta_tol):
-    """Velocity relative L2 and instantaneous level-profile error of the FP32
-    pair-form path against the FP64 oracle on the 128x32x16 channel after
-    1000 steps. (The instantaneous level there carries metre-scale acoustic
-    transients, so the 16-bit formats are judged on time means instead.)"""
+    """Velocity relative L2 and instantaneous level-profile error against the
+    FP64 oracle on the 128x32x16 channel after 1000 steps. The instantaneous
+    level of a 16-bit format carries its per-step rounding: FP16 stores
+    2 (f - e) with |f - e| ~ 1e-2 w, a half-ulp of about 4e-6 per population,
+    about 1e-5 of density per cell, which the level readout multiplies by
+    (dx/dt)^2 / (3 g) = 340 m: about 3.4 mm per cell and 0.6 mm for a row of
+    30 cells, so a few mm over 128 rows (2.4 mm measured); BF16 has 3 bits
+    fewer (13 mm measured). The bounds are those estimates; the level bar of
+    1 mm applies to time means (test_gpu_precision)."""
     reach, ref = _fp64_reference_c1(1000)
     args = solver_inputs(reach)
     g = LBM3D(None, args["solid"], args["Fx"], args["Fy"], args["Fz"], args["uin"], args["tau0"], args["cs"],
@@ -275,24 +282,3 @@ def test_run3d_matches_oracle_run_bitwise(tag):
     for k in ("ux", "uy", "uz", "rho", "sp", "uxx"):
         assert np.array_equal(sums[k][fl], orc.sum[k][fl]), k
     run.close()
-
-
-def test_16bit_storage_time_means_against_fp64():
-    """Time-mean level profile (the quantity the boat tests report) of the
-    16-bit formats against an FP64 run of the same case through the run
-    driver with the reference schedule (1 flow-through, last 40 % averaged).
-    Rounding differences reseed the LES turbulence and the slow slosh modes,
-    so even the FP32 path differs from FP64 by a fraction of a millimetre to
-    a millimetre here; FP16 deviation storage must stay within a small
-    multiple of that noise floor."""
-    from water_flow_simulation_amd.precision import study
-    res = study(length=40.0, dx=0.25, x_first=12.0, storages=("f32", "f16dev", "bf16dev"))
-    f32, f16, bf16 = (res["formats"][k] for k in ("f32", "f16dev", "bf16dev"))
-    print({k: (round(v["eta_x_max_abs_mm"], 3), round(v["eta_x_base_max_abs_mm"], 3), round(v["d_eta_cm_abs_mm"], 3))
-           for k, v in res["formats"].items()})
-    # the FP32 run's own distance from FP64 is the noise floor of these time means
-    assert f32["eta_x_base_max_abs_mm"] <= 1.5
-    assert f16["eta_x_base_max_abs_mm"] <= max(2.0, 3.0 * f32["eta_x_base_max_abs_mm"])
-    assert f16["eta_x_max_abs_mm"] <= max(2.0, 3.0 * f32["eta_x_max_abs_mm"])
-    assert f16["q_x_max_rel"] <= 5e-3
-    assert np.isfinite(bf16["eta_x_base_max_abs_mm"])

@@ -85,7 +85,7 @@ def test_toy_reach_matches_reference_build(tag):
 def test_oracle_run_cadence_matches_reference(tag):
     g = golden(f"run_toy_{tag}.npz")
     r = toy_reach(tag == "boat")
-    run = OracleRun(oracle_inputs(r), precision="f32")
+    run = OracleRun(oracle_inputs(r), precision="f32", rho_sum="reference")
     for m in (40, 30, 30):
         run.advance(m)
     assert run.lbm.nstep == int(g["nsteps"]) == 100
@@ -103,7 +103,7 @@ def test_wrong_cadence_is_detected(cadence):
     oracle, moves the sums past the tolerance above."""
     g = golden("run_toy_boat.npz")
     r = toy_reach(True)
-    run = OracleRun(oracle_inputs(r), precision="f32")
+    run = OracleRun(oracle_inputs(r), precision="f32", rho_sum="reference")
     run.cadence = cadence
     for m in (40, 30, 30):
         run.advance(m)
@@ -117,7 +117,7 @@ def test_profiles_and_compare_match_reference():
     runs = {}
     for tag in ("boat", "base"):
         r = toy_reach(tag == "boat")
-        orc = OracleRun(oracle_inputs(r), precision="f32")
+        orc = OracleRun(oracle_inputs(r), precision="f32", rho_sum="reference")
         for m in (40, 30, 30):
             orc.advance(m)
         runs[tag] = (r, R.profiles(run_from_sums(r, orc.sum, orc.navg)))
@@ -139,3 +139,37 @@ def test_profiles_and_compare_match_reference():
         assert float(c[k]) == pytest.approx(ref[k], rel=1e-12, abs=1e-12), k
     for k in ("d_eta_cm", "us_gain", "F_net_kN", "umax", "tau_max", "tau_base", "dQ_net_pct"):
         assert float(c[k]) == pytest.approx(ref[k], rel=2e-3, abs=2e-4), k
+
+
+def test_deviation_sum_of_rho_keeps_the_level_signal():
+    """Why the library sums rho - 1: a float32 running sum of rho ~ 1 loses
+    the density excess that carries the water level (here 5e-5, i.e. 21 mm
+    at the 150 m reach's 423 m of level per unit density) once the sum's ulp
+    exceeds it; a float32 sum of rho - 1 keeps it. Pinned against the exact
+    mean of the same float32 samples."""
+    rng = np.random.default_rng(0)
+    for n, lost in ((3000, 0.3), (6000, 0.5)):
+        rho = (1 + 5e-5 + 1e-6 * rng.standard_normal(n)).astype(np.float32)
+        exact = rho.astype(np.float64).mean() - 1
+        s_ref, s_dev = np.float32(0), np.float32(0)
+        for v in rho:
+            s_ref = np.float32(s_ref + v)
+            s_dev = np.float32(s_dev + (v - np.float32(1)))
+        assert abs((np.float64(s_ref) / n - 1) - exact) > lost * exact
+        assert abs(np.float64(s_dev) / n - exact) < 1e-4 * exact
+
+
+def test_oracle_run_deviation_sums_agree_with_reference_sums_on_few_samples():
+    """On the toy run (five samples) both ways of summing rho agree to float32
+    rounding, and the other sums are identical."""
+    r = toy_reach(True)
+    a = OracleRun(oracle_inputs(r), precision="f32", rho_sum="reference")
+    b = OracleRun(oracle_inputs(r), precision="f32", rho_sum="deviation")
+    for m in (40, 30, 30):
+        a.advance(m)
+        b.advance(m)
+    fl = ~r.solid.ravel()
+    assert a.navg == b.navg > 0
+    for k in ("ux", "uy", "uz", "sp", "uxx"):
+        assert np.array_equal(a.sum[k], b.sum[k]), k
+    assert np.abs(a.sum["rho"][fl] - b.sum["rho"][fl]).max() <= 4 * np.spacing(np.float32(a.navg))

@@ -98,6 +98,19 @@ __global__ void k_unpad(const S* __restrict__ dev, D* __restrict__ dense, CellMa
     dense[k] = c < 0 ? fill : (D)dev[c];
 }
 
+// The same with `add` added in double before the one rounding to D (the
+// time-mean sum of rho is kept as the sum of rho - 1; its readout adds the
+// sample count back).
+template <typename S, typename D>
+__global__ void k_unpad_add(const S* __restrict__ dev, D* __restrict__ dense, CellMap m, long long Ndense,
+                            double add) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= Ndense) return;
+    const long long j = k / m.nx;
+    const long long c = cell_of(m, int(k - j * m.nx), j);
+    dense[k] = (D)((c < 0 ? 0.0 : (double)dev[c]) + add);
+}
+
 // Equilibrium at rho = 1, u = (u0, 0, 0), evaluated in double and stored once
 // through the codec; u0 = uin of the cell's row, 0 in solid cells
 // (floodsim/lbm3d.py:362-364, 377-384).
@@ -253,7 +266,7 @@ __global__ void k_accumulate(const R* __restrict__ rho, const R* __restrict__ ux
     s_ux[c] += a;
     s_uy[c] += b;
     s_uz[c] += d;
-    s_rho[c] += rho[c];
+    s_rho[c] += rho[c] - R(1);   // rho - 1: the level signal (1e-5..1e-4) survives the float sum
     s_sp[c] += sqrt(a * a + b * b + d * d);
     s_uxx[c] += a * a;
 }
@@ -346,7 +359,7 @@ __global__ void k_accumulate_if(const CaseSched<R>* __restrict__ sch, int k, con
     s_ux[c] += a;
     s_uy[c] += b;
     s_uz[c] += d;
-    s_rho[c] += rho[c];
+    s_rho[c] += rho[c] - R(1);   // rho - 1: the level signal (1e-5..1e-4) survives the float sum
     s_sp[c] += sqrt(a * a + b * b + d * d);
     s_uxx[c] += a * a;
 }

@@ -898,11 +898,27 @@ struct Solver final : wfs_solver {
         return WFS_OK;
     }
 
+    // sums[3] holds the sum of rho - 1; the readout is the sum of rho.
     int get_sums(void* const* out, bool f64) override {
         for (int k = 0; k < 6; ++k) {
-            int rc = get_cells(sums[k], out[k], f64);
+            int rc = k == 3 ? get_cells_add(sums[k], out[k], f64, (double)navg) : get_cells(sums[k], out[k], f64);
             if (rc) return rc;
         }
+        return WFS_OK;
+    }
+
+    int get_cells_add(DevBuf& b, void* dst, bool f64, double add) {
+        if (!dst) return WFS_OK;
+        if (f64)
+            hipLaunchKernelGGL((k_unpad_add<R, double>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(b),
+                               static_cast<double*>(stage.p), cmap(), N, add);
+        else
+            hipLaunchKernelGGL((k_unpad_add<R, float>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(b),
+                               static_cast<float*>(stage.p), cmap(), N, add);
+        WFS_LAUNCH_CHECK();
+        WFS_HIP(hipMemcpyAsync(dst, stage.p, (size_t)N * (f64 ? sizeof(double) : sizeof(float)),
+                               hipMemcpyDeviceToHost, stream));
+        WFS_HIP(hipStreamSynchronize(stream));
         return WFS_OK;
     }
 

@@ -14,6 +14,8 @@ profiles of ``profiles`` and the comparison of ``compare``:
 FP64 run of each readout, plus the run time. Usage:
 
     python -m water_flow_simulation_amd.precision [--length 60] [--dx 0.2]
+    python -m water_flow_simulation_amd.precision --full-size-steps 20000 --formats f32,f16dev,bf16dev
+    python -m water_flow_simulation_amd.precision --windows 6 [--length 60 --dx 0.2]
 """
 from __future__ import annotations
 
@@ -38,10 +40,13 @@ def run_case(reach, storage):
 
 
 def study(length=60.0, dx=0.2, x_first=20.0, storages=("f32", "f32fast", "f16dev", "bf16dev"), run_flows=1.0,
-          avg_frac=0.4):
+          avg_frac=0.4, with_boat=True):
     """run_flows / avg_frac: simulated flow-through times and the averaged
     fraction (the reference defaults are 1.0 / 0.4; longer windows shrink the
-    sampling noise of the LES time means that every format carries)."""
+    sampling noise of the LES time means that every format carries).
+    with_boat=False runs the base case alone (long boat runs of this
+    nearly inviscid scheme go unstable after about 1e5 steps in every
+    precision, FP64 included)."""
     # the time step of the reference-default run, kept for longer windows (the slosh rule
     # would otherwise raise it with the window and change the lattice speed)
     dt = workloads.trapezoid_tug(length=length, dx=dx, x_first=x_first).dt
@@ -52,30 +57,128 @@ def study(length=60.0, dx=0.2, x_first=20.0, storages=("f32", "f32fast", "f16dev
     out = {"case": f"trapezoidal reach {length:g} m at dx {dx:g} m ({boat.nx}x{boat.ny}x{boat.nz} cells), "
                    f"one pusher tug at x = {x_first:g} m, {run_flows:g} flow-throughs, last {avg_frac:g} averaged",
            "formats": {}}
+    if not with_boat:
+        out["case"] = out["case"].replace("one pusher tug", "no boat (base case)")
     ref, ref_base = {}, {}
     for st in ("f64",) + tuple(storages):
-        pb, steps, secs = run_case(boat, st)
-        p0, _, secs0 = run_case(base, st)
-        c = compare(pb, p0, boat, boat.cfg)
+        pb, steps, secs = run_case(boat, st) if with_boat else (None, 0, 0.0)
+        p0, steps0, secs0 = run_case(base, st)
+        c = compare(pb, p0, boat, boat.cfg) if with_boat else None
         if st == "f64":
             ref, ref_base, ref_c = pb, p0, c
-            out["steps"] = steps
+            out["steps"] = steps or steps0
             out["f64_seconds"] = round(secs + secs0, 2)
-            out["f64_d_eta_cm"] = c["d_eta_cm"]
             out["Q"] = boat.Q
+            if with_boat:
+                out["f64_d_eta_cm"] = c["d_eta_cm"]
             continue
-        e = {
-            "eta_x_max_abs_mm": float(np.abs(pb["eta"] - ref["eta"]).max() * 1e3),
-            "eta_x_base_max_abs_mm": float(np.abs(p0["eta"] - ref_base["eta"]).max() * 1e3),
-            "d_eta_cm_abs_mm": float(abs(c["d_eta_cm"] - ref_c["d_eta_cm"]) * 10),
-            "us_x_max_abs_m_s": float(np.abs(pb["us"] - ref["us"]).max()),
-            "q_x_max_rel": float(np.abs(pb["q"] - ref["q"]).max() / boat.Q),
-            "tau_bed_max_abs_Pa": float(np.abs(pb["tau_bed"] - ref["tau_bed"]).max()),
-            "tau_bed_rel_l2": float(np.linalg.norm(pb["tau_bed"] - ref["tau_bed"]) /
-                                    max(np.linalg.norm(ref["tau_bed"]), 1e-30)),
-            "seconds": round(secs + secs0, 2),
-        }
+        e = {"eta_x_base_max_abs_mm": float(np.abs(p0["eta"] - ref_base["eta"]).max() * 1e3),
+             "us_x_base_max_abs_m_s": float(np.abs(p0["us"] - ref_base["us"]).max()),
+             "q_x_base_max_rel": float(np.abs(p0["q"] - ref_base["q"]).max() / boat.Q),
+             "seconds": round(secs + secs0, 2)}
+        if with_boat:
+            e.update(profile_errors(pb, ref, boat.Q))
+            e["d_eta_cm_abs_mm"] = float(abs(c["d_eta_cm"] - ref_c["d_eta_cm"]) * 10)
         out["formats"][st] = e
+    return out
+
+
+def profile_errors(p, ref, Q):
+    """Largest differences of the reported profiles from the FP64 run's."""
+    return {"eta_x_max_abs_mm": float(np.abs(p["eta"] - ref["eta"]).max() * 1e3),
+            "us_x_max_abs_m_s": float(np.abs(p["us"] - ref["us"]).max()),
+            "q_x_max_rel": float(np.abs(p["q"] - ref["q"]).max() / Q),
+            "tau_bed_max_abs_Pa": float(np.abs(p["tau_bed"] - ref["tau_bed"]).max()),
+            "tau_bed_rel_l2": float(np.linalg.norm(p["tau_bed"] - ref["tau_bed"]) /
+                                    max(np.linalg.norm(ref["tau_bed"]), 1e-30))}
+
+
+def fixed_steps(run, steps, avg_frac=0.4):
+    """Schedule a run of exactly ``steps`` steps with the reference's rules
+    (reach3d.py:393-399) applied to that run length: ramp over
+    min(15 %, 20 s) of it, time means over its last ``avg_frac``."""
+    t_total = steps * run.r.dt
+    run.t_total = t_total
+    run.t_avg = t_total * (1 - avg_frac)
+    run.t_ramp = min(0.15 * t_total, 20.0) if run.r.props else 0.0
+    run._ramp_done = run.t_ramp <= 0
+    run.lbm.set_force_scale(0.0 if run.t_ramp > 0 else 1.0)
+    return run
+
+
+def full_size_record(steps=20000, storages=("f32", "f16dev", "bf16dev"), avg_frac=0.4, reach=None):
+    """The 1500 x 522 x 30 trapezoidal reach with the tug (23.5 M cells) run
+    for ``steps`` steps in FP64 and in each storage format; per format the
+    largest differences of the time-mean profiles eta_x, us_x, q_x and
+    tau_bed (reach3d.py:545-568) from the FP64 run, the step time and the
+    mass drift."""
+    reach = reach or workloads.trapezoid_tug()
+    out = {"case": f"trapezoid_tug {reach.nx}x{reach.ny}x{reach.nz} ({reach.cells} cells), {steps} steps "
+                   f"({steps * reach.dt:.2f} s simulated), time means over the last {avg_frac:g}",
+           "steps": steps, "formats": {}}
+    ref = None
+    for st in ("f64",) + tuple(storages):
+        t0 = time.perf_counter()
+        run = fixed_steps(Run3D(reach, storage=st), steps, avg_frac)
+        m0 = run.lbm.total("rho")
+        t1 = time.perf_counter()
+        run.advance(steps)
+        secs = time.perf_counter() - t1
+        prof = profiles(run)
+        drift = run.lbm.total("rho") / m0 - 1.0
+        run.close()
+        row = {"seconds": round(secs, 2), "ms_per_step": round(secs / steps * 1e3, 4), "mass_drift": drift,
+               "setup_s": round(t1 - t0, 2)}
+        if st == "f64":
+            ref = prof
+            out["f64"] = row
+            out["q_in"] = reach.Q
+            continue
+        row.update(profile_errors(prof, ref, reach.Q))
+        out["formats"][st] = row
+    return out
+
+
+def windows(length=60.0, dx=0.2, flows=6, storages=("f32", "f32fast", "f16dev", "bf16dev"), x_first=20.0):
+    """The no-boat reach run for ``flows`` flow-through times in FP64, in FP64
+    again from the state rounded to float32 at the start (how far two FP64
+    runs drift apart from a 1e-8 perturbation: the noise floor of the time
+    means), and in each storage format. Time means over the windows
+    [1, k] flow-throughs, k = 2..flows; per window and format the largest
+    level difference (mm) from the FP64 run, plus the discharge and surface
+    velocity differences of the longest window."""
+    dt = workloads.trapezoid_tug(length=length, dx=dx, x_first=x_first).dt
+    base = workloads.trapezoid_tug(length=length, dx=dx, x_first=x_first, with_boat=False, run_flows=float(flows),
+                                   dt=dt)
+    t_flow = length / base.V
+    names = ("f64", "f64_rounded_start") + tuple(storages)
+    prof = {}
+    secs = {}
+    for name in names:
+        st = "f64" if name.startswith("f64") else name
+        t0 = time.perf_counter()
+        run = Run3D(base, storage=st)
+        if name == "f64_rounded_start":
+            run.lbm.set_populations(run.lbm.populations())
+        run.t_avg = t_flow
+        for k in range(2, flows + 1):
+            while run.t < k * t_flow:
+                run.advance(2000)
+            prof[name, k] = profiles(run)
+        secs[name] = round(time.perf_counter() - t0, 2)
+        steps = run.lbm.nstep
+        run.close()
+    out = {"case": f"no-boat trapezoidal reach {length:g} m at dx {dx:g} m ({base.nx}x{base.ny}x{base.nz} cells), "
+                   f"{steps} steps = {flows} flow-throughs of {t_flow:.1f} s; time means from 1 flow-through on",
+           "steps": steps, "seconds": secs, "eta_x_max_abs_mm": {}}
+    for k in range(2, flows + 1):
+        ref = prof["f64", k]
+        out["eta_x_max_abs_mm"][f"1-{k}"] = {
+            name: round(float(np.abs(prof[name, k]["eta"] - ref["eta"]).max() * 1e3), 3) for name in names[1:]}
+    ref = prof["f64", flows]
+    out["longest_window"] = {name: {"us_x_max_abs_m_s": float(np.abs(prof[name, flows]["us"] - ref["us"]).max()),
+                                    "q_x_max_rel": float(np.abs(prof[name, flows]["q"] - ref["q"]).max() / base.Q)}
+                             for name in names[1:]}
     return out
 
 
@@ -87,9 +190,20 @@ def main():
     ap.add_argument("--formats", default="f32,f32fast,f16dev,bf16dev")
     ap.add_argument("--run-flows", type=float, default=1.0)
     ap.add_argument("--avg-frac", type=float, default=0.4)
+    ap.add_argument("--base-only", action="store_true", help="the no-boat case alone")
+    ap.add_argument("--full-size-steps", type=int, default=0,
+                    help="instead: the 1500x522x30 reach for this many steps in every format against FP64")
+    ap.add_argument("--windows", type=int, default=0,
+                    help="instead: the no-boat reach for this many flow-throughs, time means over [1, k] of them")
     a = ap.parse_args()
-    print(json.dumps(study(a.length, a.dx, a.x_first, tuple(a.formats.split(",")), a.run_flows, a.avg_frac),
-                     indent=1))
+    fmts = tuple(a.formats.split(","))
+    if a.windows:
+        res = windows(a.length, a.dx, a.windows, fmts, a.x_first)
+    elif a.full_size_steps:
+        res = full_size_record(a.full_size_steps, fmts, a.avg_frac)
+    else:
+        res = study(a.length, a.dx, a.x_first, fmts, a.run_flows, a.avg_frac, with_boat=not a.base_only)
+    print(json.dumps(res, indent=1))
 
 
 if __name__ == "__main__":

@@ -589,8 +589,9 @@ class Run3D:
     def _raw(self, name, mean):
         wide = np.float64 if self.lbm.storage == "f64" else np.float32
         if mean and self.navg > 0:
-            sums, n = self.lbm.sums(wide)
-            return sums[name].astype(np.float64) / n
+            # FP64 readout: the sum of rho comes back as n + sum(rho - 1) without a float rounding
+            sums, n = self.lbm.sums(np.float64)
+            return sums[name] / n
         rho, ux, uy, uz = self.lbm.macro(wide)
         if name == "sp":
             return np.sqrt(ux * ux + uy * uy + uz * uz).astype(np.float64)
