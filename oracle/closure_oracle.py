@@ -262,6 +262,17 @@ def moments(lbm):
     return mom.reshape(10, lbm.N)
 
 
+OUTLET_SPONGE = 64   # cells before the outlet row where C^2 >= Cs^2
+
+
+def outlet_sponge(c2, nx, ny, nz, csmag2):
+    """C^2 of the learned closure with the outlet sponge: in the last
+    OUTLET_SPONGE cells along x (x >= nx - OUTLET_SPONGE) at least the
+    constant model's Cs^2 as the float kernel holds it."""
+    x = np.tile(np.arange(nx), ny * nz)
+    return np.where(x >= nx - OUTLET_SPONGE, np.maximum(c2, float(np.float32(csmag2))), c2)
+
+
 def learned_step(lbm, p, geo, hcol, inv_u0):
     """One step of the FP64 oracle (precision "f64") with the learned closure:
     moments, features, network, then the step with the per-cell C^2 field and
@@ -273,7 +284,7 @@ def learned_step(lbm, p, geo, hcol, inv_u0):
     hc = np.broadcast_to(np.asarray(hcol).reshape(1, lbm.ny, lbm.nx), (lbm.nz, lbm.ny, lbm.nx)).ravel()
     X = features(S, u, np.asarray(geo).ravel(), hc, inv_u0)
     y, _ = forward(p, X)
-    c2 = np.ascontiguousarray(c2_of(y))
+    c2 = np.ascontiguousarray(outlet_sponge(c2_of(y), lbm.nx, lbm.ny, lbm.nz, lbm.csmag2))
     _step_field(lbm, lbm.f2, c2, None, lbm.rho, lbm.ux, lbm.uy, lbm.uz)
     _orc.lib().orc_bc_f64(_orc._ptr(lbm.f2), _orc._ptr(lbm.solid), _orc._ptr(lbm.uin), lbm.nx, lbm.ny, lbm.nz, 1.0)
     lbm.f, lbm.f2 = lbm.f2, lbm.f

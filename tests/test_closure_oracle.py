@@ -231,9 +231,10 @@ def test_constant_network_step_is_the_smagorinsky_step():
     geo, hcol = CL.geometry(r.solid)
     u0 = float(kw["uin"][kw["uin"] > 0].mean())
     p = CL.smagorinsky_params(r.cfg.cs)
-    # the plain step with exactly the network's constant (cs^2 up to the float32 output bias)
+    p[-1] += np.float32(1e-3)     # C^2 just above Cs^2, so the outlet sponge (C^2 >= Cs^2) changes nothing
+    # the plain step with exactly the network's constant
     a.csmag2 = float(C.c2_of(np.float64(p[-1])))
-    assert a.csmag2 == pytest.approx(r.cfg.cs ** 2, rel=1e-6)
+    assert a.csmag2 == pytest.approx(r.cfg.cs ** 2, rel=1e-3) and a.csmag2 > float(np.float32(r.cfg.cs ** 2))
     for _ in range(3):
         a.step(1)
         c2 = C.learned_step(b, p, geo, hcol, 1.0 / u0)
@@ -250,3 +251,25 @@ def test_constant_network_step_is_the_smagorinsky_step():
     y_ok, _ = C.forward(p2, X)
     y_bad, _ = C.forward(p2, np.roll(X, 1, axis=0))
     assert np.abs(C.c2_of(y_ok) - C.c2_of(y_bad))[fl].max() > 1e-4
+
+
+def test_outlet_sponge_floors_c2_before_the_outlet():
+    """A network at the C^2 floor: the oracle's learned step gives Cs^2 (as
+    float32) in the last 64 cells along x and the floor everywhere else,
+    checked cell by cell against the cell's x from its index."""
+    r = channel128()
+    kw = solver_inputs(r)
+    b = O.OracleLBM(precision="f64", **kw)
+    b.step(10)
+    geo, hcol = CL.geometry(r.solid)
+    u0 = float(kw["uin"][kw["uin"] > 0].mean())
+    p = CL.smagorinsky_params(r.cfg.cs)
+    p[-1] = -40.0                                  # softplus(-40) ~ 4e-18: C^2 = 0.01
+    c2 = C.learned_step(b, p, geo, hcol, 1.0 / u0)
+    cs2 = float(np.float32(r.cfg.cs ** 2))
+    floor = float(C.c2_of(np.float64(-40.0)))
+    assert floor < 0.0101
+    for k in range(0, c2.size, 7):
+        x = k % r.nx
+        assert c2[k] == (cs2 if x >= r.nx - 64 else floor), k
+    assert C.OUTLET_SPONGE == 64

@@ -60,3 +60,20 @@ def test_run_test_cancel_stops_all_runs():
     res, seen = _run(True, cancel=cancel)
     assert len(res) == 3
     assert all(n == 800 for _, n in seen)          # two 400-step segments, then the stop
+
+
+def test_run_test_carries_fp64_errors():
+    """With fp64_errors every result carries |value - FP64 value| for each
+    reported quantity; the FP64 values are those of the same plan run with
+    FP64 storage on its own."""
+    print("case: boat test, FP16 storage with its FP64 twin", flush=True)
+    from water_flow_simulation_amd.boat_tests import FP64_ERROR_KEYS
+    res, _ = _run(True, storage="f16dev", fp64_errors=True)
+    ref, _ = _run(True, storage="f64")
+    for c, c64 in zip(res, ref):
+        assert set(FP64_ERROR_KEYS) <= set(c["err_fp64"])
+        for k in FP64_ERROR_KEYS:
+            assert c["fp64"][k] == c64[k] or (np.isnan(c["fp64"][k]) and np.isnan(c64[k])), k
+            assert c["err_fp64"][k] == abs(float(c[k]) - float(c64[k])), k
+        for k in ("eta_x_max_mm", "us_x_max", "q_x_max_rel", "tau_bed_max"):
+            assert np.isfinite(c["err_fp64"][k]) and c["err_fp64"][k] >= 0, k

@@ -145,3 +145,21 @@ def test_vtk_layout(tmp_path):
     assert np.allclose(sp, np.sqrt((vec.astype(np.float64) ** 2).sum(-1)), rtol=1e-6)
     assert np.array_equal(sol.astype(bool), r.solid)
 
+
+
+def test_results_csv_appends_fp64_error_columns(tmp_path):
+    """Results that carry FP64 errors keep the reference's columns first and
+    add err_fp64_<quantity> columns after them, in the order of the reported
+    quantities."""
+    from water_flow_simulation_amd.boat_tests import FP64_ERROR_KEYS
+    rows = list(csv.DictReader(io.StringIO(FIX["csv"]["text"])))[:2]
+    results = [{k: (v if k == "label" else float(v)) for k, v in row.items()} for row in rows]
+    results[0]["err_fp64"] = {k: 0.001 * (i + 1) for i, k in enumerate(FP64_ERROR_KEYS)}
+    results[0]["err_fp64"]["eta_x_max_mm"] = 0.5
+    p = tmp_path / "boat_results.csv"
+    RS.write_results_csv(p, results)
+    out = list(csv.DictReader(io.StringIO(p.read_text(encoding="utf-8-sig"))))
+    cols = list(out[0].keys())
+    assert cols[:len(RS.RESULT_COLUMNS)] == RS.RESULT_COLUMNS
+    assert cols[len(RS.RESULT_COLUMNS):] == [f"err_fp64_{k}" for k in FP64_ERROR_KEYS] + ["err_fp64_eta_x_max_mm"]
+    assert float(out[0]["err_fp64_d_eta_cm"]) == 0.001 and out[1]["err_fp64_d_eta_cm"] == ""

@@ -17,6 +17,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass, field, replace
 
+import numpy as np
+
 from .reach3d import Fleet, ReachConfig, Run3D, boat_type, build_reach, compare, profiles
 
 BOAT_TESTS = [
@@ -142,8 +144,26 @@ def simulate(reaches, be=None, storage="f32", batched=True, progress=None, cance
     return runs
 
 
+# The reported quantities that get an FP64 error: water-level change, surface
+# velocity, bed shear, discharge and the momentum budget (compare() keys).
+FP64_ERROR_KEYS = ("d_eta_cm", "us_gain", "us_gain_pct", "umax", "tau_max", "tau_area", "jet_reach", "F_net_kN",
+                   "dQ_net_pct", "dQ_per_100kW", "drop_cm_per_100kW")
+
+
+def fp64_error(c, c64, prof, prof64, Q):
+    """|value - FP64 value| for every reported quantity of a comparison,
+    and the largest differences of its profiles from the FP64 run's."""
+    err = {k: abs(float(c[k]) - float(c64[k])) for k in FP64_ERROR_KEYS}
+    err["eta_x_max_mm"] = float(np.abs(prof["eta"] - prof64["eta"]).max() * 1e3)
+    err["us_x_max"] = float(np.abs(prof["us"] - prof64["us"]).max())
+    err["q_x_max_rel"] = float(np.abs(prof["q"] - prof64["q"]).max() / Q)
+    err["tau_bed_max"] = float(np.abs(prof["tau_bed"] - prof64["tau_bed"]).max())
+    return err
+
+
 def run_test(test, river, sec, levels, h_sel, cfg: ReachConfig, be, flow_fn, x_first=40.0,
-             progress=None, cancel=None, on_result=None, boat_edits=None, storage="f32", batched=True, group=0):
+             progress=None, cancel=None, on_result=None, boat_edits=None, storage="f32", batched=True, group=0,
+             fp64_errors=False):
     """One boat test, with the reference's signature and result records: a
     list with one dict per variant, ``compare`` of the variant against the
     base at its level plus 'label', 'profiles', 'base_profiles' and 'notes'
@@ -151,19 +171,40 @@ def run_test(test, river, sec, levels, h_sel, cfg: ReachConfig, be, flow_fn, x_f
     fraction done; after ``cancel()`` turns true the runs stop where they
     are and the results are read from that state. ``on_result(result,
     run)`` is called per variant while its run is still open. ``river`` is
-    accepted for signature compatibility."""
+    accepted for signature compatibility.
+
+    fp64_errors: the same plan runs a second time with FP64 storage, and
+    every result carries 'err_fp64', the absolute difference of each
+    reported quantity (FP64_ERROR_KEYS) and of the level, surface-velocity,
+    discharge and bed-shear profiles from that FP64 run, and 'fp64', the
+    FP64 run's comparison."""
     if cancel and cancel():
         return []
     plan = plan_test(test, sec, levels, h_sel, cfg, flow_fn, x_first, boat_edits)
+    twin = fp64_errors and storage != "f64"
     tag = f"{len(plan.cases)} runs" + (" together" if batched else "")
-    prog = (lambda f: progress(f, tag)) if progress else None
+    if progress:
+        prog = (lambda f: progress(0.5 * f, tag)) if twin else (lambda f: progress(f, tag))
+    else:
+        prog = None
     runs = simulate(plan.reaches, be, storage, batched, prog, cancel, group)
     try:
         prof = [profiles(run) for run in runs]
+        prof64 = prof
+        if twin:
+            runs64 = simulate(plan.reaches, be, "f64", batched,
+                              (lambda f: progress(0.5 + 0.5 * f, tag + " (FP64)")) if progress else None, cancel, group)
+            prof64 = [profiles(run) for run in runs64]
+            for run in runs64:
+                run.close()
         results = []
         for k, (label, h, reach, kb) in plan.variants():
             c = compare(prof[k], prof[kb], reach, cfg)
             c.update(label=label, profiles=prof[k], base_profiles=prof[kb], notes=reach.notes)
+            if fp64_errors:
+                c64 = compare(prof64[k], prof64[kb], reach, cfg)
+                c["fp64"] = c64
+                c["err_fp64"] = fp64_error(c, c64, prof[k], prof64[k], reach.Q)
             results.append(c)
             if on_result:
                 on_result(c, runs[k])

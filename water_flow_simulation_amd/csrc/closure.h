@@ -49,6 +49,15 @@ constexpr int kNParams = kOffB3 + 1;   // 1377
 
 constexpr float kC2Min = 0.01f;     // floor of C^2 (Cs = 0.1): the near-inviscid lattice (tau0 ~ 0.5006) needs it
 constexpr float kC2Scale = 0.0289f; // C^2 per unit softplus (0.17^2)
+// Outlet sponge: in the last kOutletSponge cells before the outlet row
+// (x >= nx - kOutletSponge) the learned C^2 is at least the constant model's
+// Cs^2. The outlet fixes rho = 1 and copies the non-equilibrium part of
+// x = nx - 2 (floodsim/lbm3d.py:164-173); where that row sees backflow
+// (pressure waves in a closed-lid reach turn the outlet discharge negative),
+// a coefficient at the 0.01 floor let a disturbance there grow until the
+// 1 km meander went non-finite after 40,000 steps, which the constant model
+// damps. The sponge keeps the constant model's dissipation there.
+constexpr int kOutletSponge = 64;
 
 typedef short s4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));

@@ -339,6 +339,7 @@ struct StepArgs {
     const MlpPack* __restrict__ mlp;    // network weights in the wavefront layout
     float* __restrict__ c2out;          // C^2 per cell on write_all steps, when not null
     float inv_u0;                       // 1 / mean inlet speed (lattice units)
+    float cs2;                          // the constant model's Cs^2 (float): the C^2 floor of the outlet sponge
     // tile-sparse layout (k_step_sparse only): a tile is one 64-cell x-run of a
     // row; cell c = tile * 64 + lane
     const int* __restrict__ nbr;        // per tile: 27 neighbour tiles (dx+1) + 3 ((dy+1) + 3 (dz+1)), -1 none
@@ -1328,7 +1329,10 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     }
     const float y = mlp_wave(feat, P);
     if (active) {
-        const float c2 = kC2Min + kC2Scale * softplus_f(y);
+        float c2 = kC2Min + kC2Scale * softplus_f(y);
+        // outlet sponge: no less than the constant model's dissipation in the
+        // last kOutletSponge cells before the outlet row
+        if (x >= a.nx - kOutletSponge) c2 = fmaxf(c2, a.cs2);
         const R ks = R(18) * R(1.41421356) * c2;
         if (any_forced) pair_relax<R, true>(f, fx, fy, fz, m, a.tau0, a.tau0sq, ks);
         else pair_relax<R, false>(f, fx, fy, fz, m, a.tau0, a.tau0sq, ks);

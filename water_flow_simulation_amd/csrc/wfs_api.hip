@@ -554,6 +554,7 @@ struct Solver final : wfs_solver {
         a.mlp = static_cast<const MlpPack*>(mlp_pack.p);
         a.c2out = static_cast<float*>(c2buf.p);
         a.inv_u0 = inv_u0;
+        a.cs2 = (float)(d.cs * d.cs);
         a.nbr = static_cast<const int*>(nbr_d.p);
         a.tcoord = static_cast<const int*>(tcoord_d.p);
         a.blk2tile = static_cast<const int*>(blk2tile_d.p);

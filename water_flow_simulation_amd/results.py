@@ -41,12 +41,22 @@ BASE_PROFILE_DTYPES = {"x": np.float64, "eta": np.float64, "us": np.float64, "um
 
 
 def write_results_csv(path, results):
-    """One row per case result (compare() dicts), the reference's columns."""
+    """One row per case result (compare() dicts), the reference's columns.
+    Results that carry FP64 errors (run_test(fp64_errors=True)) add one
+    column per error after the reference's, named err_fp64_<quantity>."""
+    errs = sorted({k for c in results for k in c.get("err_fp64", {})}, key=_err_order)
     with open(path, "w", newline="", encoding="utf-8-sig") as f:
         w = csv.writer(f)
-        w.writerow(RESULT_COLUMNS)
+        w.writerow(RESULT_COLUMNS + [f"err_fp64_{k}" for k in errs])
         for c in results:
-            w.writerow([c[k] if isinstance(c[k], str) else f"{c[k]:.5g}" for k in RESULT_COLUMNS])
+            row = [c[k] if isinstance(c[k], str) else f"{c[k]:.5g}" for k in RESULT_COLUMNS]
+            e = c.get("err_fp64", {})
+            w.writerow(row + [f"{e[k]:.5g}" if k in e else "" for k in errs])
+
+
+def _err_order(k):
+    from .boat_tests import FP64_ERROR_KEYS
+    return (FP64_ERROR_KEYS.index(k), k) if k in FP64_ERROR_KEYS else (len(FP64_ERROR_KEYS), k)
 
 
 def write_cases_txt(path, results):
