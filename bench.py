@@ -25,6 +25,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import platform
 import re
 import sys
 import time
@@ -124,8 +125,32 @@ def cpu_baseline(reach, target_s=15.0):
     t0 = time.perf_counter()
     orc.step(n)
     dt = time.perf_counter() - t0
+    # the reference's own CPU-runnable case, whole: 1000 steps of the 128x32x16 channel with one boat
+    small = build_case("channel128")
+    orc1 = O.OracleLBM(*solver_args(small), precision="mixed")
+    t0 = time.perf_counter()
+    orc1.step(1000)
+    dt1 = time.perf_counter() - t0
     return dict(value=n * reach.cells / dt / 1e6, unit="MLUPS", cores=O.num_threads(), kind="port",
-                sample=f"{n} steps of the full {reach.nx}x{reach.ny}x{reach.nz} reach (all cells counted)")
+                sample=f"{n} steps of the full {reach.nx}x{reach.ny}x{reach.nz} reach (all cells counted)",
+                cpu=cpu_model(),
+                channel128={"value": round(1000 * small.cells / dt1 / 1e6, 1), "unit": "MLUPS", "steps": 1000,
+                            "wall_s": round(dt1, 3),
+                            "workload": "128x32x16 channel at dx 0.25 m, one boat (the reference's CPU case)"})
+
+
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo) and its logical CPU count."""
+    name = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name}, {os.cpu_count()} logical CPUs visible"
 
 
 def time_batched(cases, steps, seed=0, scale=1.0):
@@ -217,6 +242,14 @@ def main():
     log(f"building workload {a.workload}")
     reach = build_case(a.workload)
     n_fluid = int((~reach.solid).sum())
+    # the step's own access pattern without its arithmetic, on the reach's padded box
+    px = -(-reach.nx // 64) * 64
+    ceiling = {eb: _native.stream_probe(px, reach.ny, reach.nz, eb, 0, 20) for eb in (4, 2)}
+    log(f"19-stream pull ceiling {ceiling[4]:.0f} GB/s (4-byte), {ceiling[2]:.0f} GB/s (2-byte)")
+    elem = {"f32": 4, "f32fast": 4, "f16dev": 2, "bf16dev": 2}
+
+    def of_ceiling(st, gbs):
+        return round(gbs / ceiling[elem[st]], 4) if st in elem else None
     log(f"{reach.nx}x{reach.ny}x{reach.nz} = {reach.cells} cells ({n_fluid} fluid); timing {a.steps} steps")
     r = time_gpu(reach, a.storage, a.steps, a.warmup, a.block_x)
     log(f"timed: {r['wall']:.3f} s wall, step kernel {r['kernel_ms'] / max(r['launches'], 1):.4f} ms avg")
@@ -237,7 +270,9 @@ def main():
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_per_launch(),
                      "kernel": "k_step", "kernel_ms_avg": round(k_avg_s * 1e3, 5),
                      "bytes_alg_per_launch": B_ALG[a.storage] * n_fluid,
-                     "copy_bw_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
+                     "copy_bw_gbs": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4),
+                     "stream_ceiling_gbs": round(ceiling[elem[a.storage]], 1) if a.storage in elem else None,
+                     "frac_of_stream_ceiling": of_ceiling(a.storage, achieved)},
         "mass_fp64": r["mass"],
     }
     variants = {}
@@ -252,6 +287,7 @@ def main():
         variants[name] = {"mlups": round(a.steps * reach.cells / rv["wall"] / 1e6, 1),
                           "kernel_ms_avg": round(kv * 1e3, 5),
                           "roofline_frac": round(b_alg * n_fluid / kv / 1e9 / PEAK_HBM_GBS, 4),
+                          "frac_of_stream_ceiling": of_ceiling(st, b_alg * n_fluid / kv / 1e9),
                           "speedup_vs_f32": round(r["wall"] / rv["wall"], 3)}
     if variants:
         out["variants"] = variants

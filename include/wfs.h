@@ -322,6 +322,17 @@ int wfs_batch_time(wfs_batch* b, int32_t k, double* t, int32_t* navg);
  * used to calibrate hardware byte counters. */
 int wfs_bandwidth_probe(int32_t device, int64_t bytes, int32_t iters, double* gbs);
 
+/* Streaming ceiling of the lattice step's access pattern: the 19-population
+ * pull read and 19-population write of one step over a px x ny x nz box
+ * without the arithmetic (each population read from x - c_i, written with a
+ * non-temporal store), populations of elem_bytes (4: FP32, 2: the 16-bit
+ * formats), direction arrays stagger_bytes apart beyond their length.
+ * Reports 2 * 19 * elem_bytes * cells * iters / elapsed in GB/s: what any
+ * pull-stream step kernel with this layout can reach on the device. No
+ * reference counterpart (bench.py reports it beside the roofline). */
+int wfs_stream_probe(int32_t device, int32_t px, int32_t ny, int32_t nz, int32_t elem_bytes, int64_t stagger_bytes,
+                     int32_t iters, double* gbs);
+
 /* Device memory held by the solver, in bytes. */
 int64_t wfs_device_bytes(const wfs_solver* h);
 

@@ -83,6 +83,7 @@ _SIGS = {
     "wfs_batch_synchronize": ([_P], ctypes.c_int),
     "wfs_batch_time": ([_P, _I32, _P, _P], ctypes.c_int),
     "wfs_bandwidth_probe": ([_I32, ctypes.c_int64, _I32, _P], ctypes.c_int),
+    "wfs_stream_probe": ([_I32, _I32, _I32, _I32, _I32, ctypes.c_int64, _I32, _P], ctypes.c_int),
     "wfs_device_bytes": ([_P], ctypes.c_int64),
     "wfs_create_tiled": ([_P, _P, _P], ctypes.c_int),
     "wfs_stored_cells": ([_P], ctypes.c_int64),
@@ -170,6 +171,15 @@ def bandwidth_probe(nbytes=2 << 30, iters=20, device=0) -> float:
     """Device-to-device copy bandwidth in GB/s (see wfs_bandwidth_probe)."""
     out = ctypes.c_double(0.0)
     check(lib().wfs_bandwidth_probe(int(device), int(nbytes), int(iters), ctypes.byref(out)))
+    return float(out.value)
+
+
+def stream_probe(px, ny, nz, elem_bytes=4, stagger_bytes=0, iters=20, device=0) -> float:
+    """Streaming ceiling of the step's 19-population pull pattern over a
+    px x ny x nz box in GB/s (see wfs_stream_probe)."""
+    out = ctypes.c_double(0.0)
+    check(lib().wfs_stream_probe(int(device), int(px), int(ny), int(nz), int(elem_bytes), int(stagger_bytes),
+                                 int(iters), ctypes.byref(out)))
     return float(out.value)
 
 

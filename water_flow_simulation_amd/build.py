@@ -17,7 +17,6 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libwfs.so")
-PROBE = os.path.join(PKG, "ceiling_probe")
 
 SOURCES = ["wfs_api.hip"]
 HEADERS = ["lattice.h", "closure.h", "step_kernels.h", "aux_kernels.h"]
@@ -50,19 +49,7 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
-    build_probe(verbose)
     return LIB
-
-
-def build_probe(verbose: bool = False) -> str:
-    """The streaming-ceiling measurement program (csrc/ceiling_probe.hip)."""
-    flags = [f for f in HIPFLAGS if f not in ("-fPIC", "-shared")]
-    cmd = [hipcc(), *flags, os.path.join(CSRC, "ceiling_probe.hip"), "-o", PROBE + ".tmp"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(PROBE + ".tmp", PROBE)
-    return PROBE
 
 
 if __name__ == "__main__":

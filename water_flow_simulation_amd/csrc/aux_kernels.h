@@ -77,38 +77,47 @@ __global__ void k_aa_exchange(T* __restrict__ f, long long N, CellMap m) {
     });
 }
 
-// dense -> device cells; padding cells get `fill`.
+// Host exchanges move a block of whole dense rows [j0, j1) at a time; the
+// device cells of those rows are the range [c0, c1) (rows and, on the
+// tile-sparse layout, tiles are stored in row order).
+//
+// dense rows j0.. -> device cells [c0, c1); padding cells get `fill`.
 template <typename S, typename D>
-__global__ void k_pad(const S* __restrict__ dense, D* __restrict__ dev, CellMap m, long long Ndev, D fill) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= Ndev) return;
+__global__ void k_pad(const S* __restrict__ dense, D* __restrict__ dev, CellMap m, long long c0, long long c1,
+                      long long j0, D fill) {
+    const long long c = c0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= c1) return;
     int x;
     long long j;
     dense_of(m, c, x, j);
-    dev[c] = x < m.nx ? (D)dense[x + (long long)m.nx * j] : fill;
+    dev[c] = x < m.nx ? (D)dense[x + (long long)m.nx * (j - j0)] : fill;
 }
 
-// device cells -> dense; cells of tiles that are not stored get `fill`.
+// device cells -> dense cells [k0, k0 + n) (written from dense[0]); cells of
+// tiles that are not stored get `fill`.
 template <typename S, typename D>
-__global__ void k_unpad(const S* __restrict__ dev, D* __restrict__ dense, CellMap m, long long Ndense, D fill) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= Ndense) return;
+__global__ void k_unpad(const S* __restrict__ dev, D* __restrict__ dense, CellMap m, long long k0, long long n,
+                        D fill) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long k = k0 + i;
     const long long j = k / m.nx;
     const long long c = cell_of(m, int(k - j * m.nx), j);
-    dense[k] = c < 0 ? fill : (D)dev[c];
+    dense[i] = c < 0 ? fill : (D)dev[c];
 }
 
 // The same with `add` added in double before the one rounding to D (the
 // time-mean sum of rho is kept as the sum of rho - 1; its readout adds the
 // sample count back).
 template <typename S, typename D>
-__global__ void k_unpad_add(const S* __restrict__ dev, D* __restrict__ dense, CellMap m, long long Ndense,
+__global__ void k_unpad_add(const S* __restrict__ dev, D* __restrict__ dense, CellMap m, long long k0, long long n,
                             double add) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= Ndense) return;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long k = k0 + i;
     const long long j = k / m.nx;
     const long long c = cell_of(m, int(k - j * m.nx), j);
-    dense[k] = (D)((c < 0 ? 0.0 : (double)dev[c]) + add);
+    dense[i] = (D)((c < 0 ? 0.0 : (double)dev[c]) + add);
 }
 
 // Equilibrium at rho = 1, u = (u0, 0, 0), evaluated in double and stored once
@@ -138,20 +147,23 @@ __global__ void k_init(typename Codec::T* __restrict__ f, const uint8_t* __restr
 // outside the y range, below the bed (z < 0) or on a solid cell; sources
 // above the lid are mirrored into the same layer first
 // (floodsim/lbm3d.py:50-58 applies the same rules per step). Padding cells
-// (x >= nx) are flagged solid.
+// (x >= nx) are flagged solid. One call covers the device cells [c0, c1) of
+// whole z-planes; `solid` holds the dense planes from zlo on, one plane
+// either side of them included.
 template <typename R>
 __global__ void k_flags(const uint8_t* __restrict__ solid, const R* __restrict__ cfw, uint8_t* __restrict__ flag,
-                        uint32_t* __restrict__ wmask, CellMap m, int nz, long long Ndev) {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= Ndev) return;
+                        uint32_t* __restrict__ wmask, CellMap m, int nz, long long c0, long long c1, int zlo) {
+    const long long c = c0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= c1) return;
     int x;
     long long j;
     dense_of(m, c, x, j);
     const int nx = m.nx, ny = m.ny;
     const int y = int(j % ny), z = int(j / ny);
+    const long long s0 = (long long)nx * ny * zlo;   // dense cell of solid[0]
     uint8_t fl = 0;
     uint32_t wm = 0;
-    if (x >= nx || solid[x + (long long)nx * j]) {
+    if (x >= nx || solid[x + (long long)nx * j - s0]) {
         fl = FLAG_SOLID;
     } else {
         for (int i = 0; i < Q; ++i) {
@@ -160,7 +172,7 @@ __global__ void k_flags(const uint8_t* __restrict__ solid, const R* __restrict__
             if (xs < 0 || xs >= nx) continue;
             if (zs >= nz) zs = z;
             bool wall = ys < 0 || ys >= ny || zs < 0;
-            if (!wall) wall = solid[xs + (long long)nx * (ys + (long long)ny * zs)] != 0;
+            if (!wall) wall = solid[xs + (long long)nx * (ys + (long long)ny * zs) - s0] != 0;
             if (wall) wm |= 1u << i;
         }
         if (wm) fl |= FLAG_NEARWALL;
@@ -375,32 +387,35 @@ __global__ void k_fill_macro(R* __restrict__ rho, R* __restrict__ ux, R* __restr
     uz[c] = R(0);
 }
 
-// Populations: padded storage -> dense real array f[i*Nd + k], and back.
+// Populations of dense cells [k0, k0 + n): padded storage -> real array
+// out[i*n + (k - k0)], and back.
 template <class Codec, typename O>
 __global__ void k_decode(const typename Codec::T* __restrict__ f, O* __restrict__ out, CellMap m,
-                         long long Npad, long long Nd, Center ctr) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= Nd) return;
+                         long long Npad, long long k0, long long n, Center ctr) {
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const long long k = k0 + r;
     const long long j = k / m.nx;
     const long long c = cell_of(m, int(k - j * m.nx), j);
     static_for<Q>([&](auto I) {
         constexpr int i = I;
-        out[(long long)i * Nd + k] =
+        out[(long long)i * n + r] =
             c < 0 ? O(0) : (O)Codec::template ld<i>(f + (long long)i * Npad + c, ctr.c[center_class(i)]);
     });
 }
 
 template <class Codec>
 __global__ void k_encode(typename Codec::T* __restrict__ f, const float* __restrict__ in, CellMap m,
-                         long long Npad, long long Nd, Center ctr) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= Nd) return;
+                         long long Npad, long long k0, long long n, Center ctr) {
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const long long k = k0 + r;
     const long long j = k / m.nx;
     const long long c = cell_of(m, int(k - j * m.nx), j);
     if (c < 0) return;
     static_for<Q>([&](auto I) {
         constexpr int i = I;
-        Codec::template st<i>(f + (long long)i * Npad + c, (typename Codec::R)in[(long long)i * Nd + k],
+        Codec::template st<i>(f + (long long)i * Npad + c, (typename Codec::R)in[(long long)i * n + r],
                               store_key<Codec>(ctr, i));
     });
 }

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -47,6 +48,30 @@ __global__ void __launch_bounds__(256) k_copy16(const float4* __restrict__ src, 
         const long long k = base + (long long)u * blockDim.x;
         if (k < n) dst[k] = src[k];
     }
+}
+
+// The memory traffic of one pull step without its arithmetic: cell c of a
+// px x ny x nz box reads population i from direction array i of src at
+// c - (cx_i + px cy_i + plane cz_i), clamped into the box, and writes it at
+// c of direction array i of dst with a non-temporal store. Direction arrays
+// start `stride` elements apart (stride - n elements of stagger between
+// them).
+template <typename T>
+__global__ void __launch_bounds__(256) k_streams19(const T* __restrict__ src, T* __restrict__ dst, long long n,
+                                                  long long stride, int px, long long plane) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    T v[Q];
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        long long s = c - (kCX[i] + (long long)px * kCY[i] + plane * kCZ[i]);
+        s = s < 0 ? 0 : (s >= n ? n - 1 : s);
+        v[i] = src[(long long)i * stride + s];
+    });
+    static_for<Q>([&](auto I) {
+        constexpr int i = I;
+        __builtin_nontemporal_store(v[i], &dst[(long long)i * stride + c]);
+    });
 }
 
 __global__ void k_fill_zero_u32(uint32_t* p, long long n) {
@@ -220,6 +245,7 @@ struct wfs_solver {
     int ntx = 0;                                         // x-tiles per row (px / 64)
     long long T = 0;                                     // stored tiles
     std::vector<int> h_tmap, h_tcoord, h_nbr, h_blk2tile, h_blk2tile4;   // block maps: 1 and 4 tiles per group
+    std::vector<int> h_rowtile;                          // tiles stored before row j (rows + 1 entries)
     virtual int set_force_cells(long long n, const int64_t* idx, const float* fx, const float* fy,
                                 const float* fz) = 0;
 
@@ -237,6 +263,10 @@ struct wfs_solver {
 };
 
 namespace {
+
+// Largest staging buffer for host exchanges of dense fields (blocks of whole
+// rows pass through it one after another).
+constexpr size_t kStageBytes = size_t(256) << 20;
 
 template <class Codec>
 struct Solver final : wfs_solver {
@@ -291,7 +321,18 @@ struct Solver final : wfs_solver {
         if ((rc = alloc_buf(out_x, (size_t)d.nx * sizeof(double)))) return rc;
         if ((rc = alloc_buf(partial, (size_t)d.nx * 256 * sizeof(double)))) return rc;
         if ((rc = alloc_buf(counter, 256))) return rc;
-        if ((rc = alloc_buf(stage, (size_t)N * sizeof(double)))) return rc;  // dense host <-> device staging
+        // host <-> device exchanges of dense fields pass through a staging
+        // buffer of at most kStageBytes, a block of whole rows at a time; it
+        // holds at least three z-planes of bytes (the solid mask around one
+        // plane) and one row of FP64 populations
+        size_t cap = kStageBytes;
+        if (const char* e = std::getenv("WFS_STAGE_BYTES")) {   // a smaller cap (tests use it to force many blocks)
+            const long long v = std::atoll(e);
+            if (v > 0) cap = std::min(cap, (size_t)v);
+        }
+        size_t sb = std::min((size_t)N * Q * sizeof(double), cap);
+        sb = std::max({sb, (size_t)3 * d.nx * d.ny, (size_t)d.nx * Q * sizeof(double)});
+        if ((rc = alloc_buf(stage, sb))) return rc;
         if ((rc = alloc_buf(aux_d, 2 * sizeof(StepAux<R>)))) return rc;     // StepAux pair (sync_aux fills it)
         if (tiled) {
             for (auto pr : {std::make_pair(&tmap_d, &h_tmap), std::make_pair(&tcoord_d, &h_tcoord),
@@ -307,25 +348,39 @@ struct Solver final : wfs_solver {
 
     int rows() const { return d.ny * d.nz; }
 
-    // host dense float cells -> device padded R cells
-    int put_cells(DevBuf& b, const float* src) {
-        WFS_HIP(hipMemcpyAsync(stage.p, src, (size_t)N * sizeof(float), hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL((k_pad<float, R>), dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
-                           static_cast<const float*>(stage.p), rp(b), cmap(), Np, R(0));
-        WFS_LAUNCH_CHECK();
+    // Device cell where dense row j starts (rows, and on the tile-sparse
+    // layout their tiles, are stored in row order).
+    long long row_cell(long long j) const { return tiled ? (long long)h_rowtile[(size_t)j] * 64 : (long long)px * j; }
+    // Rows per staging block when one row takes row_bytes of it.
+    long long block_rows(size_t row_bytes) const {
+        return std::max<long long>(1, (long long)(stage.bytes / row_bytes));
+    }
+
+    // host dense S cells -> device padded D cells (padding cells: fill), a
+    // block of rows at a time
+    template <typename S, typename D>
+    int put_dense(DevBuf& b, const S* src, D fill) {
+        const long long nrows = rows(), step = block_rows((size_t)d.nx * sizeof(S));
+        for (long long j0 = 0; j0 < nrows; j0 += step) {
+            const long long j1 = std::min(nrows, j0 + step);
+            WFS_HIP(hipMemcpyAsync(stage.p, src + j0 * d.nx, (size_t)((j1 - j0) * d.nx) * sizeof(S),
+                                   hipMemcpyHostToDevice, stream));
+            const long long c0 = row_cell(j0), c1 = row_cell(j1);
+            if (c1 > c0) {
+                hipLaunchKernelGGL((k_pad<S, D>), dim3(blocks_for(c1 - c0, 256)), dim3(256), 0, stream,
+                                   static_cast<const S*>(stage.p), static_cast<D*>(b.p), cmap(), c0, c1, j0, fill);
+                WFS_LAUNCH_CHECK();
+            }
+        }
         WFS_HIP(hipStreamSynchronize(stream));
         return WFS_OK;
     }
 
+    // host dense float cells -> device padded R cells
+    int put_cells(DevBuf& b, const float* src) { return put_dense<float, R>(b, src, R(0)); }
+
     // host dense u8 cells -> device padded u8 cells
-    int put_bytes(DevBuf& b, const uint8_t* src) {
-        WFS_HIP(hipMemcpyAsync(stage.p, src, (size_t)N, hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL((k_pad<uint8_t, uint8_t>), dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
-                           static_cast<const uint8_t*>(stage.p), static_cast<uint8_t*>(b.p), cmap(), Np, uint8_t(0));
-        WFS_LAUNCH_CHECK();
-        WFS_HIP(hipStreamSynchronize(stream));
-        return WFS_OK;
-    }
+    int put_bytes(DevBuf& b, const uint8_t* src) { return put_dense<uint8_t, uint8_t>(b, src, uint8_t(0)); }
 
     // host float rows (ny*nz) -> device R rows
     int put_rows(DevBuf& b, const float* src) {
@@ -336,20 +391,33 @@ struct Solver final : wfs_solver {
         return WFS_OK;
     }
 
+    // device R cells -> host dense D cells, a block of rows at a time: cells
+    // of tiles that are not stored get `fill`, or with `add` set every cell
+    // gets its value plus `fill` (added in double before the rounding to D)
+    template <typename D>
+    int get_dense(DevBuf& b, D* dst, double fill, bool add) {
+        const long long nrows = rows(), step = block_rows((size_t)d.nx * sizeof(D));
+        for (long long j0 = 0; j0 < nrows; j0 += step) {
+            const long long j1 = std::min(nrows, j0 + step);
+            const long long k0 = j0 * d.nx, n = (j1 - j0) * d.nx;
+            if (add)
+                hipLaunchKernelGGL((k_unpad_add<R, D>), dim3(blocks_for(n, 256)), dim3(256), 0, stream, rp(b),
+                                   static_cast<D*>(stage.p), cmap(), k0, n, fill);
+            else
+                hipLaunchKernelGGL((k_unpad<R, D>), dim3(blocks_for(n, 256)), dim3(256), 0, stream, rp(b),
+                                   static_cast<D*>(stage.p), cmap(), k0, n, (D)fill);
+            WFS_LAUNCH_CHECK();
+            WFS_HIP(hipMemcpyAsync(dst + k0, stage.p, (size_t)n * sizeof(D), hipMemcpyDeviceToHost, stream));
+        }
+        WFS_HIP(hipStreamSynchronize(stream));
+        return WFS_OK;
+    }
+
     // device R cells -> host dense float or double cells (cells of tiles that are not stored: fill)
     int get_cells(DevBuf& b, void* dst, bool f64, double fill = 0.0) {
         if (!dst) return WFS_OK;
-        if (f64)
-            hipLaunchKernelGGL((k_unpad<R, double>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(b),
-                               static_cast<double*>(stage.p), cmap(), N, fill);
-        else
-            hipLaunchKernelGGL((k_unpad<R, float>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(b),
-                               static_cast<float*>(stage.p), cmap(), N, (float)fill);
-        WFS_LAUNCH_CHECK();
-        WFS_HIP(hipMemcpyAsync(dst, stage.p, (size_t)N * (f64 ? sizeof(double) : sizeof(float)),
-                               hipMemcpyDeviceToHost, stream));
-        WFS_HIP(hipStreamSynchronize(stream));
-        return WFS_OK;
+        return f64 ? get_dense(b, static_cast<double*>(dst), fill, false)
+                   : get_dense(b, static_cast<float*>(dst), fill, false);
     }
 
     int zero(DevBuf& b) {
@@ -376,12 +444,22 @@ struct Solver final : wfs_solver {
         }
         if ((rc = put_rows(uin, uin_h))) return rc;
         set_center(uin_h);
-        // the dense solid mask stays in the staging buffer while the flags are built
-        WFS_HIP(hipMemcpyAsync(stage.p, solid, (size_t)N, hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(k_flags<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
-                           static_cast<const uint8_t*>(stage.p), rp(cfw), static_cast<uint8_t*>(flag.p),
-                           static_cast<uint32_t*>(wmask.p), cmap(), d.nz, Np);
-        WFS_LAUNCH_CHECK();
+        // flags and wall masks a block of z-planes at a time: the staging
+        // buffer holds the block's dense solid mask and one plane either side
+        const size_t plane = (size_t)d.nx * d.ny;
+        const int pz = (int)std::max<long long>(1, (long long)(stage.bytes / plane) - 2);
+        for (int z0 = 0; z0 < d.nz; z0 += pz) {
+            const int z1 = std::min(d.nz, z0 + pz), zlo = std::max(0, z0 - 1), zhi = std::min(d.nz, z1 + 1);
+            WFS_HIP(hipMemcpyAsync(stage.p, solid + (size_t)zlo * plane, (size_t)(zhi - zlo) * plane,
+                                   hipMemcpyHostToDevice, stream));
+            const long long c0 = row_cell((long long)d.ny * z0), c1 = row_cell((long long)d.ny * z1);
+            if (c1 > c0) {
+                hipLaunchKernelGGL(k_flags<R>, dim3(blocks_for(c1 - c0, 256)), dim3(256), 0, stream,
+                                   static_cast<const uint8_t*>(stage.p), rp(cfw), static_cast<uint8_t*>(flag.p),
+                                   static_cast<uint32_t*>(wmask.p), cmap(), d.nz, c0, c1, zlo);
+                WFS_LAUNCH_CHECK();
+            }
+        }
         for (int k = 0; k < 2; ++k) {
             if (!f[k].p) continue;   // in-place streaming keeps one buffer
             hipLaunchKernelGGL(k_init<Codec>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, fp(k), flags(),
@@ -747,23 +825,26 @@ struct Solver final : wfs_solver {
         return rc;
     }
 
+    // populations f[i*N + k] of the dense cells, through the staging buffer
+    // a block of rows at a time
     int get_pops_ab(void* out, bool f64) {
-        const size_t n = (size_t)N * Q;
-        DevBuf tmp;
         const size_t esz = f64 ? sizeof(double) : sizeof(float);
-        if (hipMalloc(&tmp.p, n * esz) != hipSuccess) {
-            (void)hipGetLastError();
-            tmp.p = nullptr;
-            return fail(WFS_ENOMEM, "hipMalloc for population readout failed");
+        const long long nrows = rows(), step = block_rows((size_t)d.nx * Q * esz);
+        for (long long j0 = 0; j0 < nrows; j0 += step) {
+            const long long j1 = std::min(nrows, j0 + step);
+            const long long k0 = j0 * d.nx, n = (j1 - j0) * d.nx;
+            if (f64)
+                hipLaunchKernelGGL((k_decode<Codec, double>), dim3(blocks_for(n, 256)), dim3(256), 0, stream,
+                                   fp(cur), static_cast<double*>(stage.p), cmap(), Np, k0, n, ctr);
+            else
+                hipLaunchKernelGGL((k_decode<Codec, float>), dim3(blocks_for(n, 256)), dim3(256), 0, stream,
+                                   fp(cur), static_cast<float*>(stage.p), cmap(), Np, k0, n, ctr);
+            WFS_LAUNCH_CHECK();
+            for (int i = 0; i < Q; ++i)
+                WFS_HIP(hipMemcpyAsync(static_cast<char*>(out) + ((size_t)i * N + k0) * esz,
+                                       static_cast<const char*>(stage.p) + (size_t)i * n * esz, (size_t)n * esz,
+                                       hipMemcpyDeviceToHost, stream));
         }
-        if (f64)
-            hipLaunchKernelGGL((k_decode<Codec, double>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
-                               static_cast<double*>(tmp.p), cmap(), Np, N, ctr);
-        else
-            hipLaunchKernelGGL((k_decode<Codec, float>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
-                               static_cast<float*>(tmp.p), cmap(), Np, N, ctr);
-        WFS_LAUNCH_CHECK();
-        WFS_HIP(hipMemcpyAsync(out, tmp.p, n * esz, hipMemcpyDeviceToHost, stream));
         WFS_HIP(hipStreamSynchronize(stream));
         return WFS_OK;
     }
@@ -780,17 +861,17 @@ struct Solver final : wfs_solver {
     }
 
     int set_pops_ab(const float* in) {
-        const size_t n = (size_t)N * Q;
-        DevBuf tmp;
-        if (hipMalloc(&tmp.p, n * sizeof(float)) != hipSuccess) {
-            (void)hipGetLastError();
-            tmp.p = nullptr;
-            return fail(WFS_ENOMEM, "hipMalloc for population upload failed");
+        const long long nrows = rows(), step = block_rows((size_t)d.nx * Q * sizeof(float));
+        for (long long j0 = 0; j0 < nrows; j0 += step) {
+            const long long j1 = std::min(nrows, j0 + step);
+            const long long k0 = j0 * d.nx, n = (j1 - j0) * d.nx;
+            for (int i = 0; i < Q; ++i)
+                WFS_HIP(hipMemcpyAsync(static_cast<float*>(stage.p) + (size_t)i * n, in + (size_t)i * N + k0,
+                                       (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
+            hipLaunchKernelGGL(k_encode<Codec>, dim3(blocks_for(n, 256)), dim3(256), 0, stream, fp(cur),
+                               static_cast<const float*>(stage.p), cmap(), Np, k0, n, ctr);
+            WFS_LAUNCH_CHECK();
         }
-        WFS_HIP(hipMemcpyAsync(tmp.p, in, n * sizeof(float), hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(k_encode<Codec>, dim3(blocks_for(N, 256)), dim3(256), 0, stream, fp(cur),
-                           static_cast<const float*>(tmp.p), cmap(), Np, N, ctr);
-        WFS_LAUNCH_CHECK();
         WFS_HIP(hipStreamSynchronize(stream));
         return WFS_OK;
     }
@@ -910,17 +991,8 @@ struct Solver final : wfs_solver {
 
     int get_cells_add(DevBuf& b, void* dst, bool f64, double add) {
         if (!dst) return WFS_OK;
-        if (f64)
-            hipLaunchKernelGGL((k_unpad_add<R, double>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(b),
-                               static_cast<double*>(stage.p), cmap(), N, add);
-        else
-            hipLaunchKernelGGL((k_unpad_add<R, float>), dim3(blocks_for(N, 256)), dim3(256), 0, stream, rp(b),
-                               static_cast<float*>(stage.p), cmap(), N, add);
-        WFS_LAUNCH_CHECK();
-        WFS_HIP(hipMemcpyAsync(dst, stage.p, (size_t)N * (f64 ? sizeof(double) : sizeof(float)),
-                               hipMemcpyDeviceToHost, stream));
-        WFS_HIP(hipStreamSynchronize(stream));
-        return WFS_OK;
+        return f64 ? get_dense(b, static_cast<double*>(dst), add, true)
+                   : get_dense(b, static_cast<float*>(dst), add, true);
     }
 
     const R* field_ptr(int field) {
@@ -996,18 +1068,13 @@ struct Solver final : wfs_solver {
                     (rc = alloc_buf(mlp_pack, sizeof(MlpPack))) || (rc = alloc_buf(c2buf, (size_t)Np * sizeof(float))))
                     return rc;
             }
-            WFS_HIP(hipMemcpyAsync(stage.p, geo_h, (size_t)N * sizeof(uint16_t), hipMemcpyHostToDevice, stream));
-            hipLaunchKernelGGL((k_pad<uint16_t, uint16_t>), dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
-                               static_cast<const uint16_t*>(stage.p), static_cast<uint16_t*>(geo.p), cmap(), Np,
-                               uint16_t(0));
-            WFS_LAUNCH_CHECK();
-            WFS_HIP(hipStreamSynchronize(stream));
+            if ((rc = put_dense<uint16_t, uint16_t>(geo, geo_h, uint16_t(0)))) return rc;
             const long long ncol = (long long)px * d.ny;
             WFS_HIP(hipMemcpyAsync(stage.p, hcol_h, (size_t)d.nx * d.ny, hipMemcpyHostToDevice, stream));
             const CellMap cm{d.nx, px, d.ny, 0, nullptr, nullptr};   // per column: rows of the (x, y) plane
             hipLaunchKernelGGL((k_pad<uint8_t, uint8_t>), dim3(blocks_for(ncol, 256)), dim3(256), 0, stream,
-                               static_cast<const uint8_t*>(stage.p), static_cast<uint8_t*>(hcol.p), cm, ncol,
-                               uint8_t(0));
+                               static_cast<const uint8_t*>(stage.p), static_cast<uint8_t*>(hcol.p), cm, 0LL, ncol,
+                               0LL, uint8_t(0));
             WFS_LAUNCH_CHECK();
             WFS_HIP(hipMemcpyAsync(mlp_params.p, params, kNParams * sizeof(float), hipMemcpyHostToDevice, stream));
             hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(256), 0, stream, static_cast<const float*>(mlp_params.p),
@@ -1604,8 +1671,10 @@ static void build_tiles(wfs_solver* h, const uint8_t* solid) {
     h->h_tmap.assign((size_t)(ntx * rows), -1);
     h->h_tcoord.clear();
     std::vector<std::vector<int>> row_tiles;   // per non-empty row, its tiles
+    h->h_rowtile.assign((size_t)rows + 1, 0);
     long long T = 0;
     for (long long j = 0; j < rows; ++j) {
+        h->h_rowtile[(size_t)j] = (int)T;
         std::vector<int> rt;
         const uint8_t* r = solid + (size_t)j * d.nx;
         for (int tx = 0; tx < ntx; ++tx) {
@@ -1623,6 +1692,7 @@ static void build_tiles(wfs_solver* h, const uint8_t* solid) {
         if (!rt.empty()) row_tiles.push_back(std::move(rt));
     }
     h->T = T;
+    h->h_rowtile[(size_t)rows] = (int)T;
     h->h_nbr.assign((size_t)T * 27, -1);
     for (long long t = 0; t < T; ++t) {
         const int tx = h->h_tcoord[3 * t], y = h->h_tcoord[3 * t + 1], z = h->h_tcoord[3 * t + 2];
@@ -2017,6 +2087,66 @@ int wfs_batch_synchronize(wfs_batch* b) {
 int wfs_batch_time(wfs_batch* b, int32_t k, double* t, int32_t* navg) {
     if (!b || !t || !navg) return fail(WFS_EINVAL, "wfs_batch_time: null argument");
     return b->impl->time_of(k, t, navg);
+}
+
+int wfs_stream_probe(int32_t device, int32_t px, int32_t ny, int32_t nz, int32_t elem_bytes, int64_t stagger_bytes,
+                     int32_t iters, double* gbs) {
+    if (!gbs || px < 64 || ny < 3 || nz < 3 || iters < 1 || (elem_bytes != 2 && elem_bytes != 4) ||
+        stagger_bytes < 0 || stagger_bytes % elem_bytes)
+        return fail(WFS_EINVAL, "wfs_stream_probe: bad arguments");
+    const long long n = (long long)px * ny * nz;
+    const long long stride = n + stagger_bytes / elem_bytes;
+    if ((unsigned long long)Q * (unsigned long long)stride * (unsigned long long)elem_bytes >= (1ull << 36))
+        return fail(WFS_EINVAL, "wfs_stream_probe: box too large");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+        (void)hipGetLastError();
+        return fail(WFS_ENODEV, "wfs_stream_probe: no such device");
+    }
+    DeviceGuard g(device);
+    const size_t bytes = (size_t)Q * (size_t)stride * (size_t)elem_bytes;
+    DevBuf a, b;
+    if (hipMalloc(&a.p, bytes) != hipSuccess || hipMalloc(&b.p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(WFS_ENOMEM, "wfs_stream_probe: allocation failed");
+    }
+    hipStream_t st;
+    WFS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg{st};
+    hipLaunchKernelGGL(k_fill_zero_u32, dim3(blocks_for((long long)(bytes / 4), 256)), dim3(256), 0, st,
+                       static_cast<uint32_t*>(a.p), (long long)(bytes / 4));
+    WFS_LAUNCH_CHECK();
+    const long long plane = (long long)px * ny;
+    auto launch = [&]() {
+        if (elem_bytes == 4)
+            hipLaunchKernelGGL(k_streams19<float>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                               static_cast<const float*>(a.p), static_cast<float*>(b.p), n, stride, px, plane);
+        else
+            hipLaunchKernelGGL(k_streams19<unsigned short>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                               static_cast<const unsigned short*>(a.p), static_cast<unsigned short*>(b.p), n, stride,
+                               px, plane);
+    };
+    launch();   // warm-up
+    WFS_LAUNCH_CHECK();
+    hipEvent_t e0, e1;
+    WFS_HIP(hipEventCreate(&e0));
+    WFS_HIP(hipEventCreate(&e1));
+    WFS_HIP(hipEventRecord(e0, st));
+    for (int k = 0; k < iters; ++k) {
+        launch();
+        WFS_LAUNCH_CHECK();
+    }
+    WFS_HIP(hipEventRecord(e1, st));
+    WFS_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    WFS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbs = 2.0 * Q * (double)elem_bytes * (double)n * iters / (ms * 1e-3) / 1e9;
+    return WFS_OK;
 }
 
 int wfs_bandwidth_probe(int32_t device, int64_t bytes, int32_t iters, double* gbs) {
