@@ -241,26 +241,42 @@ def test_storage_formats_against_fp64_after_1000_steps(storage, vel_tol, eta_tol
     (dx/dt)^2 / (3 g) = 340 m: about 3.4 mm per cell and 0.6 mm for a row of
     30 cells, so a few mm over 128 rows (2.4 mm measured); BF16 has 3 bits
     fewer (13 mm measured). The bounds are those estimates; the level bar of
-    1 mm applies to time means (test_gpu_precision)."""
+    1 mm applies to time means (test_gpu_precision).
+
+    After 1000 steps the propeller's LES turbulence has made the state depend
+    on the rounding history: FP16 runs from starts that differ by half an
+    FP16 ulp of the deviations end 1.4e-4 to 5e-4 (velocity) and 2.2 to
+    9.2 mm (level) from FP64, with either form of the collision arithmetic.
+    One run is one draw of that spread, so the bounds apply to the median of
+    five: the plain start and four such perturbed starts (the perturbation
+    is applied to every format alike, including the FP32 row)."""
     reach, ref = _fp64_reference_c1(1000)
     args = solver_inputs(reach)
-    g = LBM3D(None, args["solid"], args["Fx"], args["Fy"], args["Fz"], args["uin"], args["tau0"], args["cs"],
-              args["cf_wall"], storage=storage)
-    g.set_bedwall(reach.bedwall)
-    for _ in range(50):
-        g.update_uref()
-        g.step(20)
-    rho, ux, uy, uz = g.macro()
     fl = ~reach.solid.ravel()
-    vel = np.concatenate([ux[fl], uy[fl], uz[fl]])
     vref = np.concatenate([ref.ux[fl], ref.uy[fl], ref.uz[fl]])
-    err_v = rel_l2(vel, vref)
-    err_eta = np.abs(_level_profile(reach, rho) - _level_profile(reach, ref.rho)).max()
-    print(f"{storage}: velocity rel L2 {err_v:.3e}, level max error {err_eta * 1e3:.4f} mm")
-    assert np.isfinite(vel).all()
+    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, np.float32)
+    errs = []
+    for seed in (None, 1, 2, 3, 4):
+        g = LBM3D(None, args["solid"], args["Fx"], args["Fy"], args["Fz"], args["uin"], args["tau0"], args["cs"],
+                  args["cf_wall"], storage=storage)
+        g.set_bedwall(reach.bedwall)
+        if seed is not None:
+            f = g.populations().reshape(19, -1)
+            noise = np.random.default_rng(seed).standard_normal(f.shape).astype(np.float32)
+            g.set_populations(f + w[:, None] * np.float32(1e-6) * noise)
+        for _ in range(50):
+            g.update_uref()
+            g.step(20)
+        rho, ux, uy, uz = g.macro()
+        g.close()
+        vel = np.concatenate([ux[fl], uy[fl], uz[fl]])
+        assert np.isfinite(vel).all()
+        errs.append((rel_l2(vel, vref), np.abs(_level_profile(reach, rho) - _level_profile(reach, ref.rho)).max()))
+        print(f"{storage} start {seed}: velocity rel L2 {errs[-1][0]:.3e}, level max error {errs[-1][1] * 1e3:.4f} mm")
+    err_v, err_eta = np.median([e[0] for e in errs]), np.median([e[1] for e in errs])
+    print(f"{storage} median of 5: velocity rel L2 {err_v:.3e}, level {err_eta * 1e3:.4f} mm")
     assert err_v <= vel_tol
     assert err_eta <= eta_tol
-    g.close()
 
 
 @pytest.mark.parametrize("tag", ["boat", "base"])

@@ -60,8 +60,10 @@ constexpr float kC2Scale = 0.0289f; // C^2 per unit softplus (0.17^2)
 constexpr int kOutletSponge = 64;
 
 typedef short s4v __attribute__((ext_vector_type(4)));
+typedef short s2v __attribute__((ext_vector_type(2)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef uint32_t u2v __attribute__((ext_vector_type(2)));
 typedef uint32_t u4v __attribute__((ext_vector_type(4)));
@@ -94,33 +96,48 @@ __host__ __device__ constexpr int mfma_row(int r, int h) { return 8 * (r >> 2) +
 // Hidden unit in B slot j (0..7) of k-step s for lane half h.
 __host__ __device__ constexpr int l2_unit(int s, int h, int j) { return mfma_row(8 * s + j, h); }
 
+// Two floats rounded to bfloat16 (nearest even) in one register, low half a:
+// one v_cvt_pk_bf16_f32 (a vector conversion; element-wise conversions can
+// become two conversions and a byte permute)
 __device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
-    bf2v v = {(__bf16)a, (__bf16)b};
-    return __builtin_bit_cast(uint32_t, v);
+    const f2v v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf2v));
 }
 __device__ __forceinline__ float bf16_round(float a) { return (float)(__bf16)a; }
 
+// softplus(y) = ln(1 + e^y) = ln2 log2(1 + 2^(y log2 e)), on the hardware
+// base-2 exponential and logarithm (about 1 ulp; for y below -20 the sum
+// rounds to 1 and the result to 0, against e^y < 2e-9 exactly)
 __device__ __forceinline__ float softplus_f(float y) {
-    return y > 15.f ? y : __logf(1.f + __expf(y));
+    const float e = __builtin_amdgcn_exp2f(y * 1.44269504f);
+    return y > 15.f ? y : 0.693147181f * __builtin_amdgcn_logf(1.f + e);
 }
 
-// The eight features of one cell (formulas in the header comment). The
-// network sees them rounded to bfloat16, so the hardware square root,
-// reciprocal square root and reciprocal (about 1 ulp) serve here.
-__device__ __forceinline__ void closure_features(float sxx, float syy, float szz, float sxy, float sxz, float syz,
-                                                 float ux, float uy, float uz, uint16_t geo, uint8_t hcol,
+// The eight features of one cell (formulas in the header comment) from the
+// strain rate S = k P, P trace-free (the step passes the deviatoric part of
+// Pi_neq and k = -3 / (2 rho tau_s); the training samples pass S and k = 1).
+// The invariants scale out of k: S:S = k^2 P:P, |S| = |k| sqrt(2 P:P),
+// tr(S^3) / (S:S)^1.5 = sign(k) tr(P^3) / (P:P)^1.5, and for a trace-free
+// P, tr(P^3) = 3 det P (Cayley-Hamilton), so x1 = sign(k) 3 sqrt(6) det P
+// / (P:P)^1.5 and x3 = 2 (P_xz^2 + P_yz^2) / P:P. The network sees the
+// features rounded to bfloat16, so the hardware square root, reciprocal
+// square root and base-2 logarithm (about 1 ulp) serve here.
+__device__ __forceinline__ void closure_features(float k, float pxx, float pyy, float pzz, float pxy, float pxz,
+                                                 float pyz, float ux, float uy, float uz, uint16_t geo, uint8_t hcol,
                                                  float inv_u0, float* x) {
-    const float od = sxy * sxy + sxz * sxz + syz * syz;
-    const float ss = sxx * sxx + syy * syy + szz * szz + 2.f * od;
-    const float smag = __builtin_amdgcn_sqrtf(2.f * ss);
-    x[0] = 0.25f * __logf(smag + 1e-7f) + 1.5f;
-    const float t3 = sxx * sxx * sxx + syy * syy * syy + szz * szz * szz + 3.f * sxy * sxy * (sxx + syy) +
-                     3.f * sxz * sxz * (sxx + szz) + 3.f * syz * syz * (syy + szz) + 6.f * sxy * sxz * syz;
-    const bool live = ss > 1e-20f;
-    const float rs = __builtin_amdgcn_rsqf(ss);
-    x[1] = live ? fminf(fmaxf(2.4494897f * t3 * (rs * rs * rs), -1.f), 1.f) : 0.f;
-    x[2] = __builtin_amdgcn_sqrtf(ux * ux + uy * uy + uz * uz) * inv_u0 - 1.f;
-    x[3] = live ? 2.f * (sxz * sxz + syz * syz) * (rs * rs) : 0.f;
+    const float vz = fmaf(pxz, pxz, pyz * pyz);
+    const float pp = fmaf(pxx, pxx, fmaf(pyy, pyy, pzz * pzz)) + 2.f * fmaf(pxy, pxy, vz);
+    const bool live = (k * k) * pp > 1e-20f;
+    const float smag = fabsf(k) * __builtin_amdgcn_sqrtf(2.f * pp);
+    x[0] = fmaf(0.25f * 0.693147181f, __builtin_amdgcn_logf(smag + 1e-7f), 1.5f);
+    const float rs = __builtin_amdgcn_rsqf(pp);
+    const float rs2 = rs * rs;
+    const float det = pxx * fmaf(pyy, pzz, -pyz * pyz) - pxy * fmaf(pxy, pzz, -pyz * pxz) +
+                      pxz * fmaf(pxy, pyz, -pyy * pxz);
+    const float c1 = k < 0.f ? -7.34846923f : 7.34846923f;   // 3 sqrt(6)
+    x[1] = live ? __builtin_amdgcn_fmed3f(c1 * det * (rs2 * rs), -1.f, 1.f) : 0.f;
+    x[2] = __builtin_amdgcn_sqrtf(fmaf(ux, ux, fmaf(uy, uy, uz * uz))) * inv_u0 - 1.f;
+    x[3] = live ? 2.f * vz * rs2 : 0.f;
     x[4] = uz * inv_u0;
     x[5] = float(geo & 255u) * (1.f / 255.f);
     x[6] = float(geo >> 8) * (1.f / 255.f);
@@ -132,6 +149,15 @@ __device__ __forceinline__ void closure_features(float sxx, float syy, float szz
 // costs two instructions here (the operand is first made a canonical NaN).
 __device__ __forceinline__ float relu_f(float v) {
     return __int_as_float(max(__float_as_int(v), 0));
+}
+
+// ReLU of two packed bfloat16 values as one packed 16-bit integer max:
+// negative values (and -0) have negative int16 patterns and become +0,
+// others are unchanged. Rounding to bfloat16 keeps the sign, so ReLU after
+// the rounding equals the rounding after ReLU.
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t v) {
+    const s2v z = {0, 0};
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s2v, v), z));
 }
 
 // Network output y for the cell of this lane. All 64 lanes of the wavefront
@@ -173,8 +199,8 @@ __device__ __forceinline__ float mlp_wave(const float* x, const MlpPack* P) {
         u4v hq0, hq1;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            hq0[q] = pk_bf16(relu_f(acc[2 * q]), relu_f(acc[2 * q + 1]));
-            hq1[q] = pk_bf16(relu_f(acc[8 + 2 * q]), relu_f(acc[8 + 2 * q + 1]));
+            hq0[q] = relu_bf16x2(pk_bf16(acc[2 * q], acc[2 * q + 1]));
+            hq1[q] = relu_bf16x2(pk_bf16(acc[8 + 2 * q], acc[8 + 2 * q + 1]));
         }
         f16v acc2;
 #pragma unroll
@@ -471,7 +497,7 @@ __global__ void __launch_bounds__(256) k_samples(const float* __restrict__ ubar,
         A = smag * smag * ss;
         B = smag * ts;
         P = Txx * Txx + Tyy * Tyy + Tzz * Tzz + 2.f * (Txy * Txy + Txz * Txz + Tyz * Tyz);
-        closure_features(sxx, syy, szz, sxy, sxz, syz, ubar[c], ubar[nc + c], ubar[2 * nc + c], geo[c],
+        closure_features(1.f, sxx, syy, szz, sxy, sxz, syz, ubar[c], ubar[nc + c], ubar[2 * nc + c], geo[c],
                          hcol[x + (long long)ncx * y], inv_u0, feat);
 #pragma unroll
         for (int j = 0; j < kNF; ++j) feat[j] = bf16_round(feat[j]);

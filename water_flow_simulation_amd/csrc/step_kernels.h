@@ -394,18 +394,27 @@ __device__ __forceinline__ void boundary_row(R* f, const R* v, int side, R uin, 
 
 // Regularized BGK + Smagorinsky + Guo forcing in pair form, for the storage
 // formats that are judged against an FP64 run rather than bit for bit.
-// Opposite directions i / opp(i) share the even part of their update and
-// differ in the sign of the odd part:
-//   feq_i     = w rho (1 - 1.5 u^2 + 4.5 (c.u)^2)  +  3 w rho (c.u)
-//   fneq_i    = 4.5 w (Q_i : Pi_neq),  Q_i = c c - I/3      (even)
-//   source    = w sf (9 (c.u)(c.F) - 3 u.F)  +  w (c.F)(3 sf - 1.5 (1 - omega))
-// and Pi_neq comes from the second moment of f minus rho (u u + I/3), the
-// second moment of the equilibrium, so no per-direction equilibrium is formed
-// for the stress. Same mathematics as the per-direction form; different
-// rounding (FMAs, one reciprocal).
+// With Q_i = c_i c_i - I/3 and j = rho u (u with the Guo half-force shift),
+// the post-collision population is
+//   f_i = w_i rho + 4.5 w_i (Q_i : L) + 3 w_i (c_i . J)
+//   L = rho u u + (1 - omega) Pi_neq + sf (u F + F u),   sf = 1 - omega / 2
+//   J = j + (sf - (1 - omega) / 2) F
+// which is feq + (1 - omega)(fneq + fneq1) + the Guo source of the
+// per-direction form (floodsim/lbm3d.py:103-137) regrouped: w rho (1 - 1.5
+// u^2 + 4.5 (c.u)^2) + 4.5 w (1 - omega) Q:Pi_neq + 9 w sf (c.u)(c.F) -
+// 3 w sf u.F is w rho + 4.5 w Q:L, and 3 w rho (c.u) + w (c.F)(3 sf - 1.5 (1 -
+// omega)) is 3 w c.J. Opposite directions share the even part and differ in
+// the sign of the odd part, and only the deviatoric part of L enters
+// (Q_i : L = c.L.c - tr L / 3):
+//   axis x:        Q:L = L_xx - T                      (T = tr L / 3)
+//   plane (1,+-1,0): Q:L = (T - (L_zz - T)) +- 2 L_xy
+// Pi_neq comes from the second moment of f minus rho (u u + I/3), the second
+// moment of the equilibrium, so no per-direction equilibrium is formed for
+// the stress. Same mathematics as the per-direction form; different rounding
+// (FMAs, one reciprocal).
 //
-// pair_moments: density, velocity (with the Guo half-force shift), the
-// non-equilibrium stress and its norm Q; pair_relax: the post-collision state.
+// pair_moments: density, momentum and velocity, the non-equilibrium stress
+// and its norm Q; pair_relax: the post-collision state.
 // Reciprocal and square root of the pair form: the hardware instructions
 // (about 1 ulp) in float, IEEE in double. The pair form is judged against
 // FP64 runs, not bit for bit, and is VALU-bound in the 16-bit formats.
@@ -416,7 +425,7 @@ __device__ __forceinline__ double pair_sqrt(double v) { return sqrt(v); }
 
 template <typename R>
 struct PairMoments {
-    R r, ri, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz, qn;
+    R r, ri, ux, uy, uz, jx, jy, jz, pxx, pyy, pzz, pxy, pxz, pyz, qn;
 };
 
 // kForced: the wavefront holds a forced cell (fx, fy, fz zero on the others,
@@ -431,28 +440,28 @@ __device__ __forceinline__ void pair_moments(const R* f, R fx, R fy, R fz, PairM
         sp[k] = f[2 * k + 1] + f[2 * k + 2];
         dp[k] = f[2 * k + 1] - f[2 * k + 2];
     }
-    // pairs: 0:x 1:y 2:z 3:(1,1,0) 4:(1,-1,0) 5:(1,0,1) 6:(1,0,-1) 7:(0,1,1) 8:(0,1,-1)
-    const R r = f[0] + ((sp[0] + sp[1] + sp[2]) + (sp[3] + sp[4] + sp[5]) + (sp[6] + sp[7] + sp[8]));
-    const R mx = dp[0] + dp[3] + dp[4] + dp[5] + dp[6];
-    const R my = dp[1] + dp[3] - dp[4] + dp[7] + dp[8];
-    const R mz = dp[2] + dp[5] - dp[6] + dp[7] - dp[8];
-    const R ri = pair_rcp(r);
-    R ux, uy, uz;
+    // pairs: 0:x 1:y 2:z 3:(1,1,0) 4:(1,-1,0) 5:(1,0,1) 6:(1,0,-1) 7:(0,1,1) 8:(0,1,-1);
+    // the four directions of each diagonal plane summed
+    const R dxy = sp[3] + sp[4], dxz = sp[5] + sp[6], dyz = sp[7] + sp[8];
+    const R r = f[0] + ((sp[0] + sp[1] + sp[2]) + (dxy + dxz + dyz));
+    R jx = dp[0] + dp[3] + dp[4] + dp[5] + dp[6];
+    R jy = dp[1] + dp[3] - dp[4] + dp[7] + dp[8];
+    R jz = dp[2] + dp[5] - dp[6] + dp[7] - dp[8];
     if constexpr (kForced) {
-        ux = fma(R(0.5), fx, mx) * ri; uy = fma(R(0.5), fy, my) * ri; uz = fma(R(0.5), fz, mz) * ri;
-    } else {
-        ux = mx * ri; uy = my * ri; uz = mz * ri;
+        jx = fma(R(0.5), fx, jx); jy = fma(R(0.5), fy, jy); jz = fma(R(0.5), fz, jz);
     }
-    const R rux = r * ux, ruy = r * uy, ruz = r * uz, r3 = r * (R(1) / R(3));
-    m.pxx = (sp[0] + sp[3] + sp[4] + sp[5] + sp[6]) - fma(rux, ux, r3);
-    m.pyy = (sp[1] + sp[3] + sp[4] + sp[7] + sp[8]) - fma(ruy, uy, r3);
-    m.pzz = (sp[2] + sp[5] + sp[6] + sp[7] + sp[8]) - fma(ruz, uz, r3);
-    m.pxy = (sp[3] - sp[4]) - rux * uy;
-    m.pxz = (sp[5] - sp[6]) - rux * uz;
-    m.pyz = (sp[7] - sp[8]) - ruy * uz;
+    const R ri = pair_rcp(r);
+    const R ux = jx * ri, uy = jy * ri, uz = jz * ri;
+    const R r3 = r * (R(1) / R(3));
+    m.pxx = (sp[0] + dxy + dxz) - fma(jx, ux, r3);
+    m.pyy = (sp[1] + dxy + dyz) - fma(jy, uy, r3);
+    m.pzz = (sp[2] + dxz + dyz) - fma(jz, uz, r3);
+    m.pxy = fma(-jx, uy, sp[3] - sp[4]);
+    m.pxz = fma(-jx, uz, sp[5] - sp[6]);
+    m.pyz = fma(-jy, uz, sp[7] - sp[8]);
     m.qn = pair_sqrt(fma(m.pxx, m.pxx, fma(m.pyy, m.pyy, m.pzz * m.pzz)) +
                      R(2) * fma(m.pxy, m.pxy, fma(m.pxz, m.pxz, m.pyz * m.pyz)));
-    m.r = r; m.ri = ri; m.ux = ux; m.uy = uy; m.uz = uz;
+    m.r = r; m.ri = ri; m.ux = ux; m.uy = uy; m.uz = uz; m.jx = jx; m.jy = jy; m.jz = jz;
 }
 
 // Relaxation of the pair form with eddy-viscosity constant ksmag = 18 sqrt(2) C^2.
@@ -460,51 +469,51 @@ template <typename R, bool kForced>
 __device__ __forceinline__ void pair_relax(R* f, R fx, R fy, R fz, const PairMoments<R>& m, R tau0, R tau0sq,
                                            R ksmag) {
     const R r = m.r, ux = m.ux, uy = m.uy, uz = m.uz;
-    const R pxx = m.pxx, pyy = m.pyy, pzz = m.pzz, pxy = m.pxy, pxz = m.pxz, pyz = m.pyz;
-    const R tau = R(0.5) * (tau0 + pair_sqrt(fma(ksmag * m.qn, m.ri, tau0sq)));
-    const R om = pair_rcp(tau);
+    // omega = 1 / tau = 2 / (tau0 + sqrt(tau0^2 + ksmag Q / rho))
+    const R om = R(2) * pair_rcp(tau0 + pair_sqrt(fma(ksmag * m.qn, m.ri, tau0sq)));
     const R om1 = R(1) - om;
-    const R t3 = (pxx + pyy + pzz) * (R(1) / R(3));
-    const R usq = fma(ux, ux, fma(uy, uy, uz * uz));
-    const R k0 = fma(R(-1.5), usq, R(1));
-    const R wr0 = r * (R(1) / R(3)), wr1 = r * (R(1) / R(18)), wr2 = r * (R(1) / R(36));
-    const R g1 = om1 * R(0.25), g2 = om1 * R(0.125);   // om1 * 4.5 w for w = 1/18, 1/36
-    R sf = R(0), uF = R(0), h1 = R(0), h2 = R(0);
+    R lxx = fma(om1, m.pxx, m.jx * ux), lyy = fma(om1, m.pyy, m.jy * uy), lzz = fma(om1, m.pzz, m.jz * uz);
+    R lxy = fma(om1, m.pxy, m.jx * uy), lxz = fma(om1, m.pxz, m.jx * uz), lyz = fma(om1, m.pyz, m.jy * uz);
+    // odd part 3 w (c.J) = (c.J) / 6 (axes), (c.J) / 12 (diagonals)
+    R ax = m.jx, ay = m.jy, az = m.jz;
     if constexpr (kForced) {
-        sf = R(1) - R(0.5) * om;
-        uF = fma(ux, fx, fma(uy, fy, uz * fz));
-        const R odd = fma(R(3), sf, R(-1.5) * om1);   // (3 sf - 1.5 om1)
-        h1 = odd * (R(1) / R(18));
-        h2 = odd * (R(1) / R(36));
+        const R sf = fma(R(-0.5), om, R(1));
+        const R gx = sf * fx, gy = sf * fy, gz = sf * fz;
+        lxx = fma(R(2) * ux, gx, lxx); lyy = fma(R(2) * uy, gy, lyy); lzz = fma(R(2) * uz, gz, lzz);
+        lxy = fma(ux, gy, fma(uy, gx, lxy));
+        lxz = fma(ux, gz, fma(uz, gx, lxz));
+        lyz = fma(uy, gz, fma(uz, gy, lyz));
+        const R h = fma(R(-0.5), om1, sf);
+        ax = fma(h, fx, ax); ay = fma(h, fy, ay); az = fma(h, fz, az);
     }
-    // rest population
-    {
-        R e = fma(om1 * R(1.5), -t3, wr0 * k0);
-        if constexpr (kForced) e = fma(sf * (R(1) / R(3)), R(-3) * uF, e);
-        f[0] = e;
-    }
-    auto pair = [&](int k, R cu, R nq, R cF, R wr, R g, R w, R h) {
-        const R cu2 = cu * cu;
-        R e = fma(g, nq, wr * fma(R(4.5), cu2, k0));
-        R o = R(3) * wr * cu;
-        if constexpr (kForced) {
-            e = fma(w * sf, fma(R(9) * cu, cF, R(-3) * uF), e);
-            o = fma(h, cF, o);
-        }
+    const R T = (lxx + lyy + lzz) * (R(1) / R(3));
+    const R dxx = lxx - T, dyy = lyy - T, dzz = lzz - T;
+    // rest population: w0 = 1/3, Q_0 : L = -T
+    f[0] = fma(R(-1.5), T, r * (R(1) / R(3)));
+    const R wr1 = r * (R(1) / R(18)), wr2 = r * (R(1) / R(36));
+    auto axis = [&](int k, R d, R a) {
+        const R e = fma(R(0.25), d, wr1);          // 4.5 w1 = 0.25
+        const R o = a * (R(1) / R(6));
         f[2 * k + 1] = e + o;
         f[2 * k + 2] = e - o;
     };
-    const R dxx = pxx - t3, dyy = pyy - t3, dzz = pzz - t3;
-    pair(0, ux, dxx, fx, wr1, g1, R(1) / R(18), h1);
-    pair(1, uy, dyy, fy, wr1, g1, R(1) / R(18), h1);
-    pair(2, uz, dzz, fz, wr1, g1, R(1) / R(18), h1);
-    const R sxy = pxx + pyy - t3, sxz = pxx + pzz - t3, syz = pyy + pzz - t3;
-    pair(3, ux + uy, fma(R(2), pxy, sxy), fx + fy, wr2, g2, R(1) / R(36), h2);
-    pair(4, ux - uy, fma(R(-2), pxy, sxy), fx - fy, wr2, g2, R(1) / R(36), h2);
-    pair(5, ux + uz, fma(R(2), pxz, sxz), fx + fz, wr2, g2, R(1) / R(36), h2);
-    pair(6, ux - uz, fma(R(-2), pxz, sxz), fx - fz, wr2, g2, R(1) / R(36), h2);
-    pair(7, uy + uz, fma(R(2), pyz, syz), fy + fz, wr2, g2, R(1) / R(36), h2);
-    pair(8, uy - uz, fma(R(-2), pyz, syz), fy - fz, wr2, g2, R(1) / R(36), h2);
+    axis(0, dxx, ax);
+    axis(1, dyy, ay);
+    axis(2, dzz, az);
+    const R bx = ax * (R(1) / R(12)), by = ay * (R(1) / R(12)), bz = az * (R(1) / R(12));
+    // plane of axes a, b (third axis n): Q:L = (T - d_n) +- 2 L_ab
+    auto plane = [&](int k, R dn, R lab, R oa, R ob) {
+        const R e0 = fma(R(0.125), T - dn, wr2);   // 4.5 w2 = 0.125
+        const R ep = fma(R(0.25), lab, e0), em = fma(R(-0.25), lab, e0);
+        const R op = oa + ob, on = oa - ob;
+        f[2 * k + 1] = ep + op;
+        f[2 * k + 2] = ep - op;
+        f[2 * k + 3] = em + on;
+        f[2 * k + 4] = em - on;
+    };
+    plane(3, dzz, lxy, bx, by);
+    plane(5, dyy, lxz, bx, bz);
+    plane(7, dxx, lyz, by, bz);
 }
 
 template <typename R, bool kForced>
@@ -638,48 +647,76 @@ __device__ __forceinline__ void gather(const typename Codec::T* __restrict__ fs,
     const long long N = a.N;
     const bool xlo = x == 0, xhi = x == a.nx - 1, top = z == a.nz - 1;
     // 1. pull streaming
-    if (!(fl & FLAG_NEARWALL) && !xlo && !xhi && !top) {
-        // interior: every source is the plain upstream neighbour, at a pull base
-        // uniform over the wavefront plus the lane's own offset:
+    if (!(fl & FLAG_NEARWALL)) {
+        // No wall link: every source is the plain upstream neighbour, at a pull
+        // base uniform over the wavefront plus the lane's own offset:
         //   dense:  (i N - (cx + px cy + plane cz)) + c
         //   sparse: (i N + 64 ts - cx) + lane, ts the tile of the (dy, dz) row;
         // stored tiles are numbered row by row with x ascending, so for a cell
         // with no wall link the x-neighbour tile of ts is ts -+ 1 and the lane
-        // that crosses into it needs no special case.
+        // that crosses into it needs no special case. Under the lid (a row
+        // property, uniform) a link with c_z = -1 takes direction MIRZ(i) from
+        // the in-plane source, another uniform base. An x-end lane keeps its
+        // own population for the links entering from outside the box (x-end
+        // precedes the lid): in the wavefronts that hold an x-end (tested
+        // uniformly first) those lanes add a per-direction offset to their
+        // own cell, direction i.
         using T = typename Codec::T;
         constexpr uint32_t ts = (uint32_t)sizeof(T);
-        const uint32_t voff = (uint32_t)(kSparse ? lane : c) * ts;
-        auto row_of = [&](auto I) -> int {   // sparse: source tile of direction I
+        const uint32_t cell = (uint32_t)(kSparse ? lane : c);
+        const bool ends = __any(xlo || xhi);
+        // pull base of direction I, elements (in [0, Q N))
+        auto base_of = [&](auto I) -> long long {
             constexpr int i = I;
-            return nb[1 + 3 * ((1 - kCY[i]) + 3 * (1 - kCZ[i]))];
+            constexpr int cx = kCX[i], cy = kCY[i], cz = kCZ[i];
+            const bool mir = cz == -1 && top;
+            const long long d = mir ? kMIRZ[i] : i;
+            if constexpr (kSparse) {
+                const int rt = mir ? nb[1 + 3 * ((1 - cy) + 3 * 1)] : nb[1 + 3 * ((1 - cy) + 3 * (1 - cz))];
+                return d * N + (long long)rt * 64 - cx;
+            } else {
+                return d * N - (cx + (long long)px * cy + (mir ? 0LL : (long long)plane * cz));
+            }
         };
-        if (a.pop32) {
-            // one resource for the buffer, the pull base as a uniform byte offset
-            const auto rs = buf_rsrc(fs, (uint32_t)Q * (uint32_t)N * ts);
-            static_for<Q>([&](auto I) {
-                constexpr int i = I;
-                uint32_t base;   // elements, exact modulo 2^32 (the true value is in [0, Q N))
-                if constexpr (kSparse)
-                    base = (uint32_t)i * (uint32_t)N + (uint32_t)row_of(I) * 64u - (uint32_t)kCX[i];
-                else
-                    base = (uint32_t)i * (uint32_t)N - (uint32_t)kCX[i] - (uint32_t)px * kCY[i] -
-                           (uint32_t)plane * kCZ[i];
-                f[i] = Codec::template dec<i>(buf_load<T, kLoadAux>(rs, voff, base * ts), a.ctr.c[center_class(i)]);
-            });
-        } else {
-            // a uniform pointer per direction at its pull base (i N + ... >= 0)
-            // and the lane's 32-bit offset
-            static_for<Q>([&](auto I) {
-                constexpr int i = I;
-                long long base;
-                if constexpr (kSparse)
-                    base = (long long)i * N + (long long)row_of(I) * 64 - kCX[i];
-                else
-                    base = (long long)i * N - (kCX[i] + (long long)px * kCY[i] + (long long)plane * kCZ[i]);
-                f[i] = Codec::template ld<i>(elem(fs + base, (uint32_t)(kSparse ? lane : c)),
-                                             a.ctr.c[center_class(i)]);
-            });
-        }
+        auto xout_of = [&](auto I) -> bool {
+            constexpr int cx = kCX[decltype(I)::value];
+            return (cx == 1 && xlo) || (cx == -1 && xhi);
+        };
+        auto load_all = [&](auto with_ends) {
+            constexpr bool kEnds = decltype(with_ends)::value;
+            if (a.pop32) {
+                // one resource for the buffer, the lowest candidate base as a
+                // uniform byte offset; a lane adds its (non-negative) offset
+                const auto rs = buf_rsrc(fs, (uint32_t)Q * (uint32_t)N * ts);
+                const uint32_t vc = cell * ts;
+                static_for<Q>([&](auto I) {
+                    constexpr int i = I;
+                    const uint32_t base = (uint32_t)base_of(I);
+                    uint32_t soff = base, voff = vc;
+                    if constexpr (kEnds && kCX[i] != 0) {
+                        // own cell, direction i: i N + c = lo + (i N + c - cell - lo) + cell
+                        const uint32_t own = (uint32_t)i * (uint32_t)N + ((uint32_t)c - cell);
+                        const uint32_t lo = own < base ? own : base;
+                        soff = lo;
+                        voff = vc + ((xout_of(I) ? own : base) - lo) * ts;
+                    }
+                    f[i] = Codec::template dec<i>(buf_load<T, kLoadAux>(rs, voff, soff * ts), a.ctr.c[center_class(i)]);
+                });
+            } else {
+                // a uniform pointer per direction at its pull base (>= 0) and the
+                // lane's 32-bit offset; an x-end lane's own cell, direction i
+                static_for<Q>([&](auto I) {
+                    constexpr int i = I;
+                    const T* p = elem(fs + base_of(I), cell);
+                    if constexpr (kEnds && kCX[i] != 0) {
+                        if (xout_of(I)) p = elem(fs + (long long)i * N, (uint32_t)c);
+                    }
+                    f[i] = Codec::template ld<i>(p, a.ctr.c[center_class(i)]);
+                });
+            }
+        };
+        if (ends) load_all(std::true_type{});
+        else load_all(std::false_type{});
     } else {
         // Link by link (in this order of precedence): the x-end rows keep their
         // own population (xout: own cell, same direction), a wall link bounces
@@ -1320,14 +1357,16 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
         if (forced) { fx = at(a.aux->Fx, c) * a.fscale; fy = at(a.aux->Fy, c) * a.fscale; fz = at(a.aux->Fz, c) * a.fscale; }
         if (any_forced) pair_moments<R, true>(f, fx, fy, fz, m);
         else pair_moments<R, false>(f, fx, fy, fz, m);
-        // resolved strain rate S = -3 Pi_dev / (2 rho tau), tau from the constant-Cs model
-        const R tau_s = R(0.5) * (a.tau0 + pair_sqrt(fmaf(a.ksmag * m.qn, m.ri, a.tau0sq)));
-        const R k = R(-1.5) * m.ri * __builtin_amdgcn_rcpf(tau_s);
+        // resolved strain rate S = k Pi_dev, k = -3 / (2 rho tau_s) = -3 / (rho (tau0 + sqrt(...)))
+        // with tau_s the constant-Cs relaxation time
+        const R k = R(-3) * m.ri * __builtin_amdgcn_rcpf(a.tau0 + pair_sqrt(fmaf(a.ksmag * m.qn, m.ri, a.tau0sq)));
         const R t3 = (m.pxx + m.pyy + m.pzz) * (R(1) / R(3));
-        closure_features(k * (m.pxx - t3), k * (m.pyy - t3), k * (m.pzz - t3), k * m.pxy, k * m.pxz, k * m.pyz,
-                         m.ux, m.uy, m.uz, at(a.geo, c), at(a.hcol, x + a.px * (row - a.ny * z)), a.inv_u0, feat);
+        closure_features(k, m.pxx - t3, m.pyy - t3, m.pzz - t3, m.pxy, m.pxz, m.pyz, m.ux, m.uy, m.uz, at(a.geo, c),
+                         at(a.hcol, x + a.px * (row - a.ny * z)), a.inv_u0, feat);
     }
-    const float y = mlp_wave(feat, P);
+    // the network runs on wavefronts with a fluid cell (a uniform branch)
+    float y = 0.f;
+    if (__any(active)) y = mlp_wave(feat, P);
     if (active) {
         float c2 = kC2Min + kC2Scale * softplus_f(y);
         // outlet sponge: no less than the constant model's dissipation in the
@@ -1374,8 +1413,11 @@ k_step(const typename Codec::T* __restrict__ fs, typename Codec::T* __restrict__
 
 // kStream: 0 two buffers (fs -> fd); 1 / 2 the odd / even in-place step on
 // the one buffer fd (fs unused).
+#ifndef WFS_LEARNED_WAVES
+#define WFS_LEARNED_WAVES 5
+#endif
 template <class Codec, int kStream = 0>
-__global__ void __launch_bounds__(256) k_step_learned(const typename Codec::T* __restrict__ fs,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WFS_LEARNED_WAVES))) k_step_learned(const typename Codec::T* __restrict__ fs,
                                                       typename Codec::T* __restrict__ fd, StepArgs<float> a) {
     __shared__ MlpPack P;
     int row, tile;
@@ -1391,7 +1433,7 @@ __global__ void __launch_bounds__(256) k_step_learned(const typename Codec::T* _
 constexpr int kTilesPerGroupLearned = 4;
 
 template <class Codec, int G, int kStream = 0>
-__global__ void __launch_bounds__(64 * G) k_step_learned_sparse(const typename Codec::T* __restrict__ fs,
+__global__ void __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(WFS_LEARNED_WAVES))) k_step_learned_sparse(const typename Codec::T* __restrict__ fs,
                                                                typename Codec::T* __restrict__ fd,
                                                                StepArgs<float> a) {
     __shared__ MlpPack P;
