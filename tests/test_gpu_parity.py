@@ -220,61 +220,79 @@ def test_pair_form_collision_matches_fp64_oracle_on_fixture(case):
     L.close()
 
 
-def _fp64_reference_c1(nsteps):
-    reach = channel128()
-    gpu, orc = _run_pair(reach, nsteps, storage="f64", precision="f64")
-    gpu.close()
-    return reach, orc
-
-
 @pytest.mark.parametrize("storage,vel_tol,eta_tol", [
     ("f32fast", 1e-4, 1e-3),
     ("f16dev", 3e-4, 4e-3),
     ("bf16dev", 3e-3, 2.5e-2),
 ])
-def test_storage_formats_against_fp64_after_1000_steps(storage, vel_tol, eta_tol):
+def test_storage_formats_against_fp64_over_1000_steps(storage, vel_tol, eta_tol):
     """Velocity relative L2 and instantaneous level-profile error against the
-    FP64 oracle on the 128x32x16 channel after 1000 steps. The instantaneous
+    FP64 run on the 128x32x16 channel, steps 500 to 1000. The instantaneous
     level of a 16-bit format carries its per-step rounding: FP16 stores
     2 (f - e) with |f - e| ~ 1e-2 w, a half-ulp of about 4e-6 per population,
     about 1e-5 of density per cell, which the level readout multiplies by
     (dx/dt)^2 / (3 g) = 340 m: about 3.4 mm per cell and 0.6 mm for a row of
-    30 cells, so a few mm over 128 rows (2.4 mm measured); BF16 has 3 bits
-    fewer (13 mm measured). The bounds are those estimates; the level bar of
-    1 mm applies to time means (test_gpu_precision).
+    30 cells, so a few mm over 128 rows; BF16 has 3 bits fewer. The bounds
+    are those estimates; the level bar of 1 mm applies to time means
+    (test_gpu_precision).
 
-    After 1000 steps the propeller's LES turbulence has made the state depend
-    on the rounding history: FP16 runs from starts that differ by half an
-    FP16 ulp of the deviations end 1.4e-4 to 5e-4 (velocity) and 2.2 to
-    9.2 mm (level) from FP64, with either form of the collision arithmetic.
-    One run is one draw of that spread, so the bounds apply to the median of
-    five: the plain start and four such perturbed starts (the perturbation
-    is applied to every format alike, including the FP32 row)."""
-    reach, ref = _fp64_reference_c1(1000)
+    The statistic: for each of five starts (the plain one and four perturbed
+    by half an FP16 ulp of the deviations) the mean over the snapshots at
+    steps 500, 600, ..., 1000; the bounds apply to the median over the starts.
+    A single snapshot of a single run is a poor measure here: the 16-bit
+    error concentrates at the hull's bow corners, where the flow separates,
+    (a) FP16 errors there flare up and decay again within 100 steps (one run
+    read 1.3e-4 at step 600, 4.3e-4 at step 700, 1.3e-4 at step 800), and
+    (b) a 16-bit run can settle into the other separation state at one bow
+    cell for good (BF16: 2 starts in 10, 1.8e-2 instead of 1.1e-3). FP64
+    runs from the same perturbed starts stay within 1.3e-5 of the plain one.
+    """
+    reach = channel128()
     args = solver_inputs(reach)
     fl = ~reach.solid.ravel()
-    vref = np.concatenate([ref.ux[fl], ref.uy[fl], ref.uz[fl]])
     w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, np.float32)
-    errs = []
-    for seed in (None, 1, 2, 3, 4):
+
+    def solver(st):
         g = LBM3D(None, args["solid"], args["Fx"], args["Fy"], args["Fz"], args["uin"], args["tau0"], args["cs"],
-                  args["cf_wall"], storage=storage)
+                  args["cf_wall"], storage=st)
         g.set_bedwall(reach.bedwall)
+        return g
+
+    ref = solver("f64")
+    runs = []
+    for seed in (None, 1, 2, 3, 4):
+        g = solver(storage)
         if seed is not None:
             f = g.populations().reshape(19, -1)
             noise = np.random.default_rng(seed).standard_normal(f.shape).astype(np.float32)
             g.set_populations(f + w[:, None] * np.float32(1e-6) * noise)
-        for _ in range(50):
+        runs.append(g)
+    errs = [[] for _ in runs]
+    for k in range(1, 51):
+        for g in [ref] + runs:
             g.update_uref()
             g.step(20)
-        rho, ux, uy, uz = g.macro()
+        if k >= 25 and k % 5 == 0:
+            rr, rx, ry, rz = ref.macro()
+            vref = np.concatenate([rx[fl], ry[fl], rz[fl]])
+            eref = _level_profile(reach, rr)
+            for e, g in zip(errs, runs):
+                rho, ux, uy, uz = g.macro()
+                vel = np.concatenate([ux[fl], uy[fl], uz[fl]])
+                assert np.isfinite(vel).all()
+                e.append((rel_l2(vel, vref), np.abs(_level_profile(reach, rho) - eref).max()))
+    # the FP64 run is the FP64 oracle's (to 1e-12 after 1000 steps)
+    orc = drive(OracleLBM(args["solid"], args["Fx"], args["Fy"], args["Fz"], args["uin"], args["tau0"], args["cs"],
+                          args["cf_wall"], precision="f64"), reach.bedwall, 1000)
+    assert np.abs(ref.macro(np.float64)[1] - orc.ux)[fl].max() < 1e-12
+    for g in [ref] + runs:
         g.close()
-        vel = np.concatenate([ux[fl], uy[fl], uz[fl]])
-        assert np.isfinite(vel).all()
-        errs.append((rel_l2(vel, vref), np.abs(_level_profile(reach, rho) - _level_profile(reach, ref.rho)).max()))
-        print(f"{storage} start {seed}: velocity rel L2 {errs[-1][0]:.3e}, level max error {errs[-1][1] * 1e3:.4f} mm")
-    err_v, err_eta = np.median([e[0] for e in errs]), np.median([e[1] for e in errs])
-    print(f"{storage} median of 5: velocity rel L2 {err_v:.3e}, level {err_eta * 1e3:.4f} mm")
+    means = np.array([np.mean(e, axis=0) for e in errs])
+    for e, m in zip(errs, means):
+        print(storage, "snapshots", [f"{v:.2e}/{h * 1e3:.2f}mm" for v, h in e], "mean", m)
+    err_v, err_eta = np.median(means[:, 0]), np.median(means[:, 1])
+    print(f"{storage} median over starts of the mean over steps 500-1000: velocity rel L2 {err_v:.3e}, "
+          f"level {err_eta * 1e3:.4f} mm")
     assert err_v <= vel_tol
     assert err_eta <= eta_tol
 

@@ -423,9 +423,10 @@ __device__ __forceinline__ double pair_rcp(double v) { return 1.0 / v; }
 __device__ __forceinline__ float pair_sqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
 __device__ __forceinline__ double pair_sqrt(double v) { return sqrt(v); }
 
+// qr = |Pi_neq| / rho, the Smagorinsky term's argument
 template <typename R>
 struct PairMoments {
-    R r, ri, ux, uy, uz, jx, jy, jz, pxx, pyy, pzz, pxy, pxz, pyz, qn;
+    R r, ux, uy, uz, jx, jy, jz, pxx, pyy, pzz, pxy, pxz, pyz, qr;
 };
 
 // kForced: the wavefront holds a forced cell (fx, fy, fz zero on the others,
@@ -459,9 +460,9 @@ __device__ __forceinline__ void pair_moments(const R* f, R fx, R fy, R fz, PairM
     m.pxy = fma(-jx, uy, sp[3] - sp[4]);
     m.pxz = fma(-jx, uz, sp[5] - sp[6]);
     m.pyz = fma(-jy, uz, sp[7] - sp[8]);
-    m.qn = pair_sqrt(fma(m.pxx, m.pxx, fma(m.pyy, m.pyy, m.pzz * m.pzz)) +
-                     R(2) * fma(m.pxy, m.pxy, fma(m.pxz, m.pxz, m.pyz * m.pyz)));
-    m.r = r; m.ri = ri; m.ux = ux; m.uy = uy; m.uz = uz; m.jx = jx; m.jy = jy; m.jz = jz;
+    m.qr = ri * pair_sqrt(fma(m.pxx, m.pxx, fma(m.pyy, m.pyy, m.pzz * m.pzz)) +
+                          R(2) * fma(m.pxy, m.pxy, fma(m.pxz, m.pxz, m.pyz * m.pyz)));
+    m.r = r; m.ux = ux; m.uy = uy; m.uz = uz; m.jx = jx; m.jy = jy; m.jz = jz;
 }
 
 // Relaxation of the pair form with eddy-viscosity constant ksmag = 18 sqrt(2) C^2.
@@ -470,7 +471,7 @@ __device__ __forceinline__ void pair_relax(R* f, R fx, R fy, R fz, const PairMom
                                            R ksmag) {
     const R r = m.r, ux = m.ux, uy = m.uy, uz = m.uz;
     // omega = 1 / tau = 2 / (tau0 + sqrt(tau0^2 + ksmag Q / rho))
-    const R om = R(2) * pair_rcp(tau0 + pair_sqrt(fma(ksmag * m.qn, m.ri, tau0sq)));
+    const R om = R(2) * pair_rcp(tau0 + pair_sqrt(fma(ksmag, m.qr, tau0sq)));
     const R om1 = R(1) - om;
     R lxx = fma(om1, m.pxx, m.jx * ux), lyy = fma(om1, m.pyy, m.jy * uy), lzz = fma(om1, m.pzz, m.jz * uz);
     R lxy = fma(om1, m.pxy, m.jx * uy), lxz = fma(om1, m.pxz, m.jx * uz), lyz = fma(om1, m.pyz, m.jy * uz);
@@ -1359,7 +1360,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
         else pair_moments<R, false>(f, fx, fy, fz, m);
         // resolved strain rate S = k Pi_dev, k = -3 / (2 rho tau_s) = -3 / (rho (tau0 + sqrt(...)))
         // with tau_s the constant-Cs relaxation time
-        const R k = R(-3) * m.ri * __builtin_amdgcn_rcpf(a.tau0 + pair_sqrt(fmaf(a.ksmag * m.qn, m.ri, a.tau0sq)));
+        const R k = R(-3) * __builtin_amdgcn_rcpf(m.r * (a.tau0 + pair_sqrt(fmaf(a.ksmag, m.qr, a.tau0sq))));
         const R t3 = (m.pxx + m.pyy + m.pzz) * (R(1) / R(3));
         closure_features(k, m.pxx - t3, m.pyy - t3, m.pzz - t3, m.pxy, m.pxz, m.pyz, m.ux, m.uy, m.uz, at(a.geo, c),
                          at(a.hcol, x + a.px * (row - a.ny * z)), a.inv_u0, feat);
@@ -1367,7 +1368,13 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     // the network runs on wavefronts with a fluid cell (a uniform branch)
     float y = 0.f;
     if (__any(active)) y = mlp_wave(feat, P);
+    // Less state across the network (more resident waves): the momentum is
+    // formed again from rho and u, and the forces are read again, after it;
+    // the empty asm keeps the compiler from carrying the first reads across.
+    asm volatile("" ::: "memory");
     if (active) {
+        m.jx = m.r * m.ux; m.jy = m.r * m.uy; m.jz = m.r * m.uz;   // u carries the half-force shift
+        if (forced) { fx = at(a.aux->Fx, c) * a.fscale; fy = at(a.aux->Fy, c) * a.fscale; fz = at(a.aux->Fz, c) * a.fscale; }
         float c2 = kC2Min + kC2Scale * softplus_f(y);
         // outlet sponge: no less than the constant model's dissipation in the
         // last kOutletSponge cells before the outlet row
@@ -1413,11 +1420,16 @@ k_step(const typename Codec::T* __restrict__ fs, typename Codec::T* __restrict__
 
 // kStream: 0 two buffers (fs -> fd); 1 / 2 the odd / even in-place step on
 // the one buffer fd (fs unused).
-#ifndef WFS_LEARNED_WAVES
-#define WFS_LEARNED_WAVES 5
-#endif
+// Resident waves per SIMD asked for the learned step: the two-buffer FP32
+// instances fit 6 without spills (0.683 -> 0.666 ms on 23.5 M cells); the
+// 16-bit formats keep 5 (at 6 they spill a register and run 1.5 % slower,
+// the tile-sparse meander 3 %), and so do the in-place instances (at 6 the
+// FP32 ones spill).
+template <class Codec, int kStream>
+constexpr int kLearnedWaves = (sizeof(typename Codec::T) == 4 && kStream == 0) ? 6 : 5;
+
 template <class Codec, int kStream = 0>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WFS_LEARNED_WAVES))) k_step_learned(const typename Codec::T* __restrict__ fs,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kLearnedWaves<Codec, kStream>))) k_step_learned(const typename Codec::T* __restrict__ fs,
                                                       typename Codec::T* __restrict__ fd, StepArgs<float> a) {
     __shared__ MlpPack P;
     int row, tile;
@@ -1433,7 +1445,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WFS_LE
 constexpr int kTilesPerGroupLearned = 4;
 
 template <class Codec, int G, int kStream = 0>
-__global__ void __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(WFS_LEARNED_WAVES))) k_step_learned_sparse(const typename Codec::T* __restrict__ fs,
+__global__ void __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(kLearnedWaves<Codec, kStream>))) k_step_learned_sparse(const typename Codec::T* __restrict__ fs,
                                                                typename Codec::T* __restrict__ fd,
                                                                StepArgs<float> a) {
     __shared__ MlpPack P;
