@@ -219,7 +219,7 @@ def test_trained_network_beats_constant_coefficients():
     """End to end on small reaches: fine runs at dx 0.05 m filtered onto dx
     0.1 m, training on three cases, a priori judgement on a fourth."""
     print("case: closure study, 4 small reaches (3 train, 1 held out)", flush=True)
-    p, res = CL.study(cases=4, held_out=1, seed=10, width=(5.0, 8.0), depth=1.5, length=16.0, dx_c=0.1,
+    p, res = CL.study(n_train=3, held_out=1, seed=10, width=(5.0, 8.0), depth=1.5, length=16.0, dx_c=0.1,
                       snapshots=6, epochs=8, batch=16384, lr=3e-3, posteriori=True, log=lambda m: print(m, flush=True))
     print(res, flush=True)
     tr = res["train"]

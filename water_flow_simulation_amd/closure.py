@@ -34,7 +34,7 @@ network, for Cs = 0.17 and for the best constant coefficient (a priori), and
 coarse runs with either closure against the filtered fine run's time means
 (a posteriori).
 
-    python -m water_flow_simulation_amd.closure [--cases 10 --held-out 4 --dx 0.05 ...]
+    python -m water_flow_simulation_amd.closure [--train 10 --held-out 4 --dx 0.05 --length 20 ...]
 """
 from __future__ import annotations
 
@@ -438,29 +438,41 @@ def aposteriori(coarse, fine_means, cover, closures, fine_profiles=None, storage
 
 
 # ---------------------------------------------------------------- study
-def study(cases=10, held_out=4, seed=0, width=(20.0, 40.0), depth=3.0, length=60.0, dx_c=0.1, snapshots=8,
+def study(n_train=10, held_out=4, seed=0, width=(20.0, 40.0), depth=3.0, length=60.0, dx_c=0.1, snapshots=8,
           epochs=6, batch=65536, lr=3e-3, run_flows=1.0, avg_frac=0.5, posteriori=True, log=print,
-          objective="mixed"):
-    """Fine runs of ``cases`` seeded reaches (the last ``held_out`` kept for
-    evaluation), training, and the a-priori and a-posteriori judgement."""
+          objective="mixed", spare=4):
+    """Fine runs of seeded reaches (seeds seed, seed + 1, ...): the first
+    ``n_train`` that stay finite give the training samples, the next
+    ``held_out`` are kept for the judgement; a fine run that does not stay
+    finite is left out and the next seed takes its place (at most ``spare``
+    such replacements). Then training, and the a-priori and a-posteriori
+    judgement."""
     torch = _torch()
     t0 = time.time()
     train_X, train_P, test = [], [], []
+    seeds = {"train": [], "held_out": [], "left_out": []}
     cells = 0
-    for k in range(cases):
+    k = 0
+    while (len(seeds["train"]) < n_train or len(seeds["held_out"]) < held_out) and \
+            k < n_train + held_out + spare:
         fine, coarse, label = case_pair(seed + k, width, depth, length, dx_c, run_flows, avg_frac)
+        k += 1
         cells += fine.cells
         if log:
-            log(f"case {k}: {label}; fine {fine.nx}x{fine.ny}x{fine.nz}, coarse {coarse.nx}x{coarse.ny}x{coarse.nz}")
+            log(f"case seed {seed + k - 1}: {label}; fine {fine.nx}x{fine.ny}x{fine.nz}, "
+                f"coarse {coarse.nx}x{coarse.ny}x{coarse.nz}")
         got = fine_run(fine, coarse, snapshots, log=log)
         if got is None:
+            seeds["left_out"].append(seed + k - 1)
             continue
         X, ABP, run = got
-        if k < cases - held_out:
+        if len(seeds["train"]) < n_train:
+            seeds["train"].append(seed + k - 1)
             train_X.append(X)
             train_P.append(ABP)
             run.close()
         else:
+            seeds["held_out"].append(seed + k - 1)
             means, cover = filtered_means(run, coarse) if posteriori else (None, None)
             prof = R.profiles(run) if posteriori else None
             run.close()
@@ -477,7 +489,7 @@ def study(cases=10, held_out=4, seed=0, width=(20.0, 40.0), depth=3.0, length=60
     t1 = time.time()
     p = train(X, Y, init_params(seed), epochs, batch, lr, seed, log)
     t_train = time.time() - t1
-    out = {"objective": objective, "samples": int(X.shape[0]), "fine_cells_total": int(cells),
+    out = {"objective": objective, "samples": int(X.shape[0]), "fine_cells_total": int(cells), "seeds": seeds,
            "data_seconds": round(t_data, 1), "train_seconds": round(t_train, 1),
            "train": {**consts, "learned": evaluate(X, Y, p)}, "held_out": []}
     del Y
@@ -508,7 +520,7 @@ def judge(Xh, Ph, p, best_c2, label, coarse=None, means=None, cover=None, prof=N
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--cases", type=int, default=10)
+    ap.add_argument("--train", type=int, default=10, help="training reaches (fine runs that stay finite)")
     ap.add_argument("--held-out", type=int, default=4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--width", type=float, nargs=2, default=(20.0, 40.0))
@@ -524,7 +536,7 @@ def main():
     ap.add_argument("--objective", default="mixed", choices=OBJECTIVES)
     ap.add_argument("--out", default=None, help="write the trained weights (JSON) here")
     a = ap.parse_args()
-    p, res = study(a.cases, a.held_out, a.seed, tuple(a.width), a.depth, a.length, 2 * a.dx, a.snapshots, a.epochs,
+    p, res = study(a.train, a.held_out, a.seed, tuple(a.width), a.depth, a.length, 2 * a.dx, a.snapshots, a.epochs,
                    a.batch, a.lr, a.run_flows, posteriori=not a.no_posteriori,
                    log=lambda m: print(m, flush=True), objective=a.objective)
     print(json.dumps(res), flush=True)
