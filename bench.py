@@ -77,12 +77,13 @@ def closure_weights():
     return CL.load_params()[0] if os.path.exists(CL.WEIGHTS) else CL.init_params(0)
 
 
-def time_gpu(reach, storage, steps, warmup, block_x, learned=False, column_u=False, inplace=False):
+def time_gpu(reach, storage, steps, warmup, block_x, learned=False, column_u=False, inplace=False, pop_stagger=0):
     """column_u: the wall model's column speed refreshed every step inside the
     step kernel instead of once per 20-step chunk; inplace: one population
-    buffer streamed in place."""
+    buffer streamed in place; pop_stagger: bytes between the population
+    direction arrays."""
     from water_flow_simulation_amd.lbm3d import LBM3D
-    lbm = LBM3D(None, *solver_args(reach), storage=storage, block_x=block_x)
+    lbm = LBM3D(None, *solver_args(reach), storage=storage, block_x=block_x, pop_stagger=pop_stagger)
     if inplace:
         lbm.set_inplace(True)
     lbm.set_bedwall(reach.bedwall)
@@ -218,6 +219,9 @@ def main():
     ap.add_argument("--workload", default="trapezoid_tug")
     ap.add_argument("--storage", default="f32", choices=sorted(B_ALG))
     ap.add_argument("--block-x", type=int, default=0)
+    ap.add_argument("--pop-stagger", default="auto",
+                    help="bytes between the population direction arrays of the reach's solvers (0: packed; "
+                         "auto: tuned per storage format on this GPU before timing)")
     ap.add_argument("--variants", default="f32fast,f16dev,bf16dev,f32+mlp,f16dev+mlp,f16dev+colu,f32+inplace,"
                                           "f16dev+inplace",
                     help="extra variants timed (comma list; storage format, '+mlp' for the learned closure, "
@@ -251,7 +255,12 @@ def main():
     def of_ceiling(st, gbs):
         return round(gbs / ceiling[elem[st]], 4) if st in elem else None
     log(f"{reach.nx}x{reach.ny}x{reach.nz} = {reach.cells} cells ({n_fluid} fluid); timing {a.steps} steps")
-    r = time_gpu(reach, a.storage, a.steps, a.warmup, a.block_x)
+    def stagger(st):
+        if a.pop_stagger != "auto":
+            return int(a.pop_stagger)
+        from water_flow_simulation_amd import autotune
+        return autotune.pop_stagger_for(reach, st)
+    r = time_gpu(reach, a.storage, a.steps, a.warmup, a.block_x, pop_stagger=stagger(a.storage))
     log(f"timed: {r['wall']:.3f} s wall, step kernel {r['kernel_ms'] / max(r['launches'], 1):.4f} ms avg")
     mlups = a.steps * reach.cells / r["wall"] / 1e6
     k_avg_s = r["kernel_ms"] / max(r["launches"], 1) / 1e3
@@ -264,7 +273,7 @@ def main():
         "config": {"workload": f"{a.workload}: trapezoidal reach 40 m bed, 1:2 banks, 3 m deep, 150 m at dx 0.1 m, "
                                f"{reach.nx}x{reach.ny}x{reach.nz} cells, one pusher tug (2 x 1.2 m props, 300 kW)",
                    "storage": a.storage, "cells": reach.cells, "fluid_cells": n_fluid, "cadence": 20,
-                   "block_x": a.block_x or 128},
+                   "block_x": a.block_x or 128, "pop_stagger": stagger(a.storage)},
         "mlups_fluid": round(a.steps * n_fluid / r["wall"] / 1e6, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_per_launch(),
@@ -280,7 +289,7 @@ def main():
         st, opts = name.split("+")[0], name.split("+")[1:]
         learned, colu, inplace = "mlp" in opts, "colu" in opts, "inplace" in opts
         log(f"timing variant {name}")
-        rv = time_gpu(reach, st, a.steps, a.warmup, a.block_x, learned, colu, inplace)
+        rv = time_gpu(reach, st, a.steps, a.warmup, a.block_x, learned, colu, inplace, stagger(st))
         kv = rv["kernel_ms"] / max(rv["launches"], 1) / 1e3
         # + the 2-byte geometry code of the closure; + ux, uy written for the per-step column speed
         b_alg = B_ALG[st] + (2 if learned else 0) + (8 if colu else 0)
@@ -288,7 +297,7 @@ def main():
                           "kernel_ms_avg": round(kv * 1e3, 5),
                           "roofline_frac": round(b_alg * n_fluid / kv / 1e9 / PEAK_HBM_GBS, 4),
                           "frac_of_stream_ceiling": of_ceiling(st, b_alg * n_fluid / kv / 1e9),
-                          "speedup_vs_f32": round(r["wall"] / rv["wall"], 3)}
+                          "speedup_vs_f32": round(r["wall"] / rv["wall"], 3), "pop_stagger": stagger(st)}
     if variants:
         out["variants"] = variants
     if a.batch_cases > 0:

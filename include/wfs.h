@@ -157,6 +157,23 @@ int wfs_set_column_u(wfs_solver* h, int32_t per_step);
  * arrays (floodsim/lbm3d.py:50-60); this halves the population memory. */
 int wfs_set_inplace(wfs_solver* h, int32_t on);
 
+/* Population layout: each of the 19 direction arrays starts `bytes` after
+ * the end of the previous one (a multiple of 128 bytes, at most 1 MiB; 0,
+ * the default, packs them). The 19-stream access of the step reaches
+ * different HBM rates for different offsets between its streams (the
+ * address interleave decides; 5.3-6.2 TB/s on one box), so a solver can
+ * take the stagger an autotuner found fastest for its lattice. Between
+ * wfs_create and wfs_upload only (WFS_EINVAL after): the population buffer
+ * is allocated again. Results do not depend on it; checkpoints hold the
+ * populations without the gaps and load across staggers.
+ * No reference counterpart: the reference keeps f[i*N + c]
+ * (floodsim/lbm3d.py:38-60). */
+int wfs_set_pop_stagger(wfs_solver* h, int64_t bytes);
+
+/* Population direction stride in cells (stored cells plus the stagger), or
+ * -1 for a null handle. No reference counterpart. */
+int64_t wfs_pop_stride(const wfs_solver* h);
+
 /* Checkpoints: the raw solver state as one host buffer of wfs_state_bytes
  * bytes. It holds the population buffer in use (device layout and storage
  * format as they are, the in-place phase included), the macroscopic fields,

@@ -91,6 +91,8 @@ _SIGS = {
     "wfs_set_body_mask": ([_P, _P], ctypes.c_int),
     "wfs_set_column_u": ([_P, ctypes.c_int32], ctypes.c_int),
     "wfs_set_inplace": ([_P, ctypes.c_int32], ctypes.c_int),
+    "wfs_set_pop_stagger": ([_P, ctypes.c_int64], ctypes.c_int),
+    "wfs_pop_stride": ([_P], ctypes.c_int64),
     "wfs_state_bytes": ([_P, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
     "wfs_state_save": ([_P, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
     "wfs_state_load": ([_P, ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),

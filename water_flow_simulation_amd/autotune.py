@@ -1,6 +1,6 @@
-"""Launch-shape autotuner.
+"""Launch-shape and layout autotuner.
 
-Two knobs are tuned by measurement on the case itself:
+Three knobs are tuned by measurement on the case itself:
 
 * the step kernel's work-group width (block_x: 64, 128, 192 or 256 cells of
   one x-row per work-group), timed with the library's HIP events around every
@@ -8,7 +8,15 @@ Two knobs are tuned by measurement on the case itself:
   case so the run being tuned is not advanced;
 * for batched sweeps, the number of cases per batched launch (all cases in
   one launch, or groups on separate streams that the GPU overlaps), timed
-  as wall time of a few cadence chunks.
+  as wall time of a few cadence chunks;
+* the gap between the 19 population direction arrays (pop_stagger, 0-3.5 KiB
+  in 512-byte steps): the step's 19 read and 19 write streams reach
+  different HBM rates for different offsets between them, set by the
+  address interleave. On the 1500x522x30 reach the best gap makes the step
+  0.5 % (FP32) to 2 % (BF16) faster than packed arrays, gaps of an odd
+  number of 512-byte units do best and some multiples of 1 KiB are slower
+  than packed, and the ranking is not the same on every GPU, so it is timed
+  on the GPU in use: a scratch solver per candidate, about 2 s per format.
 
 Results are cached per (storage, closure, nx, ny, nz) in memory and in a JSON
 file (WFS_AUTOTUNE_CACHE, default ~/.cache/water_flow_simulation_amd/
@@ -89,6 +97,45 @@ def block_x_for(reach, storage="f32", closure=None, retune=False):
     return int(best)
 
 
+STAGGERS = (0, 512, 1024, 1536, 2048, 2560, 3072, 3584)
+
+
+def time_staggers(reach, storage="f32", steps=60, warmup=10, staggers=STAGGERS, rounds=2):
+    """Average step-kernel milliseconds at each population stagger (bytes),
+    each on a scratch solver of the case, the candidates taken in turn
+    ``rounds`` times (the best round counts)."""
+    from .reach3d import Run3D
+    out = {s: float("inf") for s in staggers}
+    for _ in range(rounds):
+        for s in staggers:
+            run = Run3D(reach, storage=storage, pop_stagger=s)
+            try:
+                lbm = run.lbm
+                lbm.step(warmup)
+                lbm.timing(True)
+                lbm.step(steps)
+                ms, n = lbm.timing_read()
+                lbm.timing(False)
+                out[s] = min(out[s], ms / max(n, 1))
+            finally:
+                run.close()
+    return out
+
+
+def pop_stagger_for(reach, storage="f32", retune=False):
+    """The tuned population stagger for this case shape and storage format
+    (tunes on first use; the closure does not change the access pattern)."""
+    key = "stagger/" + _key(reach, storage, None)
+    c = _load()
+    if not retune and key in c:
+        return int(c[key]["pop_stagger"])
+    t = time_staggers(reach, storage)
+    best = min(t, key=t.get)
+    _store(key, {"pop_stagger": int(best), "kernel_ms": {str(k): round(v, 5) for k, v in t.items()},
+                 "tuned": time.strftime("%Y-%m-%d %H:%M:%S")})
+    return int(best)
+
+
 def tune_group(batch, steps=40, candidates=None):
     """Times the batch with several cases-per-launch settings (each for
     ``steps`` steps of the real run, so tuning is part of the spin-up) and
@@ -119,4 +166,6 @@ if __name__ == "__main__":
     r = workloads.trapezoid_tug()
     for st in ("f32", "f16dev"):
         t = time_widths(r, st)
-        print(st, {k: round(v, 4) for k, v in t.items()}, "best", min(t, key=t.get), flush=True)
+        print(st, "block_x", {k: round(v, 4) for k, v in t.items()}, "best", min(t, key=t.get), flush=True)
+        t = time_staggers(r, st)
+        print(st, "pop_stagger", {k: round(v, 4) for k, v in t.items()}, "best", min(t, key=t.get), flush=True)

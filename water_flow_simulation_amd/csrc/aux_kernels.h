@@ -53,8 +53,9 @@ __device__ __forceinline__ void dense_of(const CellMap& m, long long c, int& x, 
 // keeps its own value. S -> A-B -> S is exact. A-B -> S drops only the
 // populations that leave the box through an x-end row (no step reads them);
 // A-B -> S -> A-B then gives the entering population of the pair there.
+// N cells, population direction stride Ns.
 template <typename T>
-__global__ void k_aa_exchange(T* __restrict__ f, long long N, CellMap m) {
+__global__ void k_aa_exchange(T* __restrict__ f, long long N, long long Ns, CellMap m) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
     int x;
@@ -68,8 +69,8 @@ __global__ void k_aa_exchange(T* __restrict__ f, long long N, CellMap m) {
             static_assert(o == i + 1, "opposite directions are adjacent pairs");
             const bool keep_i = (kCX[i] == 1 && xlo) || (kCX[i] == -1 && xhi);
             const bool keep_o = (kCX[o] == 1 && xlo) || (kCX[o] == -1 && xhi);
-            T* pi = f + (long long)i * N + c;
-            T* po = f + (long long)o * N + c;
+            T* pi = f + (long long)i * Ns + c;
+            T* po = f + (long long)o * Ns + c;
             const T vi = *pi, vo = *po;
             *pi = keep_i ? vi : vo;
             *po = keep_o ? vo : vi;
@@ -122,10 +123,10 @@ __global__ void k_unpad_add(const S* __restrict__ dev, D* __restrict__ dense, Ce
 
 // Equilibrium at rho = 1, u = (u0, 0, 0), evaluated in double and stored once
 // through the codec; u0 = uin of the cell's row, 0 in solid cells
-// (floodsim/lbm3d.py:362-364, 377-384).
+// (floodsim/lbm3d.py:362-364, 377-384). N cells, direction stride Ns.
 template <class Codec>
 __global__ void k_init(typename Codec::T* __restrict__ f, const uint8_t* __restrict__ flag,
-                       const typename Codec::R* __restrict__ uin, CellMap m, long long N, Center ctr) {
+                       const typename Codec::R* __restrict__ uin, CellMap m, long long N, long long Ns, Center ctr) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
     int x;
@@ -138,7 +139,7 @@ __global__ void k_init(typename Codec::T* __restrict__ f, const uint8_t* __restr
         const double w = i == 0 ? W[0] : (i < 7 ? W[1] : W[2]);
         const double cu = 3.0 * kCX[i] * u;
         const double v = w * (1.0 + cu + 0.5 * cu * cu - 1.5 * u * u);
-        Codec::template st<i>(f + (long long)i * N + c, (typename Codec::R)v, store_key<Codec>(ctr, i));
+        Codec::template st<i>(f + (long long)i * Ns + c, (typename Codec::R)v, store_key<Codec>(ctr, i));
     });
 }
 
@@ -387,8 +388,8 @@ __global__ void k_fill_macro(R* __restrict__ rho, R* __restrict__ ux, R* __restr
     uz[c] = R(0);
 }
 
-// Populations of dense cells [k0, k0 + n): padded storage -> real array
-// out[i*n + (k - k0)], and back.
+// Populations of dense cells [k0, k0 + n): padded storage (direction stride
+// Npad) -> real array out[i*n + (k - k0)], and back.
 template <class Codec, typename O>
 __global__ void k_decode(const typename Codec::T* __restrict__ f, O* __restrict__ out, CellMap m,
                          long long Npad, long long k0, long long n, Center ctr) {

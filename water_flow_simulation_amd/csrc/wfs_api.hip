@@ -148,6 +148,7 @@ struct wfs_solver {
     wfs_desc d{};
     long long N = 0;    // dense cell count nx*ny*nz (host arrays)
     long long Np = 0;   // device cell count px*ny*nz (rows padded to px)
+    long long Ns = 0;   // population direction stride in elements: Np plus the stagger (wfs_set_pop_stagger)
     int px = 0;         // row pitch: nx rounded up to a multiple of 64
     int64_t nstep = 0;
     int32_t navg = 0;
@@ -195,6 +196,7 @@ struct wfs_solver {
         return WFS_OK;
     }
     virtual int alloc() = 0;
+    virtual int set_pop_stagger(long long bytes) = 0;
     virtual int upload(const uint8_t* solid, const float* cf, const float* uin) = 0;
     virtual int set_force(const float* Fx, const float* Fy, const float* Fz) = 0;
     virtual int set_force_scale(float s) = 0;
@@ -309,7 +311,7 @@ struct Solver final : wfs_solver {
         int rc;
         // the second population buffer comes with the upload, unless the
         // solver streams in place by then
-        if ((rc = alloc_buf(f[0], n * Q * sizeof(T)))) return rc;
+        if ((rc = alloc_buf(f[0], (size_t)Ns * Q * sizeof(T)))) return rc;
         if ((rc = alloc_buf(flag, n))) return rc;
         if ((rc = alloc_buf(wmask, n * sizeof(uint32_t)))) return rc;
         for (DevBuf* b : {&Fx, &Fy, &Fz, &rho, &ux, &uy, &uz, &cfw, &uref})
@@ -429,6 +431,25 @@ struct Solver final : wfs_solver {
         return WFS_OK;
     }
 
+    // Population direction arrays `bytes` apart beyond their Np cells (a
+    // multiple of 128 bytes, at most 1 MiB; 0 packs them). Before the upload
+    // only: the population buffer is allocated again.
+    int set_pop_stagger(long long stagger) override {
+        if (uploaded) return fail(WFS_EINVAL, "wfs_set_pop_stagger: set it before the upload");
+        if (stagger < 0 || stagger % 128 || stagger > (1ll << 20))
+            return fail(WFS_EINVAL, "wfs_set_pop_stagger: the stagger must be a multiple of 128 bytes in [0, 1 MiB]");
+        const long long ns = Np + stagger / (long long)sizeof(T);
+        if ((unsigned long long)ns * sizeof(T) >= (1ull << 32))
+            return fail(WFS_EINVAL, "wfs_set_pop_stagger: a population direction would exceed 4 GiB");
+        if (ns == Ns) return WFS_OK;
+        for (auto& b : f) {
+            bytes -= b.bytes;
+            b = DevBuf();
+        }
+        Ns = ns;
+        return alloc_buf(f[0], (size_t)Ns * Q * sizeof(T));
+    }
+
     int upload(const uint8_t* solid, const float* cf, const float* uin_h) override {
         int rc;
         const int32_t dims[4] = {d.nx, d.ny, d.nz, tiled ? 1 : 0};
@@ -436,7 +457,7 @@ struct Solver final : wfs_solver {
         geo_hash = fingerprint(solid, (size_t)N, geo_hash);
         if (cf) geo_hash = fingerprint(cf, (size_t)N * sizeof(float), geo_hash);
         for (int k = 0; k < 2; ++k)
-            if (!inplace && !f[k].p && (rc = alloc_buf(f[k], (size_t)Np * Q * sizeof(T)))) return rc;
+            if (!inplace && !f[k].p && (rc = alloc_buf(f[k], (size_t)Ns * Q * sizeof(T)))) return rc;
         if (cf) {
             if ((rc = put_cells(cfw, cf))) return rc;
         } else if ((rc = zero(cfw))) {
@@ -462,8 +483,9 @@ struct Solver final : wfs_solver {
         }
         for (int k = 0; k < 2; ++k) {
             if (!f[k].p) continue;   // in-place streaming keeps one buffer
+            if (Ns != Np && (rc = zero(f[k]))) return rc;   // the gaps between direction arrays hold zeros
             hipLaunchKernelGGL(k_init<Codec>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, fp(k), flags(),
-                               rp(uin), cmap(), Np, ctr);
+                               rp(uin), cmap(), Np, Ns, ctr);
             WFS_LAUNCH_CHECK();
         }
         if (inplace) {
@@ -621,10 +643,10 @@ struct Solver final : wfs_solver {
         a.px = px;
         a.rows = rows();
         a.tiles = (d.nx + bx - 1) / bx;
-        a.N = Np;
+        a.N = Ns;
         a.uin = rp(uin);
         a.bc_out_fused = (d.nx - 1) % 64 != 0 ? 1 : 0;
-        a.pop32 = (unsigned long long)Q * (unsigned long long)Np * sizeof(T) < (1ull << 32) ? 1 : 0;
+        a.pop32 = (unsigned long long)Q * (unsigned long long)Ns * sizeof(T) < (1ull << 32) ? 1 : 0;
         a.write_all = 0;
         a.ctr = ctr;
         a.geo = static_cast<const uint16_t*>(geo.p);
@@ -652,7 +674,7 @@ struct Solver final : wfs_solver {
 
     // A-B layout <-> state S of the in-place buffer (the same exchange both ways).
     int aa_exchange() {
-        hipLaunchKernelGGL(k_aa_exchange<T>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, fp(cur), Np, cmap());
+        hipLaunchKernelGGL(k_aa_exchange<T>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, fp(cur), Np, Ns, cmap());
         WFS_LAUNCH_CHECK();
         return WFS_OK;
     }
@@ -688,7 +710,8 @@ struct Solver final : wfs_solver {
             return WFS_OK;
         }
         if ((rc = aa_readable("wfs_set_inplace"))) return rc;
-        if ((rc = alloc_buf(f[cur ^ 1], (size_t)Np * Q * sizeof(T)))) return rc;
+        if ((rc = alloc_buf(f[cur ^ 1], (size_t)Ns * Q * sizeof(T)))) return rc;
+        if (Ns != Np && (rc = zero(f[cur ^ 1]))) return rc;
         if (uploaded && (rc = aa_exchange())) return rc;
         WFS_HIP(hipStreamSynchronize(stream));
         inplace = false;
@@ -782,7 +805,7 @@ struct Solver final : wfs_solver {
             }
             if (!a.bc_out_fused) {   // x = nx-1 opens a wavefront: outlet row in its own pass
                 hipLaunchKernelGGL(k_bc<Codec>, dim3(blocks_for(a.rows, 64)), dim3(64), 0, stream, fp(cur ^ 1),
-                                   flags(), rp(uin), d.nx, px, a.rows, Np, R(1), 2, ctr);
+                                   flags(), rp(uin), d.nx, px, a.rows, Ns, R(1), 2, ctr);
                 WFS_LAUNCH_CHECK();
             }
             if (inplace)
@@ -835,10 +858,10 @@ struct Solver final : wfs_solver {
             const long long k0 = j0 * d.nx, n = (j1 - j0) * d.nx;
             if (f64)
                 hipLaunchKernelGGL((k_decode<Codec, double>), dim3(blocks_for(n, 256)), dim3(256), 0, stream,
-                                   fp(cur), static_cast<double*>(stage.p), cmap(), Np, k0, n, ctr);
+                                   fp(cur), static_cast<double*>(stage.p), cmap(), Ns, k0, n, ctr);
             else
                 hipLaunchKernelGGL((k_decode<Codec, float>), dim3(blocks_for(n, 256)), dim3(256), 0, stream,
-                                   fp(cur), static_cast<float*>(stage.p), cmap(), Np, k0, n, ctr);
+                                   fp(cur), static_cast<float*>(stage.p), cmap(), Ns, k0, n, ctr);
             WFS_LAUNCH_CHECK();
             for (int i = 0; i < Q; ++i)
                 WFS_HIP(hipMemcpyAsync(static_cast<char*>(out) + ((size_t)i * N + k0) * esz,
@@ -869,7 +892,7 @@ struct Solver final : wfs_solver {
                 WFS_HIP(hipMemcpyAsync(static_cast<float*>(stage.p) + (size_t)i * n, in + (size_t)i * N + k0,
                                        (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
             hipLaunchKernelGGL(k_encode<Codec>, dim3(blocks_for(n, 256)), dim3(256), 0, stream, fp(cur),
-                               static_cast<const float*>(stage.p), cmap(), Np, k0, n, ctr);
+                               static_cast<const float*>(stage.p), cmap(), Ns, k0, n, ctr);
             WFS_LAUNCH_CHECK();
         }
         WFS_HIP(hipStreamSynchronize(stream));
@@ -895,19 +918,26 @@ struct Solver final : wfs_solver {
         int32_t nx, ny, nz, pad;
         uint64_t geo_hash;
     };
-    std::vector<std::pair<DevBuf*, size_t>> state_parts() {
-        std::vector<std::pair<DevBuf*, size_t>> v;
-        v.push_back({&f[cur], (size_t)Np * Q * sizeof(T)});
-        for (DevBuf* b : {&rho, &ux, &uy, &uz, &uref}) v.push_back({b, (size_t)Np * sizeof(R)});
-        for (auto& sb : sums) v.push_back({&sb, (size_t)Np * sizeof(R)});
+    // (buffer, byte offset, bytes) of each piece; the populations direction by
+    // direction, without the gaps of a staggered layout, so a state loads
+    // into a solver of any population stagger
+    struct StatePart {
+        DevBuf* b;
+        size_t off, n;
+    };
+    std::vector<StatePart> state_parts() {
+        std::vector<StatePart> v;
+        for (int i = 0; i < Q; ++i) v.push_back({&f[cur], (size_t)i * Ns * sizeof(T), (size_t)Np * sizeof(T)});
+        for (DevBuf* b : {&rho, &ux, &uy, &uz, &uref}) v.push_back({b, 0, (size_t)Np * sizeof(R)});
+        for (auto& sb : sums) v.push_back({&sb, 0, (size_t)Np * sizeof(R)});
         if (colu_on)
             for (auto& pr : colu)
-                for (auto& b : pr) v.push_back({&b, (size_t)Np * sizeof(R)});
+                for (auto& b : pr) v.push_back({&b, 0, (size_t)Np * sizeof(R)});
         return v;
     }
     int state_bytes(long long* out) override {
         size_t n = sizeof(StateHead);
-        for (auto& pr : state_parts()) n += pr.second;
+        for (auto& pr : state_parts()) n += pr.n;
         *out = (long long)n;
         return WFS_OK;
     }
@@ -924,8 +954,8 @@ struct Solver final : wfs_solver {
         WFS_HIP(hipStreamSynchronize(stream));
         char* o = static_cast<char*>(out) + sizeof h;
         for (auto& pr : state_parts()) {
-            WFS_HIP(hipMemcpy(o, pr.first->p, pr.second, hipMemcpyDeviceToHost));
-            o += pr.second;
+            WFS_HIP(hipMemcpy(o, static_cast<const char*>(pr.b->p) + pr.off, pr.n, hipMemcpyDeviceToHost));
+            o += pr.n;
         }
         return WFS_OK;
     }
@@ -952,8 +982,8 @@ struct Solver final : wfs_solver {
         WFS_HIP(hipStreamSynchronize(stream));
         const char* p = static_cast<const char*>(in) + sizeof h;
         for (auto& pr : state_parts()) {
-            WFS_HIP(hipMemcpy(pr.first->p, p, pr.second, hipMemcpyHostToDevice));
-            p += pr.second;
+            WFS_HIP(hipMemcpy(static_cast<char*>(pr.b->p) + pr.off, p, pr.n, hipMemcpyHostToDevice));
+            p += pr.n;
         }
         nstep = h.nstep;
         navg = h.navg;
@@ -1651,6 +1681,7 @@ int wfs_create(const wfs_desc* desc, wfs_solver** out) {
     if ((unsigned long long)h->Np * rsz >= (1ull << 32))
         return fail(WFS_EINVAL, "wfs_create: one field or population direction would exceed 4 GiB "
                                 "(at most 2^30 padded cells with float fields, 2^29 with FP64 storage)");
+    h->Ns = h->Np;
     h->device = d.device;
     DeviceGuard g(d.device);
     WFS_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -1752,6 +1783,7 @@ int wfs_create_tiled(const wfs_desc* desc, const uint8_t* solid, wfs_solver** ou
     const unsigned long long rsz = d.storage == WFS_STORE_F64 ? 8ull : 4ull;
     if (h->Np >= (1ll << 31) || (unsigned long long)h->Np * rsz >= (1ull << 32))
         return fail(WFS_EINVAL, "wfs_create_tiled: stored cells exceed 4 GiB per field or population direction");
+    h->Ns = h->Np;
     h->device = d.device;
     DeviceGuard g(d.device);
     WFS_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
@@ -2234,6 +2266,14 @@ int wfs_set_column_u(wfs_solver* h, int32_t per_step) {
     DeviceGuard g(h->device);
     return h->set_column_u(per_step);
 }
+
+int wfs_set_pop_stagger(wfs_solver* h, int64_t bytes) {
+    if (!h) return fail(WFS_EINVAL, "wfs_set_pop_stagger: null handle");
+    DeviceGuard g(h->device);
+    return h->set_pop_stagger((long long)bytes);
+}
+
+int64_t wfs_pop_stride(const wfs_solver* h) { return h ? (int64_t)h->Ns : -1; }
 
 int wfs_set_inplace(wfs_solver* h, int32_t on) {
     if (!h) return fail(WFS_EINVAL, "wfs_set_inplace: null handle");

@@ -23,8 +23,11 @@ read-only. ``xp`` is numpy: the arrays a caller gets and gives are host
 arrays (the reference's ``xp`` is the array module of its backend,
 lbm3d.py:348-351). ``be`` is accepted for signature compatibility and only a
 torch backend is rejected, as in the reference (lbm3d.py:349-350).
-Keyword-only extensions choose the population storage, the work-group width
-and the device, and give the device-side cadence helpers used by ``Run3D``.
+Keyword-only extensions choose the population storage, the work-group width,
+the device, the layout, in-place streaming and the gap between population
+direction arrays (``pop_stagger``, bytes: a layout choice with no effect on
+results, tuned by autotune.pop_stagger_for), and give the device-side cadence helpers
+used by ``Run3D``.
 """
 from __future__ import annotations
 
@@ -45,7 +48,7 @@ class LBM3D:
     xp = np
 
     def __init__(self, be, solid, Fx, Fy, Fz, uin, tau0, cs=0.17, cf_wall=None, *, storage="f32", block_x=0,
-                 device=0, layout="dense", streaming="ab"):
+                 device=0, layout="dense", streaming="ab", pop_stagger=0):
         if be is not None and getattr(be, "name", None) == "torch":
             raise RuntimeError("LBM3D: the 3-D model runs on the HIP library, not on a torch backend")
         if storage not in nat.STORAGE:
@@ -87,6 +90,9 @@ class LBM3D:
         self._uin = A(uin)
         if self._uin.size != self.ny * self.nz:
             raise ValueError("LBM3D: uin must be (nz, ny)")
+        if pop_stagger:               # bytes between population direction arrays (layout only)
+            nat.check(L.wfs_set_pop_stagger(h, int(pop_stagger)), "wfs_set_pop_stagger")
+        self.pop_stagger = int(pop_stagger)
         if streaming == "inplace":    # before the upload: the second buffer is never allocated
             self.set_inplace(True)
         nat.check(L.wfs_upload(h, nat.ptr(sol), nat.ptr(cfw), nat.ptr(self._uin)), "wfs_upload")

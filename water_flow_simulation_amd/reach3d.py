@@ -454,7 +454,7 @@ class Run3D:
     cadence = 20
 
     def __init__(self, reach: Reach, be=None, *, storage="f32", block_x=0, device=0, closure=None,
-                 column_u="cadence", streaming="ab"):
+                 column_u="cadence", streaming="ab", pop_stagger=0):
         """closure: None for the constant-Cs Smagorinsky model, or the weights
         of the learned closure (closure.py), evaluated inside the step.
         block_x: work-group width, 0 for the default, "auto" for the tuned one.
@@ -463,7 +463,9 @@ class Run3D:
         refreshes it every step inside the step kernel.
         streaming: "ab" streams between two population buffers, as the
         reference does; "inplace" keeps one (half the population memory, the
-        same results bit for bit)."""
+        same results bit for bit).
+        pop_stagger: bytes between the population direction arrays (a layout
+        choice, results unchanged), "auto" for the tuned one."""
         if column_u not in ("cadence", "per_step"):
             raise ValueError("column_u: 'cadence' or 'per_step'")
         if streaming not in ("ab", "inplace"):
@@ -475,11 +477,16 @@ class Run3D:
             block_x = block_x_for(reach, storage, closure)
         elif block_x == "auto":
             block_x = 0
+        if pop_stagger == "auto" and reach.layout == "dense":
+            from .autotune import pop_stagger_for
+            pop_stagger = pop_stagger_for(reach, storage)
+        elif pop_stagger == "auto":
+            pop_stagger = 0
         nu_lb = max(1e-6 * reach.dt / reach.dx ** 2, 2e-4)
         uin_lb = (reach.uin_phys / reach.vel).astype(np.float32)
         self.lbm = LBM3D(be, reach.solid, reach.Fx, reach.Fy, reach.Fz, uin_lb, tau0=0.5 + 3 * nu_lb,
                          cs=reach.cfg.cs, cf_wall=reach.cf, storage=storage, block_x=block_x, device=device,
-                         layout=reach.layout, streaming=streaming)
+                         layout=reach.layout, streaming=streaming, pop_stagger=pop_stagger)
         if reach.force_cells is not None:
             self.lbm.set_force_cells(*reach.force_cells)
         if closure is not None:
