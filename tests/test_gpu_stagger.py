@@ -103,3 +103,43 @@ def test_stagger_autotuner(tmp_path, monkeypatch):
     tiles = R.Run3D(replace(channel128(), layout="tiles"), pop_stagger="auto")
     assert tiles.lbm.pop_stagger == 0
     tiles.close()
+
+
+def test_released_buffers_return_device_memory():
+    """Dropping a buffer (a new stagger re-allocates the populations,
+    in-place streaming switched on after the upload frees the second buffer,
+    the column buffers switched off) gives its memory back to the device, and
+    the solver's count follows."""
+    import ctypes
+    from water_flow_simulation_amd.lbm3d import LBM3D
+    hip = ctypes.CDLL("libamdhip64.so.7")    # the runtime instance the solver library uses
+
+    def free_bytes():
+        f, t = ctypes.c_size_t(), ctypes.c_size_t()
+        assert hip.hipDeviceSynchronize() == 0
+        assert hip.hipMemGetInfo(ctypes.byref(f), ctypes.byref(t)) == 0
+        return f.value
+    print("case: 500x128x64 box, f32, stagger 1 KiB, in place after the upload, column buffers on and off", flush=True)
+    nz, ny, nx = 64, 128, 500
+    solid = np.zeros((nz, ny, nx), bool)
+    solid[0] = True
+    z3 = np.zeros(solid.shape, np.float32)
+    uin = np.full((nz, ny), 0.02, np.float32)
+    nat.device_count()                       # the device is initialised
+    free0 = free_bytes()
+    g = LBM3D(None, solid, z3, z3, z3, uin, 0.51, pop_stagger=1024)
+    g.step(2)
+    g.set_inplace(True)
+    bw = np.zeros(solid.shape, np.uint8)
+    bw[1] = 1                                # the bed layer carries the wall model
+    g.set_bedwall(bw)
+    g.set_column_u(True)
+    g.step(2)
+    g.set_column_u(False)
+    g.step(2)
+    held = g.memory_mb() * 1e6
+    used = free0 - free_bytes()
+    pops = 19 * 4 * g.stored_cells
+    assert used < held + 0.25 * pops, (used, held, pops)   # a dropped buffer would add one or more pops
+    g.close()
+    assert free0 - free_bytes() < 0.05 * pops

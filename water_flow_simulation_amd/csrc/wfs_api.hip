@@ -133,10 +133,28 @@ int scratch_of(int device, double** out) {
     return WFS_OK;
 }
 
-// Device buffer owned by a solver.
+// Device buffer owned by a solver: moved, never copied; assigning an empty
+// one (buf = DevBuf()) frees what it held.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            if (p) (void)hipFree(p);
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
     ~DevBuf() {
         if (p) (void)hipFree(p);
     }
@@ -251,7 +269,15 @@ struct wfs_solver {
     virtual int set_force_cells(long long n, const int64_t* idx, const float* fx, const float* fy,
                                 const float* fz) = 0;
 
+    // Frees a buffer and takes it off the device-memory count.
+    void release(DevBuf& b) {
+        bytes -= b.bytes;
+        b = DevBuf();
+    }
+
+    // (Re)allocates b; what it held before is freed first.
     int alloc_buf(DevBuf& b, size_t bytes_) {
+        if (b.p) release(b);
         bytes_ = (bytes_ + 255) & ~size_t(255);  // whole 256-byte blocks: fills and vector access stay in bounds
         b.bytes = bytes_;
         if (hipMalloc(&b.p, bytes_) != hipSuccess) {
@@ -442,10 +468,8 @@ struct Solver final : wfs_solver {
         if ((unsigned long long)ns * sizeof(T) >= (1ull << 32))
             return fail(WFS_EINVAL, "wfs_set_pop_stagger: a population direction would exceed 4 GiB");
         if (ns == Ns) return WFS_OK;
-        for (auto& b : f) {
-            bytes -= b.bytes;
-            b = DevBuf();
-        }
+        WFS_HIP(hipStreamSynchronize(stream));
+        for (auto& b : f) release(b);
         Ns = ns;
         return alloc_buf(f[0], (size_t)Ns * Q * sizeof(T));
     }
@@ -703,8 +727,7 @@ struct Solver final : wfs_solver {
             WFS_HIP(hipStreamSynchronize(stream));
             if (uploaded && (rc = aa_exchange())) return rc;
             WFS_HIP(hipStreamSynchronize(stream));
-            bytes -= f[cur ^ 1].bytes;
-            f[cur ^ 1] = DevBuf();
+            release(f[cur ^ 1]);
             inplace = true;
             aa_phase = 0;
             return WFS_OK;
@@ -1125,9 +1148,10 @@ struct Solver final : wfs_solver {
     int set_column_u(int per_step) override {
         if (!per_step) {
             colu_on = false;
+            WFS_HIP(hipStreamSynchronize(stream));
             for (auto& p : colu)
-                for (auto& b : p) b = DevBuf();
-            colcount = DevBuf();
+                for (auto& b : p) release(b);
+            release(colcount);
             return WFS_OK;
         }
         if (d.nz > 255) return fail(WFS_EINVAL, "wfs_set_column_u: more than 255 cells per column");
@@ -1154,8 +1178,9 @@ struct Solver final : wfs_solver {
     // place, so only positions cross to the host.
     int particles_set(long long n, const double* xyz, long long ninlet, const int32_t* inlet, uint64_t seed) override {
         if (n < 0 || ninlet < 0) return fail(WFS_EINVAL, "wfs_particles_set: negative count");
-        part_xyz = DevBuf();
-        part_inlet = DevBuf();
+        WFS_HIP(hipStreamSynchronize(stream));
+        release(part_xyz);
+        release(part_inlet);
         part_n = 0;
         if (n == 0) return WFS_OK;
         int rc;
@@ -1204,7 +1229,8 @@ struct Solver final : wfs_solver {
     // of 64 solid cells): the budget looks up the wall cell of every link.
     int set_body_mask(const uint8_t* m) override {
         if (!m) {
-            body = DevBuf();
+            WFS_HIP(hipStreamSynchronize(stream));
+            release(body);
             return WFS_OK;
         }
         const size_t words = (size_t)((N + 31) / 32);
