@@ -382,6 +382,11 @@ int wfs_set_closure(wfs_solver* h, int32_t kind, const float* params, const uint
 /* C^2 of every cell as used in the last step of the previous wfs_step call
  * (learned closure; dense (nz, ny, nx) float). */
 int wfs_get_closure_field(wfs_solver* h, float* c2);
+/* on != 0 (the default): the learned step keeps the C^2 field that
+ * wfs_get_closure_field reads (4 bytes per stored cell); 0 frees it, for runs
+ * that do not read it. No reference counterpart (the reference holds no
+ * learned closure). */
+int wfs_set_closure_field(wfs_solver* h, int32_t on);
 
 /* Training samples from a fine run (its state after the last wfs_step call):
  * the velocity is box-filtered 2x2x2 onto the coarse grid ncx x ncy x ncz

@@ -95,8 +95,9 @@ def test_block_width_switch_mid_run_and_autotuner(tmp_path, monkeypatch):
 
 def test_batched_case_refuses_direct_stepping_and_state_changes():
     """While a batch holds a solver, stepping it, loading a state or
-    replacing its populations or macroscopic fields directly is refused (the
-    batch's next run would overwrite it); once the batch is gone it works."""
+    replacing its populations, macroscopic fields or forces directly is
+    refused (the batch's next run would overwrite it, or it holds the old
+    force table); once the batch is gone it works."""
     print("case: batch of 2 reaches, direct calls on a held case", flush=True)
     reaches, _ = ensemble(2, seed=3, scale=0.2)
     b = BatchRun(reaches)
@@ -104,8 +105,9 @@ def test_batched_case_refuses_direct_stepping_and_state_changes():
     L = b.runs[0].lbm
     st = L.state()
     rho, ux, uy, uz = L.macro()
+    one = np.zeros(1, np.float32)
     for call in (lambda: L.step(1), lambda: L.load_state(st), lambda: L.set_populations(L.populations()),
-                 lambda: L.set_macro(rho, ux, uy, uz)):
+                 lambda: L.set_macro(rho, ux, uy, uz), lambda: L.set_force_cells(np.array([0]), one, one, one)):
         with pytest.raises(ValueError, match="case of a batch"):
             call()
     assert L.nstep == 20

@@ -71,5 +71,6 @@ def test_full_meander_fp16_in_place_learned():
     for name, f in res["forces_N"].items():
         assert np.all(np.isfinite(f)), name
     assert 0.0 < abs(res["forces_N"]["thrust"][0]) < 0.01 * res["thrust_design_N"]
-    assert res["device_gb"] < 85.0, res["device_gb"]
+    # 38 B of FP16 populations, 29 B of per-cell fields and 2 B of closure geometry per stored cell
+    assert res["device_gb"] < 55.0, res["device_gb"]
     print("kernel ms", res["kernel_ms"], "device GB", res["device_gb"], flush=True)

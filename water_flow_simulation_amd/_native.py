@@ -105,6 +105,7 @@ _SIGS = {
     "wfs_block_x": ([_P], _I32),
     "wfs_set_closure": ([_P, _I32, _P, _P, _P, ctypes.c_double], ctypes.c_int),
     "wfs_get_closure_field": ([_P, _P], ctypes.c_int),
+    "wfs_set_closure_field": ([_P, ctypes.c_int32], ctypes.c_int),
     "wfs_closure_samples": ([_P, _P, _P, _I32, _I32, _I32, ctypes.c_double, ctypes.c_double, _P, _P, _P],
                             ctypes.c_int),
     "wfs_mlp_work_floats": ([ctypes.c_int64], ctypes.c_int64),

@@ -183,29 +183,19 @@ __global__ void k_flags(const uint8_t* __restrict__ solid, const R* __restrict__
     wmask[c] = wm;
 }
 
-// Three values per listed dense cell written into device fields (forces given
-// as a cell list). Cells whose tile is not stored are skipped.
+// FLAG_FORCED where any force component of the group side table
+// (StepAux::fslot / fval) is nonzero.
 template <typename R>
-__global__ void k_scatter3(const int64_t* __restrict__ idx, const float* __restrict__ v, long long n, CellMap m,
-                           R* __restrict__ a, R* __restrict__ b, R* __restrict__ c3) {
-    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const long long j = idx[k] / m.nx;
-    const long long c = cell_of(m, int(idx[k] - j * m.nx), j);
-    if (c < 0) return;
-    a[c] = R(v[k]);
-    b[c] = R(v[n + k]);
-    c3[c] = R(v[2 * n + k]);
-}
-
-// FLAG_FORCED where any force component is nonzero.
-template <typename R>
-__global__ void k_forced(uint8_t* __restrict__ flag, const R* __restrict__ Fx, const R* __restrict__ Fy,
-                         const R* __restrict__ Fz, long long N) {
+__global__ void k_forced(uint8_t* __restrict__ flag, const int* __restrict__ fslot, const R* __restrict__ fval,
+                         long long N) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= N) return;
     uint8_t fl = flag[c] & uint8_t(~FLAG_FORCED);
-    if (!(fl & FLAG_SOLID) && (Fx[c] != R(0) || Fy[c] != R(0) || Fz[c] != R(0))) fl |= FLAG_FORCED;
+    const int s = fslot[c / kForceGroup];
+    if (!(fl & FLAG_SOLID) && s >= 0) {
+        const R* v = fval + (long long)s * kForceSlot + c % kForceGroup;
+        if (v[0] != R(0) || v[kForceGroup] != R(0) || v[2 * kForceGroup] != R(0)) fl |= FLAG_FORCED;
+    }
     flag[c] = fl;
 }
 
@@ -214,8 +204,8 @@ __global__ void k_forced(uint8_t* __restrict__ flag, const R* __restrict__ Fx, c
 // z = 0..nz-1 in order.
 template <typename R>
 __device__ __forceinline__ void uref_column(const R* __restrict__ ux, const R* __restrict__ uy,
-                                            const uint8_t* __restrict__ flag, const uint8_t* __restrict__ bedwall,
-                                            R* __restrict__ uref, CellMap m, int nz) {
+                                            const uint8_t* __restrict__ flag, R* __restrict__ uref, CellMap m,
+                                            int nz) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= m.nx) return;
@@ -232,14 +222,23 @@ __device__ __forceinline__ void uref_column(const R* __restrict__ ux, const R* _
     const R U = sqrt(sx * sx + sy * sy) / ncol;
     for (int z = 0; z < nz; ++z) {
         const long long c = cell_of(m, x, y + (long long)m.ny * z);
-        if (c >= 0) uref[c] = bedwall[c] ? U : R(0);
+        if (c >= 0) uref[c] = (flag[c] & FLAG_BEDWALL) ? U : R(0);
     }
 }
 
 template <typename R>
 __global__ void k_uref(const R* __restrict__ ux, const R* __restrict__ uy, const uint8_t* __restrict__ flag,
-                       const uint8_t* __restrict__ bedwall, R* __restrict__ uref, CellMap m, int nz) {
-    uref_column(ux, uy, flag, bedwall, uref, m, nz);
+                       R* __restrict__ uref, CellMap m, int nz) {
+    uref_column(ux, uy, flag, uref, m, nz);
+}
+
+// FLAG_BEDWALL from a per-cell byte mask (nonzero: bed or bank cell of the
+// wall model).
+__global__ void k_bedwall_flag(uint8_t* __restrict__ flag, const uint8_t* __restrict__ bw, long long N) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const uint8_t fl = flag[c] & uint8_t(~FLAG_BEDWALL);
+    flag[c] = bw[c] ? uint8_t(fl | FLAG_BEDWALL) : fl;
 }
 
 // Fluid cells per column (at least 1), for the per-step column velocity.
@@ -353,10 +352,10 @@ __global__ void k_batch_post(CaseSched<R>* __restrict__ sch, int* __restrict__ a
 // sequential run would).
 template <typename R>
 __global__ void k_uref_if(const CaseSched<R>* __restrict__ sch, int k, const R* __restrict__ ux,
-                          const R* __restrict__ uy, const uint8_t* __restrict__ flag,
-                          const uint8_t* __restrict__ bedwall, R* __restrict__ uref, CellMap m, int nz) {
+                          const R* __restrict__ uy, const uint8_t* __restrict__ flag, R* __restrict__ uref,
+                          CellMap m, int nz) {
     if (sch[k].active <= 0) return;
-    uref_column(ux, uy, flag, bedwall, uref, m, nz);
+    uref_column(ux, uy, flag, uref, m, nz);
 }
 
 // k_accumulate for one case, skipped when its schedule says so.
@@ -489,9 +488,11 @@ __device__ __forceinline__ void wall_force_cell(const typename Codec::T* __restr
         const uint8_t fl = x < a.nx ? a.flag[c] : uint8_t(FLAG_SOLID);
         if (!(fl & FLAG_SOLID)) {
             if (fl & FLAG_FORCED) {
-                acc[0] += double(a.aux->Fx[c] * a.fscale);
-                acc[1] += double(a.aux->Fy[c] * a.fscale);
-                acc[2] += double(a.aux->Fz[c] * a.fscale);
+                R fx, fy, fz;
+                body_force(*a.aux, int(c), a.fscale, fx, fy, fz);
+                acc[0] += double(fx);
+                acc[1] += double(fy);
+                acc[2] += double(fz);
             }
             if (fl & FLAG_NEARWALL) {
                 const uint32_t wm = a.wmask[c];

@@ -44,6 +44,7 @@ enum : uint8_t {
     FLAG_NEARWALL = 1u << 1,  // at least one pull source is a wall (bounce-back link)
     FLAG_CF = 1u << 2,        // wall-model friction coefficient > 0
     FLAG_FORCED = 1u << 3,    // actuator-disk body force present
+    FLAG_BEDWALL = 1u << 4,   // bed or bank cell of the wall model (takes the column speed as uref)
 };
 
 }  // namespace wfs

@@ -290,18 +290,22 @@ __device__ __forceinline__ void store_cell(typename Codec::T* __restrict__ fd, l
 // in device memory behind one pointer, so a kernel loads each where it is
 // used; as kernel arguments the compiler loads them all at entry and spills
 // them to vector-register lanes.
+// Body forces are few (actuator disks), so they are held per 64-cell group
+// (cells 64 g .. 64 g + 63) rather than per cell: fslot[g] is the group's slot
+// or -1, and fval holds per slot 64 values of Fx, then of Fy, then of Fz.
+constexpr int kForceGroup = 64;
+constexpr int kForceSlot = 3 * kForceGroup;
+
 template <typename R>
 struct StepAux {
-    const R* Fx;
-    const R* Fy;
-    const R* Fz;
+    const int* fslot;
+    const R* fval;
     R* rho;
     R* ux;
     R* uy;
     R* uz;
     const R* cfw;
     const R* uref;
-    const uint8_t* bedwall;   // per cell: bed or bank cell of the wall model
     const uint8_t* colcount;  // per column (x + px y): fluid cells, at least 1
     // per-step column velocity for the wall model (when StepArgs::colu_on):
     // every step writes ux, uy of its fluid cells to colu_out_*, and a bed/bank
@@ -312,6 +316,16 @@ struct StepAux {
     R* colu_out_x;
     R* colu_out_y;
 };
+
+// Force on cell c, one of FLAG_FORCED (its group has a slot), times s.
+template <typename R>
+__device__ __forceinline__ void body_force(const StepAux<R>& ax, int c, R s, R& fx, R& fy, R& fz) {
+    const unsigned u = unsigned(c);
+    const int b = ax.fslot[u / kForceGroup] * kForceSlot + int(u % kForceGroup);
+    fx = ax.fval[b] * s;
+    fy = ax.fval[b + kForceGroup] * s;
+    fz = ax.fval[b + 2 * kForceGroup] * s;
+}
 
 template <typename R>
 struct StepArgs {
@@ -621,7 +635,7 @@ __device__ __forceinline__ void wall_function(const StepArgs<typename Codec::R>&
                 });
                 if (K > R(0)) {
                     const R ur = !a.colu_on ? at(ax.uref, c)
-                                 : (at(ax.bedwall, c) ? column_u<kSparse>(a, x, y) : R(0));
+                                 : ((fl & FLAG_BEDWALL) ? column_u<kSparse>(a, x, y) : R(0));
                     const R U = ur > R(0) ? ur : ut;
                     const R us0 = (R(-2) * D - rw * cf * U * U * area) / (R(6) * rw * K);
                     const R us = fmin(fmax(us0, -ut), ut);
@@ -868,7 +882,7 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename Codec::R>& 
         const bool forced = (fl & FLAG_FORCED) != 0;
         R fx = R(0), fy = R(0), fz = R(0);
         if (forced) {
-            fx = at(a.aux->Fx, c) * a.fscale; fy = at(a.aux->Fy, c) * a.fscale; fz = at(a.aux->Fz, c) * a.fscale;
+            body_force(*a.aux, int(c), a.fscale, fx, fy, fz);
             ux = (mx + R(0.5) * fx) / r; uy = (my + R(0.5) * fy) / r; uz = (mz + R(0.5) * fz) / r;
         } else {
             ux = mx / r; uy = my / r; uz = mz / r;
@@ -939,7 +953,7 @@ __device__ __forceinline__ void collide_cell(const StepArgs<typename Codec::R>& 
         const bool forced = (fl & FLAG_FORCED) != 0;
         if (__any(forced)) {   // uniform over the wavefront; forced cells are few
             R fx = R(0), fy = R(0), fz = R(0);
-            if (forced) { fx = at(a.aux->Fx, c) * a.fscale; fy = at(a.aux->Fy, c) * a.fscale; fz = at(a.aux->Fz, c) * a.fscale; }
+            if (forced) body_force(*a.aux, int(c), a.fscale, fx, fy, fz);
             collide_pairs<R, true>(f, fx, fy, fz, a.tau0, a.tau0sq, a.ksmag, r, ux, uy, uz);
         } else {
             collide_pairs<R, false>(f, R(0), R(0), R(0), a.tau0, a.tau0sq, a.ksmag, r, ux, uy, uz);
@@ -1355,7 +1369,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
             aa_gather<Codec, kStream == 1, kSparse>(fd, a, c, x, z, wm, f, nb, lane);
             wall_function<Codec, kSparse>(a, c, x, row - a.ny * z, fl, wm, f);
         }
-        if (forced) { fx = at(a.aux->Fx, c) * a.fscale; fy = at(a.aux->Fy, c) * a.fscale; fz = at(a.aux->Fz, c) * a.fscale; }
+        if (forced) body_force(*a.aux, int(c), a.fscale, fx, fy, fz);
         if (any_forced) pair_moments<R, true>(f, fx, fy, fz, m);
         else pair_moments<R, false>(f, fx, fy, fz, m);
         // resolved strain rate S = k Pi_dev, k = -3 / (2 rho tau_s) = -3 / (rho (tau0 + sqrt(...)))
@@ -1374,7 +1388,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     asm volatile("" ::: "memory");
     if (active) {
         m.jx = m.r * m.ux; m.jy = m.r * m.uy; m.jz = m.r * m.uz;   // u carries the half-force shift
-        if (forced) { fx = at(a.aux->Fx, c) * a.fscale; fy = at(a.aux->Fy, c) * a.fscale; fz = at(a.aux->Fz, c) * a.fscale; }
+        if (forced) body_force(*a.aux, int(c), a.fscale, fx, fy, fz);
         float c2 = kC2Min + kC2Scale * softplus_f(y);
         // outlet sponge: no less than the constant model's dissipation in the
         // last kOutletSponge cells before the outlet row

@@ -248,6 +248,7 @@ struct wfs_solver {
     virtual int forces(double* out) = 0;
     virtual int forces_noalloc(double* out) = 0;
     virtual int get_closure_field(float* c2) = 0;
+    virtual int set_closure_field(int on) = 0;
     virtual int closure_samples(const uint16_t* geo, const uint8_t* hcol, int ncx, int ncy, int ncz, double kv,
                                 double u0, float* X, float* ABP, uint8_t* valid) = 0;
     int closure = WFS_CLOSURE_SMAGORINSKY;
@@ -300,9 +301,10 @@ template <class Codec>
 struct Solver final : wfs_solver {
     using T = typename Codec::T;
     using R = typename Codec::R;
-    DevBuf f[2], flag, wmask, Fx, Fy, Fz, rho, ux, uy, uz, cfw, uref, uin, bedwall;
+    DevBuf f[2], flag, wmask, fslot, fval, rho, ux, uy, uz, cfw, uref, uin;
     DevBuf sums[6], partial, out_x, counter, stage;
     DevBuf geo, hcol, mlp_params, mlp_pack, c2buf;   // learned closure (allocated when first set)
+    bool c2_keep = true;                              // c2buf kept (wfs_set_closure_field)
     DevBuf body;                                      // hull cells for the force budget (when set): one bit per dense cell
     DevBuf colu[2][2], colcount;                      // per-step column velocity: [parity][ux, uy]; fluid cells per column
     DevBuf part_xyz, part_inlet, part_count;          // particle tracer: positions, inlet rows, respawn counter
@@ -340,12 +342,9 @@ struct Solver final : wfs_solver {
         if ((rc = alloc_buf(f[0], (size_t)Ns * Q * sizeof(T)))) return rc;
         if ((rc = alloc_buf(flag, n))) return rc;
         if ((rc = alloc_buf(wmask, n * sizeof(uint32_t)))) return rc;
-        for (DevBuf* b : {&Fx, &Fy, &Fz, &rho, &ux, &uy, &uz, &cfw, &uref})
+        for (DevBuf* b : {&rho, &ux, &uy, &uz, &cfw, &uref})
             if ((rc = alloc_buf(*b, n * sizeof(R)))) return rc;
         if ((rc = alloc_buf(uin, (size_t)rows() * sizeof(R)))) return rc;
-        if ((rc = alloc_buf(bedwall, n))) return rc;
-        for (auto& s : sums)
-            if ((rc = alloc_buf(s, n * sizeof(R)))) return rc;
         if ((rc = alloc_buf(out_x, (size_t)d.nx * sizeof(double)))) return rc;
         if ((rc = alloc_buf(partial, (size_t)d.nx * 256 * sizeof(double)))) return rc;
         if ((rc = alloc_buf(counter, 256))) return rc;
@@ -519,8 +518,8 @@ struct Solver final : wfs_solver {
         hipLaunchKernelGGL(k_fill_macro<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, rp(rho), rp(ux),
                            rp(uy), rp(uz), Np);
         WFS_LAUNCH_CHECK();
-        for (DevBuf* b : {&Fx, &Fy, &Fz, &uref, &bedwall})
-            if ((rc = zero(*b))) return rc;
+        if ((rc = zero(uref))) return rc;
+        if ((rc = put_forces({}, {}))) return rc;   // no body forces
         if ((rc = reset_sums())) return rc;
         WFS_HIP(hipStreamSynchronize(stream));
         if (!inplace) cur = 0;
@@ -557,39 +556,79 @@ struct Solver final : wfs_solver {
         }
     }
 
-    int set_force(const float* a, const float* b, const float* c) override {
-        int rc;
-        if ((rc = put_cells(Fx, a)) || (rc = put_cells(Fy, b)) || (rc = put_cells(Fz, c))) return rc;
+    // Stored cell of dense (x, row j), or -1 where the tile-sparse layout
+    // stores no tile (the host twin of cell_of).
+    long long cell_host(int x, long long j) const {
+        if (!tiled) return x + (long long)px * j;
+        const int t = h_tmap[(size_t)((x >> 6) + (long long)ntx * j)];
+        return t < 0 ? -1 : (long long)t * 64 + (x & 63);
+    }
+
+    // Replaces the body forces by the listed stored cells (fx, fy, fz each;
+    // a later entry of the same cell wins) through the 64-cell group side
+    // table: a slot per group with a listed cell, every other group none.
+    int put_forces(const std::vector<long long>& cells, const std::vector<float>& v) {
+        const long long ng = Np / kForceGroup;
+        std::vector<int> slot((size_t)ng, -1);
+        int ns = 0;
+        for (long long c : cells) {
+            int& sl = slot[(size_t)(c / kForceGroup)];
+            if (sl < 0) sl = ns++;
+        }
+        std::vector<R> val((size_t)std::max(ns, 1) * kForceSlot, R(0));
+        for (size_t k = 0; k < cells.size(); ++k) {
+            R* dst = val.data() + (size_t)slot[(size_t)(cells[k] / kForceGroup)] * kForceSlot + cells[k] % kForceGroup;
+            dst[0] = R(v[3 * k]);
+            dst[kForceGroup] = R(v[3 * k + 1]);
+            dst[2 * kForceGroup] = R(v[3 * k + 2]);
+        }
+        DevBuf ns_d, nv_d;
+        int rc = alloc_buf(ns_d, slot.size() * sizeof(int));
+        if (!rc) rc = alloc_buf(nv_d, val.size() * sizeof(R));
+        if (rc) {
+            release(ns_d);
+            release(nv_d);
+            return rc;
+        }
+        WFS_HIP(hipStreamSynchronize(stream));   // steps in flight read the old table
+        WFS_HIP(hipMemcpy(ns_d.p, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice));
+        WFS_HIP(hipMemcpy(nv_d.p, val.data(), val.size() * sizeof(R), hipMemcpyHostToDevice));
+        release(fslot);
+        release(fval);
+        fslot = std::move(ns_d);
+        fval = std::move(nv_d);
         hipLaunchKernelGGL(k_forced<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
-                           static_cast<uint8_t*>(flag.p), rp(Fx), rp(Fy), rp(Fz), Np);
+                           static_cast<uint8_t*>(flag.p), static_cast<const int*>(fslot.p), rp(fval), Np);
         WFS_LAUNCH_CHECK();
-        return WFS_OK;
+        return sync_aux();
+    }
+
+    int set_force(const float* a, const float* b, const float* c) override {
+        std::vector<long long> cells;
+        std::vector<float> v;
+        for (long long k = 0; k < N; ++k)
+            if (a[k] != 0.f || b[k] != 0.f || c[k] != 0.f) {
+                const long long j = k / d.nx;
+                const long long cell = cell_host(int(k - j * d.nx), j);
+                if (cell < 0) continue;
+                cells.push_back(cell);
+                v.insert(v.end(), {a[k], b[k], c[k]});
+            }
+        return put_forces(cells, v);
     }
 
     int set_force_cells(long long n, const int64_t* idx, const float* fx, const float* fy,
                         const float* fz) override {
-        int rc;
-        for (DevBuf* b : {&Fx, &Fy, &Fz})
-            if ((rc = zero(*b))) return rc;
-        if (n > 0) {
-            DevBuf di, dv;
-            if ((rc = alloc_buf(di, (size_t)n * sizeof(int64_t))) || (rc = alloc_buf(dv, (size_t)n * 3 * sizeof(float))))
-                return rc;
-            WFS_HIP(hipMemcpyAsync(di.p, idx, (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice, stream));
-            float* v = static_cast<float*>(dv.p);
-            WFS_HIP(hipMemcpyAsync(v, fx, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
-            WFS_HIP(hipMemcpyAsync(v + n, fy, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
-            WFS_HIP(hipMemcpyAsync(v + 2 * n, fz, (size_t)n * sizeof(float), hipMemcpyHostToDevice, stream));
-            hipLaunchKernelGGL(k_scatter3<R>, dim3(blocks_for(n, 256)), dim3(256), 0, stream,
-                               static_cast<const int64_t*>(di.p), v, n, cmap(), rp(Fx), rp(Fy), rp(Fz));
-            WFS_LAUNCH_CHECK();
-            WFS_HIP(hipStreamSynchronize(stream));
-            bytes -= di.bytes + dv.bytes;
+        std::vector<long long> cells;
+        std::vector<float> v;
+        for (long long k = 0; k < n; ++k) {
+            const long long j = idx[k] / d.nx;
+            const long long cell = cell_host(int(idx[k] - j * d.nx), j);
+            if (cell < 0) continue;   // a cell of a tile that is not stored
+            cells.push_back(cell);
+            v.insert(v.end(), {fx[k], fy[k], fz[k]});
         }
-        hipLaunchKernelGGL(k_forced<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
-                           static_cast<uint8_t*>(flag.p), rp(Fx), rp(Fy), rp(Fz), Np);
-        WFS_LAUNCH_CHECK();
-        return WFS_OK;
+        return put_forces(cells, v);
     }
 
     int set_force_scale(float s) override {
@@ -600,8 +639,19 @@ struct Solver final : wfs_solver {
     int set_uin(const float* u) override { return put_rows(uin, u); }
     int set_uref(const float* u) override { return put_cells(uref, u); }
 
+    // The mask becomes the FLAG_BEDWALL bit of the flag byte (through a
+    // transient per-cell byte buffer).
     int set_bedwall(const uint8_t* bw) override {
-        int rc = put_bytes(bedwall, bw);
+        DevBuf tmp;
+        int rc = alloc_buf(tmp, (size_t)Np);
+        if (!rc) rc = put_bytes(tmp, bw);
+        if (!rc) {
+            hipLaunchKernelGGL(k_bedwall_flag, dim3(blocks_for(Np, 256)), dim3(256), 0, stream,
+                               static_cast<uint8_t*>(flag.p), static_cast<const uint8_t*>(tmp.p), Np);
+            if (hipGetLastError() != hipSuccess) rc = fail(WFS_EHIP, "wfs_set_bedwall: launch");
+            if (hipStreamSynchronize(stream) != hipSuccess && !rc) rc = fail(WFS_EHIP, "wfs_set_bedwall: synchronize");
+        }
+        release(tmp);
         if (rc) return rc;
         have_bedwall = true;
         return WFS_OK;
@@ -610,7 +660,7 @@ struct Solver final : wfs_solver {
     int update_uref() override {
         if (!have_bedwall) return fail(WFS_EINVAL, "wfs_update_uref: no bed/bank mask set (wfs_set_bedwall)");
         hipLaunchKernelGGL(k_uref<R>, dim3(blocks_for(d.nx, 64), d.ny), dim3(64), 0, stream, rp(ux), rp(uy), flags(),
-                           static_cast<const uint8_t*>(bedwall.p), rp(uref), cmap(), d.nz);
+                           rp(uref), cmap(), d.nz);
         WFS_LAUNCH_CHECK();
         return WFS_OK;
     }
@@ -621,16 +671,14 @@ struct Solver final : wfs_solver {
     int sync_aux() {
         StepAux<R> v[2]{};
         for (int p = 0; p < 2; ++p) {
-            v[p].Fx = rp(Fx);
-            v[p].Fy = rp(Fy);
-            v[p].Fz = rp(Fz);
+            v[p].fslot = static_cast<const int*>(fslot.p);
+            v[p].fval = rp(fval);
             v[p].rho = rp(rho);
             v[p].ux = rp(ux);
             v[p].uy = rp(uy);
             v[p].uz = rp(uz);
             v[p].cfw = rp(cfw);
             v[p].uref = rp(uref);
-            v[p].bedwall = static_cast<const uint8_t*>(bedwall.p);
             v[p].colcount = static_cast<const uint8_t*>(colcount.p);
             if (colu_on) {
                 v[p].colu_in_x = rp(colu[p][0]);
@@ -977,7 +1025,8 @@ struct Solver final : wfs_solver {
         WFS_HIP(hipStreamSynchronize(stream));
         char* o = static_cast<char*>(out) + sizeof h;
         for (auto& pr : state_parts()) {
-            WFS_HIP(hipMemcpy(o, static_cast<const char*>(pr.b->p) + pr.off, pr.n, hipMemcpyDeviceToHost));
+            if (pr.b->p) WFS_HIP(hipMemcpy(o, static_cast<const char*>(pr.b->p) + pr.off, pr.n, hipMemcpyDeviceToHost));
+            else std::memset(o, 0, pr.n);   // time-mean sums not allocated: all zero
             o += pr.n;
         }
         return WFS_OK;
@@ -1005,7 +1054,13 @@ struct Solver final : wfs_solver {
         WFS_HIP(hipStreamSynchronize(stream));
         const char* p = static_cast<const char*>(in) + sizeof h;
         for (auto& pr : state_parts()) {
-            WFS_HIP(hipMemcpy(static_cast<char*>(pr.b->p) + pr.off, p, pr.n, hipMemcpyHostToDevice));
+            if (!pr.b->p) {   // time-mean sums not allocated: only a state that holds some allocates them
+                const bool nonzero = std::any_of(p, p + pr.n, [](char ch) { return ch != 0; });
+                int rc = nonzero ? ensure_sums() : WFS_OK;
+                if (rc) return rc;
+                WFS_HIP(hipStreamSynchronize(stream));
+            }
+            if (pr.b->p) WFS_HIP(hipMemcpy(static_cast<char*>(pr.b->p) + pr.off, p, pr.n, hipMemcpyHostToDevice));
             p += pr.n;
         }
         nstep = h.nstep;
@@ -1016,7 +1071,24 @@ struct Solver final : wfs_solver {
         return WFS_OK;
     }
 
+    // The six time-mean sums (6 values per stored cell) are allocated, zeroed,
+    // by the first accumulation or read: a run that never averages holds none.
+    int ensure_sums() {
+        if (sums[0].p) return WFS_OK;
+        for (auto& sb : sums) {
+            int rc = alloc_buf(sb, (size_t)Np * sizeof(R));
+            if (!rc) rc = zero(sb);
+            if (rc) {
+                for (auto& b : sums) release(b);
+                return rc;
+            }
+        }
+        return WFS_OK;
+    }
+
     int accumulate() override {
+        int rc = ensure_sums();
+        if (rc) return rc;
         hipLaunchKernelGGL(k_accumulate<R>, dim3(blocks_for(Np, 256)), dim3(256), 0, stream, rp(rho), rp(ux), rp(uy),
                            rp(uz), rp(sums[0]), rp(sums[1]), rp(sums[2]), rp(sums[3]), rp(sums[4]), rp(sums[5]), Np);
         WFS_LAUNCH_CHECK();
@@ -1025,16 +1097,19 @@ struct Solver final : wfs_solver {
     }
 
     int reset_sums() override {
-        for (auto& s : sums) {
-            int rc = zero(s);
-            if (rc) return rc;
-        }
+        if (sums[0].p)
+            for (auto& sb : sums) {
+                int rc = zero(sb);
+                if (rc) return rc;
+            }
         navg = 0;
         return WFS_OK;
     }
 
     // sums[3] holds the sum of rho - 1; the readout is the sum of rho.
     int get_sums(void* const* out, bool f64) override {
+        int rc0 = ensure_sums();
+        if (rc0) return rc0;
         for (int k = 0; k < 6; ++k) {
             int rc = k == 3 ? get_cells_add(sums[k], out[k], f64, (double)navg) : get_cells(sums[k], out[k], f64);
             if (rc) return rc;
@@ -1067,6 +1142,10 @@ struct Solver final : wfs_solver {
     int profile_dev(int field, int z0, int z1) {
         if (field < 0 || field > WFS_F_ONE) return fail(WFS_EINVAL, "wfs_profile_x: unknown field");
         if (z0 < 0 || z1 > d.nz || z0 >= z1) return fail(WFS_EINVAL, "wfs_profile_x: bad z range");
+        if (field >= WFS_F_SUM_UX && field <= WFS_F_SUM_UXX) {
+            int rc = ensure_sums();
+            if (rc) return rc;
+        }
         const R* v = field_ptr(field);
         const int j0 = z0 * d.ny, j1 = z1 * d.ny;
         const int nrows = j1 - j0;
@@ -1118,9 +1197,10 @@ struct Solver final : wfs_solver {
             if (!geo.p) {
                 if ((rc = alloc_buf(geo, (size_t)Np * sizeof(uint16_t))) ||
                     (rc = alloc_buf(hcol, (size_t)px * d.ny)) || (rc = alloc_buf(mlp_params, kNParams * sizeof(float))) ||
-                    (rc = alloc_buf(mlp_pack, sizeof(MlpPack))) || (rc = alloc_buf(c2buf, (size_t)Np * sizeof(float))))
+                    (rc = alloc_buf(mlp_pack, sizeof(MlpPack))))
                     return rc;
             }
+            if (c2_keep && !c2buf.p && (rc = alloc_buf(c2buf, (size_t)Np * sizeof(float)))) return rc;
             if ((rc = put_dense<uint16_t, uint16_t>(geo, geo_h, uint16_t(0)))) return rc;
             const long long ncol = (long long)px * d.ny;
             WFS_HIP(hipMemcpyAsync(stage.p, hcol_h, (size_t)d.nx * d.ny, hipMemcpyHostToDevice, stream));
@@ -1133,7 +1213,7 @@ struct Solver final : wfs_solver {
             hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(256), 0, stream, static_cast<const float*>(mlp_params.p),
                                static_cast<MlpPack*>(mlp_pack.p));
             WFS_LAUNCH_CHECK();
-            if ((rc = zero(c2buf))) return rc;
+            if (c2buf.p && (rc = zero(c2buf))) return rc;
             WFS_HIP(hipStreamSynchronize(stream));
             inv_u0 = (float)(1.0 / u0);
             closure = kind;
@@ -1324,8 +1404,24 @@ struct Solver final : wfs_solver {
     }
 
     int get_closure_field(float* out) override {
-        if (!c2buf.p) return fail(WFS_EINVAL, "wfs_get_closure_field: no learned closure set");
+        if (!c2buf.p) return fail(WFS_EINVAL, "wfs_get_closure_field: no learned closure set, or its field is off "
+                                              "(wfs_set_closure_field)");
         return get_cells(c2buf, out, false);
+    }
+
+    // The learned step records C^2 per cell (4 bytes per stored cell) only
+    // while this is on; off frees the field. On again, it holds zeros until
+    // the next step.
+    int set_closure_field(int on) override {
+        c2_keep = on != 0;
+        if (!c2_keep) {
+            WFS_HIP(hipStreamSynchronize(stream));
+            release(c2buf);
+            return WFS_OK;
+        }
+        if (closure != WFS_CLOSURE_MLP || c2buf.p) return WFS_OK;
+        int rc = alloc_buf(c2buf, (size_t)Np * sizeof(float));
+        return rc ? rc : zero(c2buf);
     }
 
     // Training samples from this (fine) run, box-filtered 2x2x2 onto the
@@ -1430,6 +1526,8 @@ struct Batch final : BatchBase {
             if (c->inplace) return fail(WFS_EINVAL, "wfs_batch_create: batched cases use two population buffers");
             if (c->closure != WFS_CLOSURE_SMAGORINSKY)
                 return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the constant-Cs closure");
+            int rs = c->ensure_sums();   // the batch accumulates into them
+            if (rs) return rs;
             WFS_HIP(hipStreamSynchronize(c->stream));
             cases.push_back(c);
             ++c->batch_refs;
@@ -1524,7 +1622,7 @@ struct Batch final : BatchBase {
             if (!c->have_bedwall) continue;
             hipLaunchKernelGGL(k_uref_if<R>, dim3(blocks_for(c->d.nx, 64), c->d.ny), dim3(64), 0, stream,
                                static_cast<const CaseSched<R>*>(ds), (int)k, c->rp(c->ux), c->rp(c->uy), c->flags(),
-                               static_cast<const uint8_t*>(c->bedwall.p), c->rp(c->uref), c->cmap(), c->d.nz);
+                               c->rp(c->uref), c->cmap(), c->d.nz);
             WFS_LAUNCH_CHECK();
         }
         const CaseArgs<Codec>* da = static_cast<const CaseArgs<Codec>*>(d_args.p);
@@ -1825,6 +1923,7 @@ int wfs_set_force_cells(wfs_solver* h, int64_t n, const int64_t* cells, const fl
                         const float* fz) {
     int rc = check_handle(h);
     if (rc) return rc;
+    if ((rc = check_not_batched(h, "wfs_set_force_cells"))) return rc;
     if (n < 0 || (n > 0 && (!cells || !fx || !fy || !fz))) return fail(WFS_EINVAL, "wfs_set_force_cells: bad arguments");
     for (int64_t k = 0; k < n; ++k)
         if (cells[k] < 0 || cells[k] >= h->N) return fail(WFS_EINVAL, "wfs_set_force_cells: cell index out of range");
@@ -1863,6 +1962,7 @@ int wfs_upload(wfs_solver* h, const uint8_t* solid, const float* cf, const float
 int wfs_set_force(wfs_solver* h, const float* Fx, const float* Fy, const float* Fz) {
     int rc = check_handle(h);
     if (rc) return rc;
+    if ((rc = check_not_batched(h, "wfs_set_force"))) return rc;
     if (!Fx || !Fy || !Fz) return fail(WFS_EINVAL, "wfs_set_force: null force array");
     DeviceGuard g(h->device);
     return h->set_force(Fx, Fy, Fz);
@@ -2352,6 +2452,13 @@ int wfs_forces(wfs_solver* h, double* out) {
     if (!out) return fail(WFS_EINVAL, "wfs_forces: null output");
     DeviceGuard g(h->device);
     return h->forces(out);
+}
+
+int wfs_set_closure_field(wfs_solver* h, int32_t on) {
+    int rc = check_handle(h, false);
+    if (rc) return rc;
+    DeviceGuard g(h->device);
+    return h->set_closure_field(on);
 }
 
 int wfs_get_closure_field(wfs_solver* h, float* c2) {

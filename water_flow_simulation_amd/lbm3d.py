@@ -423,6 +423,11 @@ class LBM3D:
         nat.check(nat.lib().wfs_get_closure_field(self.handle, nat.ptr(out)))
         return out
 
+    def keep_closure_field(self, on=True):
+        """Whether the learned step keeps the C^2 field closure_field reads
+        (on by default; off frees 4 bytes per stored cell)."""
+        nat.check(nat.lib().wfs_set_closure_field(self.handle, 1 if on else 0))
+
     # ------------------------------------------------------------ reference helpers
     def shape3(self, a):
         return np.asarray(a).reshape(self.nz, self.ny, self.nx)

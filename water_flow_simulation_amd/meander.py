@@ -230,7 +230,7 @@ def run(scale=1.0, steps=200, warmup=20, storage="f16dev", learned=False, log=pr
     if learned:
         from .closure import load_params
         closure = load_params()[0]
-    run_ = R.Run3D(reach, storage=storage, closure=closure, streaming=streaming)
+    run_ = R.Run3D(reach, storage=storage, closure=closure, streaming=streaming, closure_field=False)
     t_up = time.time() - t1
     L = run_.lbm
     stored = L.stored_cells

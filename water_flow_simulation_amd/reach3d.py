@@ -454,7 +454,7 @@ class Run3D:
     cadence = 20
 
     def __init__(self, reach: Reach, be=None, *, storage="f32", block_x=0, device=0, closure=None,
-                 column_u="cadence", streaming="ab", pop_stagger=0):
+                 column_u="cadence", streaming="ab", pop_stagger=0, closure_field=True):
         """closure: None for the constant-Cs Smagorinsky model, or the weights
         of the learned closure (closure.py), evaluated inside the step.
         block_x: work-group width, 0 for the default, "auto" for the tuned one.
@@ -465,7 +465,9 @@ class Run3D:
         reference does; "inplace" keeps one (half the population memory, the
         same results bit for bit).
         pop_stagger: bytes between the population direction arrays (a layout
-        choice, results unchanged), "auto" for the tuned one."""
+        choice, results unchanged), "auto" for the tuned one.
+        closure_field: False keeps no per-cell C^2 field of the learned
+        closure (lbm.closure_field() then refuses; 4 bytes per cell less)."""
         if column_u not in ("cadence", "per_step"):
             raise ValueError("column_u: 'cadence' or 'per_step'")
         if streaming not in ("ab", "inplace"):
@@ -492,6 +494,8 @@ class Run3D:
         if closure is not None:
             from .closure import geometry
             geo, hcol = geometry(reach.solid)
+            if not closure_field:
+                self.lbm.keep_closure_field(False)
             self.lbm.set_closure("mlp", closure, geo, hcol, float(uin_lb[uin_lb > 0].mean()))
         if reach.bedwall is not None:
             self.lbm.set_bedwall(reach.bedwall)
