@@ -80,7 +80,7 @@ typedef uint32_t u4v __attribute__((ext_vector_type(4)));
 // between lanes from layer 1 to layer 2.
 // The per-lane-row vectors come first, within reach of the LDS pair-read
 // offsets (1020 bytes) from one base per lane; w2 stays 16-byte aligned.
-struct MlpPack {
+struct alignas(16) MlpPack {
     float b1[2 * 16];          // [h][r]: b1[row(r, h)], the layer-1 accumulator start
     float b2[2 * 16];          // [h][r]: b2[row(r, h)]
     float w3[2 * 16];          // [h][r]: w3[row(r, h)]
@@ -91,6 +91,8 @@ struct MlpPack {
 };
 static_assert(offsetof(MlpPack, w2) % 16 == 0, "w2 is read as 16-byte vectors");
 constexpr int kPackWords = (int)(sizeof(MlpPack) / 4);
+static_assert(kPackWords % 4 == 0, "the pack is copied as 16-byte vectors");
+constexpr int kPackVec = kPackWords / 4;   // 185 vectors of 16 bytes
 
 __host__ __device__ constexpr int mfma_row(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
 // Hidden unit in B slot j (0..7) of k-step s for lane half h.
@@ -216,12 +218,55 @@ __device__ __forceinline__ float mlp_wave(const float* x, const MlpPack* P) {
     return y;
 }
 
-// Copies the pack (global) into LDS; every thread of the block calls it.
+// Copies the pack (global) into LDS; every thread of the block (at least 64)
+// calls it. All loads are issued before the first LDS write.
 __device__ __forceinline__ void stage_pack(const MlpPack* __restrict__ g, MlpPack* s) {
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(g);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(s);
-    for (int k = int(threadIdx.x); k < kPackWords; k += int(blockDim.x)) dst[k] = src[k];
+    const u4v* src = reinterpret_cast<const u4v*>(g);
+    u4v* dst = reinterpret_cast<u4v*>(s);
+    const int t = int(threadIdx.x), nt = int(blockDim.x);
+    u4v v[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (t + i * nt < kPackVec) v[i] = src[t + i * nt];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (t + i * nt < kPackVec) dst[t + i * nt] = v[i];
     __syncthreads();
+}
+
+// The step stages the pack per wavefront, in two parts: pack_load issues the
+// global loads of the 64 lanes' share (3 x 16 bytes) into registers early,
+// before the population loads, and pack_store writes them into the
+// wavefront's own LDS slot once the populations are in flight. No
+// work-group barrier is involved (each wavefront reads only its own slot,
+// and one wavefront's LDS accesses complete in order), so no wavefront
+// waits for another, and the weights' load latency hides behind the
+// population loads. Staging once per work-group behind a barrier held every
+// wavefront's population loads until the weights had arrived: the learned
+// FP16 step took 0.552 ms on 23.5 M cells that way, 0.452 ms this way.
+struct PackRegs {
+    u4v v[3];
+};
+__device__ __forceinline__ void pack_load(const MlpPack* __restrict__ g, PackRegs& r) {
+    const int lane = int(threadIdx.x) & 63;
+    const u4v* src = reinterpret_cast<const u4v*>(g);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (lane + 64 * i < kPackVec) r.v[i] = src[lane + 64 * i];
+}
+// All 64 lanes call it together; returns the wavefront's slot.
+__device__ __forceinline__ const MlpPack* pack_store(const PackRegs& r, MlpPack* slots) {
+    const int lane = int(threadIdx.x) & 63;
+    MlpPack* s = slots + (int(threadIdx.x) >> 6);
+    u4v* dst = reinterpret_cast<u4v*>(s);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (lane + 64 * i < kPackVec) dst[lane + 64 * i] = r.v[i];
+    // the compiler must not move the lanes' reads of other lanes' words above the writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return s;
 }
 
 // ---------------------------------------------------------------- kernels

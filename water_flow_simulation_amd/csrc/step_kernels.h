@@ -1341,8 +1341,10 @@ __global__ void __launch_bounds__(64) k_step_aa_sparse(typename Codec::T* __rest
 template <class Codec, bool kSparse = false, int kStream = 0>
 __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __restrict__ fs,
                                                   typename Codec::T* __restrict__ fd, const StepArgs<float>& a,
-                                                  int row, int tile, const MlpPack* P, int t = 0) {
+                                                  int row, int tile, MlpPack* slots, int t = 0) {
     using R = float;
+    PackRegs pr;
+    pack_load(a.mlp, pr);   // the network weights, issued before the population loads
     const int lane = int(threadIdx.x) & 63;
     const int x = kSparse ? tile * 64 + lane : tile * int(blockDim.x) + int(threadIdx.x);
     const int xw = kSparse ? tile * 64 : tile * int(blockDim.x) + (int(threadIdx.x) & ~63);
@@ -1381,6 +1383,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     }
     // the network runs on wavefronts with a fluid cell (a uniform branch)
     float y = 0.f;
+    const MlpPack* P = pack_store(pr, slots);   // all lanes (reconverged)
     if (__any(active)) y = mlp_wave(feat, P);
     // Less state across the network (more resident waves): the momentum is
     // formed again from rho and u, and the forces are read again, after it;
@@ -1445,30 +1448,28 @@ constexpr int kLearnedWaves = (sizeof(typename Codec::T) == 4 && kStream == 0) ?
 template <class Codec, int kStream = 0>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kLearnedWaves<Codec, kStream>))) k_step_learned(const typename Codec::T* __restrict__ fs,
                                                       typename Codec::T* __restrict__ fd, StepArgs<float> a) {
-    __shared__ MlpPack P;
+    extern __shared__ MlpPack slots[];   // one network pack per wavefront (dynamic LDS, blockDim / 64 of them)
     int row, tile;
     if (!tile_of(blockIdx.x, a.tiles, a.rows, row, tile)) return;   // whole work-group
-    stage_pack(a.mlp, &P);
-    step_tile_learned<Codec, false, kStream>(fs, fd, a, row, tile, &P);
+    step_tile_learned<Codec, false, kStream>(fs, fd, a, row, tile, slots);
 }
 
 // Tile-sparse layout: G wavefronts per work-group, one stored tile each
 // (blk2tile[group * G + wave]; -1 idle); the block map keeps every tile of a
-// row on one XCD. The plain step runs one tile per work-group; the learned
-// step four, so that one LDS staging of the network serves four tiles.
+// row on one XCD. The plain step runs one tile per work-group, the learned
+// step four.
 constexpr int kTilesPerGroupLearned = 4;
 
 template <class Codec, int G, int kStream = 0>
 __global__ void __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(kLearnedWaves<Codec, kStream>))) k_step_learned_sparse(const typename Codec::T* __restrict__ fs,
                                                                typename Codec::T* __restrict__ fd,
                                                                StepArgs<float> a) {
-    __shared__ MlpPack P;
-    stage_pack(a.mlp, &P);
+    __shared__ MlpPack slots[G];   // one network pack per wavefront
     // the tile is uniform over the wavefront: scalar registers, scalar loads of its tables
     const int t = __builtin_amdgcn_readfirstlane(a.blk2tile[blockIdx.x * G + (threadIdx.x >> 6)]);
     if (t < 0) return;   // whole wavefront
     const int tx = a.tcoord[3 * t], y = a.tcoord[3 * t + 1], z = a.tcoord[3 * t + 2];
-    step_tile_learned<Codec, true, kStream>(fs, fd, a, y + a.ny * z, tx, &P, t);
+    step_tile_learned<Codec, true, kStream>(fs, fd, a, y + a.ny * z, tx, slots, t);
 }
 
 template <class Codec, int G>

@@ -789,6 +789,9 @@ struct Solver final : wfs_solver {
         return WFS_OK;
     }
 
+    // Dynamic LDS of the dense learned step: one network pack per wavefront.
+    unsigned learned_lds() const { return (unsigned)((bx / 64) * sizeof(MlpPack)); }
+
     // One in-place step: odd (from state S) or even, by aa_phase.
     template <bool kOdd>
     void launch_step_aa(const dim3& grid, const StepArgs<R>& a) {
@@ -803,8 +806,8 @@ struct Solver final : wfs_solver {
                                        dim3((unsigned)(h_blk2tile4.size() / G)), dim3(64 * G), 0, stream, nullptr,
                                        fp(cur), b);
                 } else {
-                    hipLaunchKernelGGL((k_step_learned<Codec, kStream>), grid, dim3(bx), 0, stream, nullptr, fp(cur),
-                                       a);
+                    hipLaunchKernelGGL((k_step_learned<Codec, kStream>), grid, dim3(bx), learned_lds(), stream,
+                                       nullptr, fp(cur), a);
                 }
                 return;
             }
@@ -844,7 +847,8 @@ struct Solver final : wfs_solver {
         }
         if constexpr (std::is_same<R, float>::value) {
             if (closure == WFS_CLOSURE_MLP) {
-                hipLaunchKernelGGL(k_step_learned<Codec>, grid, dim3(bx), 0, stream, fp(cur), fp(cur ^ 1), a);
+                hipLaunchKernelGGL(k_step_learned<Codec>, grid, dim3(bx), learned_lds(), stream, fp(cur), fp(cur ^ 1),
+                                   a);
                 return;
             }
         }
