@@ -1454,19 +1454,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kLearn
     step_tile_learned<Codec, false, kStream>(fs, fd, a, row, tile, slots);
 }
 
-// Tile-sparse layout: G wavefronts per work-group, one stored tile each
-// (blk2tile[group * G + wave]; -1 idle); the block map keeps every tile of a
-// row on one XCD. The plain step runs one tile per work-group, the learned
-// step four.
-constexpr int kTilesPerGroupLearned = 4;
-
-template <class Codec, int G, int kStream = 0>
-__global__ void __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(kLearnedWaves<Codec, kStream>))) k_step_learned_sparse(const typename Codec::T* __restrict__ fs,
+// Tile-sparse layout: one stored tile per work-group of one wavefront
+// (blk2tile[group]; -1 idle); the block map keeps every tile of a row on one
+// XCD. (Four tiles per work-group, which shared one staging of the network
+// while it was staged per work-group, ran the 1 km meander's in-place
+// learned step at 18.7 ms against 17.6 ms for one, with two at 17.9 ms.)
+template <class Codec, int kStream = 0>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kLearnedWaves<Codec, kStream>))) k_step_learned_sparse(const typename Codec::T* __restrict__ fs,
                                                                typename Codec::T* __restrict__ fd,
                                                                StepArgs<float> a) {
-    __shared__ MlpPack slots[G];   // one network pack per wavefront
+    __shared__ MlpPack slots[1];   // the network pack of the wavefront
     // the tile is uniform over the wavefront: scalar registers, scalar loads of its tables
-    const int t = __builtin_amdgcn_readfirstlane(a.blk2tile[blockIdx.x * G + (threadIdx.x >> 6)]);
+    const int t = __builtin_amdgcn_readfirstlane(a.blk2tile[blockIdx.x]);
     if (t < 0) return;   // whole wavefront
     const int tx = a.tcoord[3 * t], y = a.tcoord[3 * t + 1], z = a.tcoord[3 * t + 2];
     step_tile_learned<Codec, true, kStream>(fs, fd, a, y + a.ny * z, tx, slots, t);

@@ -265,7 +265,7 @@ struct wfs_solver {
     bool tiled = false;
     int ntx = 0;                                         // x-tiles per row (px / 64)
     long long T = 0;                                     // stored tiles
-    std::vector<int> h_tmap, h_tcoord, h_nbr, h_blk2tile, h_blk2tile4;   // block maps: 1 and 4 tiles per group
+    std::vector<int> h_tmap, h_tcoord, h_nbr, h_blk2tile;   // block map: one tile per work-group
     std::vector<int> h_rowtile;                          // tiles stored before row j (rows + 1 entries)
     virtual int set_force_cells(long long n, const int64_t* idx, const float* fx, const float* fy,
                                 const float* fz) = 0;
@@ -310,7 +310,7 @@ struct Solver final : wfs_solver {
     DevBuf part_xyz, part_inlet, part_count;          // particle tracer: positions, inlet rows, respawn counter
     long long part_n = 0, part_ninlet = 0, part_step = 0;
     uint64_t part_seed = 0;
-    DevBuf tmap_d, tcoord_d, nbr_d, blk2tile_d, blk2tile4_d;   // tile-sparse tables (when tiled)
+    DevBuf tmap_d, tcoord_d, nbr_d, blk2tile_d;   // tile-sparse tables (when tiled)
     DevBuf aux_d;                                     // StepAux for column-buffer parity 0 and 1
     StepAux<R> aux_h[2]{};                            // what aux_d holds
     bool aux_up = false;
@@ -363,8 +363,7 @@ struct Solver final : wfs_solver {
         if ((rc = alloc_buf(aux_d, 2 * sizeof(StepAux<R>)))) return rc;     // StepAux pair (sync_aux fills it)
         if (tiled) {
             for (auto pr : {std::make_pair(&tmap_d, &h_tmap), std::make_pair(&tcoord_d, &h_tcoord),
-                            std::make_pair(&nbr_d, &h_nbr), std::make_pair(&blk2tile_d, &h_blk2tile),
-                            std::make_pair(&blk2tile4_d, &h_blk2tile4)}) {
+                            std::make_pair(&nbr_d, &h_nbr), std::make_pair(&blk2tile_d, &h_blk2tile)}) {
                 if ((rc = alloc_buf(*pr.first, pr.second->size() * sizeof(int)))) return rc;
                 WFS_HIP(hipMemcpy(pr.first->p, pr.second->data(), pr.second->size() * sizeof(int),
                                   hipMemcpyHostToDevice));
@@ -799,12 +798,8 @@ struct Solver final : wfs_solver {
         if constexpr (std::is_same<R, float>::value) {
             if (closure == WFS_CLOSURE_MLP) {
                 if (tiled) {
-                    constexpr int G = kTilesPerGroupLearned;
-                    StepArgs<R> b = a;
-                    b.blk2tile = static_cast<const int*>(blk2tile4_d.p);
-                    hipLaunchKernelGGL((k_step_learned_sparse<Codec, G, kStream>),
-                                       dim3((unsigned)(h_blk2tile4.size() / G)), dim3(64 * G), 0, stream, nullptr,
-                                       fp(cur), b);
+                    hipLaunchKernelGGL((k_step_learned_sparse<Codec, kStream>), dim3((unsigned)h_blk2tile.size()),
+                                       dim3(64), 0, stream, nullptr, fp(cur), a);
                 } else {
                     hipLaunchKernelGGL((k_step_learned<Codec, kStream>), grid, dim3(bx), learned_lds(), stream,
                                        nullptr, fp(cur), a);
@@ -833,11 +828,8 @@ struct Solver final : wfs_solver {
         if (tiled) {
             if constexpr (std::is_same<R, float>::value) {
                 if (closure == WFS_CLOSURE_MLP) {
-                    constexpr int G = kTilesPerGroupLearned;
-                    StepArgs<R> b = a;
-                    b.blk2tile = static_cast<const int*>(blk2tile4_d.p);
-                    hipLaunchKernelGGL((k_step_learned_sparse<Codec, G>), dim3((unsigned)(h_blk2tile4.size() / G)),
-                                       dim3(64 * G), 0, stream, fp(cur), fp(cur ^ 1), b);
+                    hipLaunchKernelGGL((k_step_learned_sparse<Codec>), dim3((unsigned)h_blk2tile.size()), dim3(64), 0,
+                                       stream, fp(cur), fp(cur ^ 1), a);
                     return;
                 }
             }
@@ -1878,7 +1870,6 @@ static void build_tiles(wfs_solver* h, const uint8_t* solid) {
             for (size_t k = 0; k < slot[s8].size(); ++k) out[(8 * (k / G) + s8) * G + k % G] = slot[s8][k];
     };
     block_map(1, h->h_blk2tile);
-    block_map(kTilesPerGroupLearned, h->h_blk2tile4);
 }
 
 int wfs_create_tiled(const wfs_desc* desc, const uint8_t* solid, wfs_solver** out) {
