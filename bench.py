@@ -77,6 +77,12 @@ def closure_weights():
     return CL.load_params()[0] if os.path.exists(CL.WEIGHTS) else CL.init_params(0)
 
 
+# Untimed steps before each variant line at least: a compute-bound step (the
+# learned closure) runs about 12 % slow for its first 20-odd steps after the
+# GPU sat idle during the solver set-up; the HBM-bound plain step does not.
+VARIANT_WARMUP = 50
+
+
 def time_gpu(reach, storage, steps, warmup, block_x, learned=False, column_u=False, inplace=False, pop_stagger=0):
     """column_u: the wall model's column speed refreshed every step inside the
     step kernel instead of once per 20-step chunk; inplace: one population
@@ -190,7 +196,7 @@ def time_meander(variants, steps, scale=1.0):
         st, opts = name.split("+")[0], name.split("+")[1:]
         learned = "mlp" in opts
         log(f"meander {name}")
-        r = meander.run(scale, steps, warmup=20, storage=st, learned=learned, log=log,
+        r = meander.run(scale, steps, warmup=VARIANT_WARMUP, storage=st, learned=learned, log=log,
                         streaming="inplace" if "inplace" in opts else "ab")
         k_s = r["kernel_ms"] / 1e3
         b_alg = B_ALG[st] + (2 if learned else 0)
@@ -218,7 +224,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="trapezoid_tug")
     ap.add_argument("--storage", default="f32", choices=sorted(B_ALG))
-    ap.add_argument("--block-x", type=int, default=0)
+    ap.add_argument("--block-x", default="auto",
+                    help="work-group width of the step (64, 128, 192, 256; auto: tuned per variant on this GPU)")
     ap.add_argument("--pop-stagger", default="auto",
                     help="bytes between the population direction arrays of the reach's solvers (0: packed; "
                          "auto: tuned per storage format on this GPU before timing)")
@@ -260,7 +267,13 @@ def main():
             return int(a.pop_stagger)
         from water_flow_simulation_amd import autotune
         return autotune.pop_stagger_for(reach, st)
-    r = time_gpu(reach, a.storage, a.steps, a.warmup, a.block_x, pop_stagger=stagger(a.storage))
+
+    def width(st, learned=False):
+        if a.block_x != "auto":
+            return int(a.block_x)
+        from water_flow_simulation_amd import autotune
+        return autotune.block_x_for(reach, st, closure_weights() if learned else None)
+    r = time_gpu(reach, a.storage, a.steps, a.warmup, width(a.storage), pop_stagger=stagger(a.storage))
     log(f"timed: {r['wall']:.3f} s wall, step kernel {r['kernel_ms'] / max(r['launches'], 1):.4f} ms avg")
     mlups = a.steps * reach.cells / r["wall"] / 1e6
     k_avg_s = r["kernel_ms"] / max(r["launches"], 1) / 1e3
@@ -273,7 +286,7 @@ def main():
         "config": {"workload": f"{a.workload}: trapezoidal reach 40 m bed, 1:2 banks, 3 m deep, 150 m at dx 0.1 m, "
                                f"{reach.nx}x{reach.ny}x{reach.nz} cells, one pusher tug (2 x 1.2 m props, 300 kW)",
                    "storage": a.storage, "cells": reach.cells, "fluid_cells": n_fluid, "cadence": 20,
-                   "block_x": a.block_x or 128, "pop_stagger": stagger(a.storage)},
+                   "block_x": width(a.storage), "pop_stagger": stagger(a.storage)},
         "mlups_fluid": round(a.steps * n_fluid / r["wall"] / 1e6, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_per_launch(),
@@ -289,7 +302,8 @@ def main():
         st, opts = name.split("+")[0], name.split("+")[1:]
         learned, colu, inplace = "mlp" in opts, "colu" in opts, "inplace" in opts
         log(f"timing variant {name}")
-        rv = time_gpu(reach, st, a.steps, a.warmup, a.block_x, learned, colu, inplace, stagger(st))
+        rv = time_gpu(reach, st, a.steps, max(a.warmup, VARIANT_WARMUP), width(st, learned), learned, colu, inplace,
+                      stagger(st))
         kv = rv["kernel_ms"] / max(rv["launches"], 1) / 1e3
         # + the 2-byte geometry code of the closure; + ux, uy written for the per-step column speed
         b_alg = B_ALG[st] + (2 if learned else 0) + (8 if colu else 0)
@@ -297,9 +311,11 @@ def main():
                           "kernel_ms_avg": round(kv * 1e3, 5),
                           "roofline_frac": round(b_alg * n_fluid / kv / 1e9 / PEAK_HBM_GBS, 4),
                           "frac_of_stream_ceiling": of_ceiling(st, b_alg * n_fluid / kv / 1e9),
-                          "speedup_vs_f32": round(r["wall"] / rv["wall"], 3), "pop_stagger": stagger(st)}
+                          "speedup_vs_f32": round(r["wall"] / rv["wall"], 3), "pop_stagger": stagger(st),
+                          "block_x": width(st, learned)}
     if variants:
         out["variants"] = variants
+        out["variant_warmup"] = max(a.warmup, VARIANT_WARMUP)
     if a.batch_cases > 0:
         log(f"timing the batched sweep ({a.batch_cases} cases)")
         out["batched"] = time_batched(a.batch_cases, a.batch_steps, scale=a.batch_scale)

@@ -63,21 +63,27 @@ def _key(reach, storage, closure):
     return f"{storage}/{'mlp' if closure is not None else 'smag'}/{reach.nx}x{reach.ny}x{reach.nz}"
 
 
-def time_widths(reach, storage="f32", closure=None, steps=40, warmup=6, widths=WIDTHS):
-    """Average step-kernel milliseconds at each width, on a scratch solver of the case."""
+def time_widths(reach, storage="f32", closure=None, steps=40, warmup=6, widths=WIDTHS, rounds=2, settle=40):
+    """Average step-kernel milliseconds at each width, on a scratch solver of
+    the case: ``settle`` steps first (a compute-bound step such as the
+    learned one runs about 12 % slow for its first 20-odd steps after the GPU
+    sat idle during the set-up), then the widths in turn ``rounds`` times,
+    the best round counting."""
     from .reach3d import Run3D
     run = Run3D(reach, storage=storage, closure=closure)
     lbm = run.lbm
-    out = {}
+    out = {bx: float("inf") for bx in widths}
     try:
-        for bx in widths:
-            lbm.block_x = bx
-            lbm.step(warmup)
-            lbm.timing(True)
-            lbm.step(steps)
-            ms, n = lbm.timing_read()
-            lbm.timing(False)
-            out[bx] = ms / max(n, 1)
+        lbm.step(settle)
+        for _ in range(rounds):
+            for bx in widths:
+                lbm.block_x = bx
+                lbm.step(warmup)
+                lbm.timing(True)
+                lbm.step(steps)
+                ms, n = lbm.timing_read()
+                lbm.timing(False)
+                out[bx] = min(out[bx], ms / max(n, 1))
         lbm.check_finite()
     finally:
         run.close()
@@ -103,7 +109,8 @@ STAGGERS = (0, 512, 1024, 1536, 2048, 2560, 3072, 3584)
 def time_staggers(reach, storage="f32", steps=60, warmup=10, staggers=STAGGERS, rounds=2):
     """Average step-kernel milliseconds at each population stagger (bytes),
     each on a scratch solver of the case, the candidates taken in turn
-    ``rounds`` times (the best round counts)."""
+    ``rounds`` times (the best round counts; the plain step it times is
+    bound by HBM, which needs no settling)."""
     from .reach3d import Run3D
     out = {s: float("inf") for s in staggers}
     for _ in range(rounds):
