@@ -56,6 +56,24 @@ def test_stagger_leaves_results_unchanged(storage, layout, streaming, stagger):
         run.close()
 
 
+@pytest.mark.parametrize("layout,streaming", [("dense", "inplace"), ("tiles", "ab")])
+def test_stagger_with_the_learned_closure(layout, streaming):
+    print(f"case: 128x32x16 channel, f16dev, learned closure, {layout}, {streaming}, stagger 0 vs 1536 bytes",
+          flush=True)
+    from water_flow_simulation_amd import closure as CL
+    reach = replace(channel128(), layout=layout)
+    w = CL.init_params(4)
+    runs = [R.Run3D(reach, storage="f16dev", streaming=streaming, closure=w, pop_stagger=s) for s in (0, 1536)]
+    for run in runs:
+        run.advance(40)
+    fluid = ~reach.solid.ravel()
+    for u, v in zip(runs[0].lbm.macro(np.float64), runs[1].lbm.macro(np.float64)):
+        assert np.array_equal(u[fluid], v[fluid])
+    assert np.array_equal(runs[0].lbm.closure_field()[fluid], runs[1].lbm.closure_field()[fluid])
+    for run in runs:
+        run.close()
+
+
 def test_state_loads_across_staggers(tmp_path):
     print("case: 128x32x16 channel, f16dev, state saved packed, loaded at a 1 KiB stagger", flush=True)
     reach = channel128()
