@@ -170,9 +170,13 @@ def summary():
 
 if __name__ == "__main__":
     from . import workloads
+    from .closure import WEIGHTS, init_params, load_params
     r = workloads.trapezoid_tug()
-    for st in ("f32", "f16dev"):
-        t = time_widths(r, st)
-        print(st, "block_x", {k: round(v, 4) for k, v in t.items()}, "best", min(t, key=t.get), flush=True)
+    w = load_params()[0] if os.path.exists(WEIGHTS) else init_params(0)
+    for st, cl in (("f32", None), ("f16dev", None), ("bf16dev", None), ("f32", w), ("f16dev", w)):
+        t = time_widths(r, st, cl)
+        print(st, "learned" if cl is not None else "smagorinsky", "block_x ms",
+              {k: round(v, 4) for k, v in t.items()}, "best", min(t, key=t.get), flush=True)
+    for st in ("f32", "f16dev", "bf16dev"):
         t = time_staggers(r, st)
-        print(st, "pop_stagger", {k: round(v, 4) for k, v in t.items()}, "best", min(t, key=t.get), flush=True)
+        print(st, "pop_stagger ms", {k: round(v, 4) for k, v in t.items()}, "best", min(t, key=t.get), flush=True)
