@@ -57,6 +57,28 @@ def test_batch_matches_sequential_bitwise(use_graph, group):
     b.close()
 
 
+def test_batch_of_staggered_cases_matches_sequential_bitwise():
+    """A case's population stride comes with its step arguments, so a batch
+    of cases with their direction arrays 1.5 KiB apart steps exactly as the
+    packed cases run one after another."""
+    print("case: batch of 3 reaches, population stagger 1536 bytes vs packed sequential runs", flush=True)
+    reaches = _cases()[:3]
+    seq = []
+    for r in reaches:
+        run = R.Run3D(r)
+        run.advance(60)
+        seq.append(run)
+    b = BatchRun(reaches, pop_stagger=1536)
+    assert all(run.lbm.pop_stagger == 1536 for run in b.runs)
+    b.advance(60)
+    for k, (rs, rb) in enumerate(zip(seq, b.runs)):
+        fl = np.tile(~reaches[k].solid.ravel(), 19)
+        assert np.array_equal(rs.lbm.populations()[fl], rb.lbm.populations()[fl]), k
+    for run in seq:
+        run.close()
+    b.close()
+
+
 def test_block_width_switch_mid_run_and_autotuner(tmp_path, monkeypatch):
     """Changing the work-group width between steps leaves the result bit for
     bit unchanged, and the autotuner times every width and caches its pick."""

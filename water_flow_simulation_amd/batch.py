@@ -24,15 +24,20 @@ from .reach3d import Run3D
 
 
 class BatchRun:
-    def __init__(self, reaches, storage="f32", block_x=0, device=0, be=None, group=0):
+    def __init__(self, reaches, storage="f32", block_x=0, device=0, be=None, group=0, pop_stagger=0):
         """block_x: work-group width (0: library default, "auto": autotuned on
         the largest case); group: cases per batched launch (0: all, "auto":
-        autotuned)."""
-        bx = block_x
+        autotuned); pop_stagger: bytes between each case's population
+        direction arrays (a layout choice, results unchanged; "auto": tuned on
+        the largest case)."""
+        bx, ps = block_x, pop_stagger
         if block_x == "auto":
             from .autotune import block_x_for
             bx = block_x_for(max(reaches, key=lambda r: r.cells), storage)
-        self.runs = [Run3D(r, be, storage=storage, block_x=bx, device=device) for r in reaches]
+        if pop_stagger == "auto":
+            from .autotune import pop_stagger_for
+            ps = pop_stagger_for(max(reaches, key=lambda r: r.cells), storage)
+        self.runs = [Run3D(r, be, storage=storage, block_x=bx, device=device, pop_stagger=ps) for r in reaches]
         self._hs = []
         self.group = 0
         self.steps_wall = 0.0
