@@ -236,8 +236,10 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=32, help="cases of the batched-sweep line (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=400)
     ap.add_argument("--batch-scale", type=float, default=1.0, help="length and width factor of the ensemble")
-    ap.add_argument("--meander", default="f16dev,f16dev+mlp",
-                    help="variants timed on the meandering reach, tile-sparse layout ('' to skip)")
+    ap.add_argument("--meander", default="f16dev+inplace,f16dev+mlp+inplace",
+                    help="variants timed on the meandering reach, tile-sparse layout ('' to skip); the default is "
+                         "the configuration as specified: FP16 streamed in place, without and with the learned "
+                         "closure")
     ap.add_argument("--meander-steps", type=int, default=200)
     ap.add_argument("--meander-scale", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
