@@ -356,6 +356,7 @@ struct StepArgs {
     float cs2;                          // the constant model's Cs^2 (float): the C^2 floor of the outlet sponge
     // tile-sparse layout (k_step_sparse only): a tile is one 64-cell x-run of a
     // row; cell c = tile * 64 + lane
+    const int* __restrict__ wgrec;      // per work-group of the sparse step kernels: kWgRec ints (below)
     const int* __restrict__ nbr;        // per tile: 27 neighbour tiles (dx+1) + 3 ((dy+1) + 3 (dz+1)), -1 none
     const int* __restrict__ tcoord;     // per tile: x-tile, y, z
     const int* __restrict__ blk2tile;   // work-group -> tile (-1: idle), rows kept on one XCD
@@ -991,7 +992,8 @@ __device__ __forceinline__ bool boundary_wave(const StepArgs<R>& a, int xw) {
 // (64 cells) and t its index in the compact storage.
 template <class Codec, bool kSparse = false>
 __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ fs, typename Codec::T* __restrict__ fd,
-                                          const StepArgs<typename Codec::R>& a, int row, int tile, int t = 0) {
+                                          const StepArgs<typename Codec::R>& a, int row, int tile, int t = 0,
+                                          const int* nbp = nullptr) {
     using R = typename Codec::R;
     const int lane = int(threadIdx.x) & 63;
     const int x = kSparse ? tile * 64 + lane : tile * int(blockDim.x) + int(threadIdx.x);
@@ -1000,7 +1002,7 @@ __device__ __forceinline__ void step_tile(const typename Codec::T* __restrict__ 
     const bool inrow = x < a.nx;
     const int z = row / a.ny;
     const int c = kSparse ? t * 64 + lane : x + a.px * row;
-    const int* nb = kSparse ? a.nbr + 27 * t : nullptr;
+    const int* nb = kSparse ? (nbp ? nbp : a.nbr + 27 * t) : nullptr;
     // Lanes outside the row or on solid cells do no work but stay in the wave
     // for the boundary-row shuffles.
     const uint8_t fl = inrow ? at(a.flag, c) : uint8_t(FLAG_SOLID);
@@ -1341,7 +1343,8 @@ __global__ void __launch_bounds__(64) k_step_aa_sparse(typename Codec::T* __rest
 template <class Codec, bool kSparse = false, int kStream = 0>
 __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __restrict__ fs,
                                                   typename Codec::T* __restrict__ fd, const StepArgs<float>& a,
-                                                  int row, int tile, MlpPack* slots, int t = 0) {
+                                                  int row, int tile, MlpPack* slots, int t = 0,
+                                                  const int* nbp = nullptr) {
     using R = float;
     PackRegs pr;
     pack_load(a.mlp, pr);   // the network weights, issued before the population loads
@@ -1351,7 +1354,7 @@ __device__ __forceinline__ void step_tile_learned(const typename Codec::T* __res
     const bool inrow = x < a.nx;
     const int z = row / a.ny;
     const int c = kSparse ? t * 64 + lane : x + a.px * row;
-    const int* nb = kSparse ? a.nbr + 27 * t : nullptr;
+    const int* nb = kSparse ? (nbp ? nbp : a.nbr + 27 * t) : nullptr;
     const uint8_t fl = inrow ? at(a.flag, c) : uint8_t(FLAG_SOLID);
     const bool active = !(fl & FLAG_SOLID);
     const bool forced = active && (fl & FLAG_FORCED) != 0;
@@ -1454,31 +1457,42 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kLearn
     step_tile_learned<Codec, false, kStream>(fs, fd, a, row, tile, slots);
 }
 
-// Tile-sparse layout: one stored tile per work-group of one wavefront
-// (blk2tile[group]; -1 idle); the block map keeps every tile of a row on one
-// XCD. (Four tiles per work-group, which shared one staging of the network
-// while it was staged per work-group, ran the 1 km meander's in-place
-// learned step at 18.7 ms against 17.6 ms for one, with two at 17.9 ms.)
+// Per-work-group record of the tile-sparse step kernels, in block order: the
+// tile (-1: idle work-group), its (x-tile, y, z) and its 27 neighbour tiles,
+// padded to 128 bytes. A work-group finds all it needs to start its loads in
+// one round of scalar loads from blockIdx, where blk2tile -> (tcoord, nbr)
+// took two dependent rounds: the two-buffer FP16 step on the 1 km meander
+// 13.15 -> 12.43 ms, its in-place learned step 17.16 -> 17.01 ms. (The
+// in-place constant-Cs step ran 4 % slower with it and keeps the per-tile
+// tables.)
+constexpr int kWgRec = 32;
+
+// Tile-sparse layout: one stored tile per work-group of one wavefront; the
+// block map keeps every tile of a row on one XCD. (Four tiles per
+// work-group, which shared one staging of the network while it was staged
+// per work-group, ran the 1 km meander's in-place learned step at 18.7 ms
+// against 17.6 ms for one, with two at 17.9 ms.)
 template <class Codec, int kStream = 0>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kLearnedWaves<Codec, kStream>))) k_step_learned_sparse(const typename Codec::T* __restrict__ fs,
                                                                typename Codec::T* __restrict__ fd,
                                                                StepArgs<float> a) {
     __shared__ MlpPack slots[1];   // the network pack of the wavefront
-    // the tile is uniform over the wavefront: scalar registers, scalar loads of its tables
-    const int t = __builtin_amdgcn_readfirstlane(a.blk2tile[blockIdx.x]);
+    // the tile is uniform over the wavefront: scalar registers, scalar loads of its record
+    const int* rec = a.wgrec + kWgRec * (long long)blockIdx.x;
+    const int t = __builtin_amdgcn_readfirstlane(rec[0]);
     if (t < 0) return;   // whole wavefront
-    const int tx = a.tcoord[3 * t], y = a.tcoord[3 * t + 1], z = a.tcoord[3 * t + 2];
-    step_tile_learned<Codec, true, kStream>(fs, fd, a, y + a.ny * z, tx, slots, t);
+    step_tile_learned<Codec, true, kStream>(fs, fd, a, rec[2] + a.ny * rec[3], rec[1], slots, t, rec + 4);
 }
 
 template <class Codec, int G>
 __global__ void __launch_bounds__(64 * G) k_step_sparse(const typename Codec::T* __restrict__ fs,
                                                        typename Codec::T* __restrict__ fd,
                                                        StepArgs<typename Codec::R> a) {
-    const int t = __builtin_amdgcn_readfirstlane(a.blk2tile[blockIdx.x * G + (threadIdx.x >> 6)]);
+    static_assert(G == 1, "one tile per work-group (the records are per work-group)");
+    const int* rec = a.wgrec + kWgRec * (long long)blockIdx.x;
+    const int t = __builtin_amdgcn_readfirstlane(rec[0]);
     if (t < 0) return;   // whole wavefront
-    const int tx = a.tcoord[3 * t], y = a.tcoord[3 * t + 1], z = a.tcoord[3 * t + 2];
-    step_tile<Codec, true>(fs, fd, a, y + a.ny * z, tx, t);
+    step_tile<Codec, true>(fs, fd, a, rec[2] + a.ny * rec[3], rec[1], t, rec + 4);
 }
 
 // ---------------------------------------------------------------- many reaches in one launch

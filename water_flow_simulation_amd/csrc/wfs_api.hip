@@ -265,7 +265,7 @@ struct wfs_solver {
     bool tiled = false;
     int ntx = 0;                                         // x-tiles per row (px / 64)
     long long T = 0;                                     // stored tiles
-    std::vector<int> h_tmap, h_tcoord, h_nbr, h_blk2tile;   // block map: one tile per work-group
+    std::vector<int> h_tmap, h_tcoord, h_nbr, h_blk2tile, h_wgrec;   // block map: one tile per work-group
     std::vector<int> h_rowtile;                          // tiles stored before row j (rows + 1 entries)
     virtual int set_force_cells(long long n, const int64_t* idx, const float* fx, const float* fy,
                                 const float* fz) = 0;
@@ -310,7 +310,7 @@ struct Solver final : wfs_solver {
     DevBuf part_xyz, part_inlet, part_count;          // particle tracer: positions, inlet rows, respawn counter
     long long part_n = 0, part_ninlet = 0, part_step = 0;
     uint64_t part_seed = 0;
-    DevBuf tmap_d, tcoord_d, nbr_d, blk2tile_d;   // tile-sparse tables (when tiled)
+    DevBuf tmap_d, tcoord_d, nbr_d, blk2tile_d, wgrec_d;   // tile-sparse tables (when tiled)
     DevBuf aux_d;                                     // StepAux for column-buffer parity 0 and 1
     StepAux<R> aux_h[2]{};                            // what aux_d holds
     bool aux_up = false;
@@ -363,7 +363,8 @@ struct Solver final : wfs_solver {
         if ((rc = alloc_buf(aux_d, 2 * sizeof(StepAux<R>)))) return rc;     // StepAux pair (sync_aux fills it)
         if (tiled) {
             for (auto pr : {std::make_pair(&tmap_d, &h_tmap), std::make_pair(&tcoord_d, &h_tcoord),
-                            std::make_pair(&nbr_d, &h_nbr), std::make_pair(&blk2tile_d, &h_blk2tile)}) {
+                            std::make_pair(&nbr_d, &h_nbr), std::make_pair(&blk2tile_d, &h_blk2tile),
+                            std::make_pair(&wgrec_d, &h_wgrec)}) {
                 if ((rc = alloc_buf(*pr.first, pr.second->size() * sizeof(int)))) return rc;
                 WFS_HIP(hipMemcpy(pr.first->p, pr.second->data(), pr.second->size() * sizeof(int),
                                   hipMemcpyHostToDevice));
@@ -729,6 +730,7 @@ struct Solver final : wfs_solver {
         a.nbr = static_cast<const int*>(nbr_d.p);
         a.tcoord = static_cast<const int*>(tcoord_d.p);
         a.blk2tile = static_cast<const int*>(blk2tile_d.p);
+        a.wgrec = static_cast<const int*>(wgrec_d.p);
         a.tmap = static_cast<const int*>(tmap_d.p);
         a.ntx = ntx;
         set_colu(a);
@@ -1870,6 +1872,16 @@ static void build_tiles(wfs_solver* h, const uint8_t* solid) {
             for (size_t k = 0; k < slot[s8].size(); ++k) out[(8 * (k / G) + s8) * G + k % G] = slot[s8][k];
     };
     block_map(1, h->h_blk2tile);
+    // the same map as per-work-group records (kWgRec ints: tile, x-tile, y, z, 27 neighbour tiles)
+    h->h_wgrec.assign(h->h_blk2tile.size() * kWgRec, -1);
+    for (size_t L = 0; L < h->h_blk2tile.size(); ++L) {
+        const int t = h->h_blk2tile[L];
+        if (t < 0) continue;
+        int* r = &h->h_wgrec[(size_t)kWgRec * L];
+        r[0] = t;
+        for (int k = 0; k < 3; ++k) r[1 + k] = h->h_tcoord[3 * (size_t)t + k];
+        for (int k = 0; k < 27; ++k) r[4 + k] = h->h_nbr[27 * (size_t)t + k];
+    }
 }
 
 int wfs_create_tiled(const wfs_desc* desc, const uint8_t* solid, wfs_solver** out) {
