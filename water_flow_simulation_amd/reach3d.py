@@ -695,81 +695,118 @@ class Run3D:
 
 
 # ================================================================ readouts
+class CrossSections:
+    """Reductions of (nz, ny, nx) time means to one value per cross-section x
+    of the wet section (~solid), with cell faces dx by dx."""
+
+    def __init__(self, solid: np.ndarray, dx: float):
+        self.wet = ~solid
+        self.dx = dx
+        self.lid = self.wet[-1]                         # (ny, nx): fluid cells right under the lid
+        self.n_lid = np.maximum(self.lid.sum(0), 1)
+        dry_below = np.ones_like(self.wet)
+        dry_below[1:] = ~self.wet[:-1]
+        self.bed = self.wet & dry_below                  # first fluid cell above solid in each column
+
+    def under_lid(self, f: np.ndarray) -> np.ndarray:
+        """Mean of f over the top fluid layer."""
+        return (f[-1] * self.lid).sum(0) / self.n_lid
+
+    def integral(self, f: np.ndarray) -> np.ndarray:
+        """Integral of f over the wet section."""
+        return (f * self.wet).sum(axis=(0, 1)) * self.dx * self.dx
+
+    def peak(self, f: np.ndarray) -> np.ndarray:
+        return np.where(self.wet, f, 0).max(axis=(0, 1))
+
+    def at_bed(self, f: np.ndarray) -> np.ndarray:
+        """(ny, nx) map of f at the first fluid cell above the bed (the
+        largest over the column where it meets solid more than once)."""
+        return np.where(self.bed, f, 0.0).max(axis=0)
+
+
 def profiles(run: Run3D) -> dict:
     """Along-river profiles of the time-mean fields (floodsim/reach3d.py:545-568):
     level under the lid relative to the outlet, surface velocity, peak speed,
-    discharge, momentum + pressure flux, and bed shear from the first fluid
-    cell above the bed (log law with the Manning roughness)."""
+    discharge, momentum + pressure flux (rho u^2 + rho g eta) and bed shear
+    rho cf |u|^2 with the log-law cf of the Manning roughness at dx/2."""
     r = run.r
-    ux = run.field("ux", mean=True)
-    sp = run.field("sp", mean=True)
-    eta = run.field("rho", mean=True)
-    uxx = run.field("uxx", mean=True)
-    fl = ~r.solid
-    top = fl[-1]
-    n_top = np.maximum(top.sum(0), 1)
-    eta_x = (eta[-1] * top).sum(0) / n_top
-    us_x = (ux[-1] * top).sum(0) / n_top
-    umax_x = np.where(fl, sp, 0).max(axis=(0, 1))
-    q_x = (ux * fl).sum(axis=(0, 1)) * r.dx * r.dx
-    mom_x = ((RHO * uxx + RHO * G * eta) * fl).sum(axis=(0, 1)) * r.dx * r.dx
-    below = np.concatenate([np.ones((1,) + fl.shape[1:], bool), ~fl[:-1]], 0)
-    first = fl & below
-    cf = log_cf(0.5 * r.dx, manning_ks(r.sec.n))
-    tau = np.where(first, RHO * cf * sp ** 2, 0.0)
-    tau_bed = tau.max(axis=0)
-    return dict(x=r.x, eta=eta_x - eta_x[-1], us=us_x, umax=umax_x, q=q_x, tau_bed=tau_bed,
-                tau_max_x=tau_bed.max(0), mom=mom_x)
+    mean = {k: run.field(k, mean=True) for k in ("ux", "sp", "rho", "uxx")}
+    xs = CrossSections(r.solid, r.dx)
+    level = xs.under_lid(mean["rho"])                 # field("rho") is the level eta
+    cf_bed = log_cf(0.5 * r.dx, manning_ks(r.sec.n))
+    tau_bed = xs.at_bed(RHO * cf_bed * mean["sp"] ** 2)
+    return dict(x=r.x, eta=level - level[-1], us=xs.under_lid(mean["ux"]), umax=xs.peak(mean["sp"]),
+                q=xs.integral(mean["ux"]), tau_bed=tau_bed, tau_max_x=tau_bed.max(0),
+                mom=xs.integral(RHO * mean["uxx"] + RHO * G * mean["rho"]))
+
+
+def _upstream_gauge(x: np.ndarray, hulls) -> int:
+    """Index of the upstream level gauge: 8 % down the reach, but at least
+    10 m ahead of the first bow, and never the inlet column."""
+    i = int(0.08 * len(x))
+    if hulls:
+        i = min(i, int(np.searchsorted(x, min(h[0] for h in hulls) - 10.0)))
+    return max(i, 1)
+
+
+def _jet_reach(x: np.ndarray, excess: np.ndarray, x_from: float, threshold: float) -> tuple[float, bool]:
+    """How far past x_from the peak speed still exceeds the base's by more
+    than threshold, and whether that reaches the last three columns."""
+    hit = np.flatnonzero((x > x_from) & (excess > threshold))
+    if not len(hit):
+        return 0.0, False
+    return float(x[hit[-1]] - x_from), bool(hit[-1] >= len(x) - 3)
+
+
+def _shear_window(x: np.ndarray, hulls, dx: float) -> np.ndarray:
+    """Columns where bed shear is judged: from 5 m ahead of the first bow
+    (of the point 15 % down the reach without hulls), but not within the
+    first 10 % of the reach, to three cells before the outlet."""
+    first_bow = min([h[0] for h in hulls], default=0.15 * x[-1])
+    return (x >= max(first_bow - 5.0, 0.1 * x[-1])) & (x <= x[-1] - 3 * dx)
 
 
 def compare(res: dict, base: dict, reach: Reach, cfg: ReachConfig) -> dict:
     """A boat case against the no-boat base at the same level
-    (floodsim/reach3d.py:571-618)."""
+    (floodsim/reach3d.py:571-618): level change at the upstream gauge,
+    jet reach, surface-speed gain downstream of the last propeller, bed area
+    above the critical shear, the net force from the change of momentum +
+    pressure flux, and the discharge gain if such a group stood every
+    cfg.spacing_km (momentum balance against the bed slope)."""
     x = res["x"]
-    fl = reach.fleet
-    T = fl.total_thrust()
-    P = fl.total_power()
-    xs = [p["x"] for p in reach.props]
-    x_last = max(xs) if xs else 0.0
-    # upstream gauge: 8 % down the reach, but always at least 10 m ahead of the first bow
-    i_up = max(int(0.08 * len(x)), 1)
-    if reach.hulls:
-        i_up = max(min(i_up, int(np.searchsorted(x, min(hb[0] for hb in reach.hulls) - 10.0))), 1)
-    d_eta = (res["eta"][i_up] - res["eta"][-1]) - (base["eta"][i_up] - base["eta"][-1])
-    excess = res["umax"] - base["umax"]
-    thr = max(0.2, 0.15 * reach.V)
-    down = x > x_last
-    hit = np.flatnonzero(down & (excess > thr))
-    reach_m = float(x[hit[-1]] - x_last) if len(hit) else 0.0
-    reach_capped = bool(len(hit) and hit[-1] >= len(x) - 3)
-    m = down & (x < x[-1] - 2 * reach.dx)
-    us_gain = float((res["us"][m] - base["us"][m]).mean()) if m.any() else 0.0
-    a_cell = reach.dx * reach.dx
-    # bed shear from just ahead of the boats to just before the outlet
-    x0 = min([hb[0] for hb in reach.hulls], default=0.15 * x[-1]) - 5.0
-    zone = (x >= max(x0, 0.1 * x[-1])) & (x <= x[-1] - 3 * reach.dx)
-    tb, tbb = res["tau_bed"][:, zone], base["tau_bed"][:, zone]
-    over = float((tb > cfg.tau_crit).sum() * a_cell)
-    over_b = float((tbb > cfg.tau_crit).sum() * a_cell)
-    # net force given to the water: change of momentum+pressure flux across the reach vs the base
-    i_out = len(x) - 4
-    dm = res["mom"] - base["mom"]
-    F_net = float(dm[i_out] - dm[i_up])
-    # discharge gain if such a group stood every spacing_km (momentum balance with Manning friction)
-    gravity = RHO * G * reach.A * (cfg.spacing_km * 1000) * max(reach.S, 1e-7)
+    fleet = reach.fleet
+    thrust, power = fleet.total_thrust(), fleet.total_power()
+    x_last = max((p["x"] for p in reach.props), default=0.0)
+    i_up, i_out = _upstream_gauge(x, reach.hulls), len(x) - 4
 
-    def dq(F):
-        return math.sqrt(max(1 + F / gravity, 0.0)) - 1
+    def rise(p):   # level at the gauge above the outlet
+        return p["eta"][i_up] - p["eta"][-1]
 
-    dQ = dq(T)
-    dQn = dq(F_net)
-    return dict(label=fl.label(), n=fl.n_boats, T_kN=T / 1000, P_kW=P, h=reach.h, Q=reach.Q, V=reach.V,
-                d_eta_cm=100 * d_eta, jet_reach=reach_m, reach_capped=reach_capped,
+    d_eta = rise(res) - rise(base)
+    jet_m, jet_capped = _jet_reach(x, res["umax"] - base["umax"], x_last, max(0.2, 0.15 * reach.V))
+    downstream = (x > x_last) & (x < x[-1] - 2 * reach.dx)
+    us_gain = float((res["us"] - base["us"])[downstream].mean()) if downstream.any() else 0.0
+    win = _shear_window(x, reach.hulls, reach.dx)
+    tau, tau_b = res["tau_bed"][:, win], base["tau_bed"][:, win]
+    cell = reach.dx * reach.dx
+    d_mom = res["mom"] - base["mom"]
+    f_net = float(d_mom[i_out] - d_mom[i_up])
+    weight_along = RHO * G * reach.A * (cfg.spacing_km * 1000) * max(reach.S, 1e-7)
+
+    def q_gain(force):
+        return math.sqrt(max(1 + force / weight_along, 0.0)) - 1
+
+    dq_t, dq_n = q_gain(thrust), q_gain(f_net)
+    per_100kw = power / 100
+    return dict(label=fleet.label(), n=fleet.n_boats, T_kN=thrust / 1000, P_kW=power, h=reach.h, Q=reach.Q,
+                V=reach.V, d_eta_cm=100 * d_eta, jet_reach=jet_m, reach_capped=jet_capped,
                 us_gain=us_gain, us_gain_pct=100 * us_gain / max(reach.V, 1e-6),
-                umax=float(res["umax"].max()), tau_max=float(tb.max()), tau_base=float(tbb.max()),
-                tau_area=over, tau_area_base=over_b,
-                dQ_pct=100 * dQ, dQ_net_pct=100 * dQn, F_net_kN=F_net / 1000,
-                dQ_per_100kW=100 * dQn / max(P / 100, 1e-9) if P > 0 else 0.0,
-                drop_cm_per_100kW=-100 * d_eta / (P / 100) if P > 0 else 0.0,
-                kW_per_cm=P / (-100 * d_eta) if (P > 0 and d_eta < -1e-5) else float("nan"),
-                T_per_kW=T / max(P, 1e-9) if P > 0 else 0.0)
+                umax=float(res["umax"].max()), tau_max=float(tau.max()), tau_base=float(tau_b.max()),
+                tau_area=float((tau > cfg.tau_crit).sum() * cell),
+                tau_area_base=float((tau_b > cfg.tau_crit).sum() * cell),
+                dQ_pct=100 * dq_t, dQ_net_pct=100 * dq_n, F_net_kN=f_net / 1000,
+                dQ_per_100kW=100 * dq_n / max(per_100kw, 1e-9) if power > 0 else 0.0,
+                drop_cm_per_100kW=-100 * d_eta / per_100kw if power > 0 else 0.0,
+                kW_per_cm=power / (-100 * d_eta) if (power > 0 and d_eta < -1e-5) else float("nan"),
+                T_per_kW=thrust / max(power, 1e-9) if power > 0 else 0.0)
