@@ -176,12 +176,18 @@ def time_batched(cases, steps, seed=0, scale=1.0):
         run.close()
     b, bat_all_s, bat_s = sweep.run_batched(reaches, steps)
     b.close()
+    # the same batch with every case streamed in place (one population buffer)
+    b, _, aa_s = sweep.run_batched(reaches, steps, streaming="inplace")
+    b.close()
+    pop_gb = cells * 19 * 4 / 1e9
     # speedup: the stepping alone on both sides; speedup_with_setup: the
     # whole sweep, the solvers' set-up included on both sides
     return {"cases": cases, "seed": seed, "cells_total": cells, "steps": steps,
             "sequential_mlups": round(steps * cells / seq_s / 1e6, 1),
             "batched_mlups": round(steps * cells / bat_s / 1e6, 1), "speedup": round(seq_s / bat_s, 3),
             "speedup_with_setup": round(seq_all_s / bat_all_s, 3),
+            "inplace": {"batched_mlups": round(steps * cells / aa_s / 1e6, 1), "speedup": round(seq_s / aa_s, 3),
+                        "population_gb": round(pop_gb, 2), "population_gb_two_buffers": round(2 * pop_gb, 2)},
             "ensemble": f"seeded boat-test reaches at dx 0.2 m, lengths and widths x{scale}"}
 
 

@@ -302,7 +302,8 @@ int wfs_timing_read(wfs_solver* h, double* kernel_ms, int64_t* launches);
 
 /* ---- Batches: many reaches stepped together (the boat-test sweep).
  * A batch references solvers that were created and uploaded as usual (same
- * storage format, block_x and device). Each run step is one launch over every
+ * storage format, block_x, device and streaming mode: all two-buffer or all
+ * in place, wfs_set_inplace). Each run step is one launch over every
  * case; the run driver's 20-step cadence of floodsim/reach3d.py:424-453
  * (thrust ramp, uref refresh, time-mean sums once t >= t_avg) runs on the
  * device from per-case schedules, and with use_graph a 20-step chunk is

@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 from tests.helpers import channel128
+from tests.test_gpu_inplace import _kept
 from water_flow_simulation_amd import reach3d as R
 from water_flow_simulation_amd.batch import BatchRun
 from water_flow_simulation_amd.sweep import ensemble
@@ -190,3 +191,112 @@ def test_batch_budgets_match_sequential_bitwise():
     for run in seq:
         run.close()
     b.close()
+
+
+@pytest.mark.parametrize("storage,use_graph", [("f32", True), ("f16dev", True), ("f32", False)])
+def test_inplace_batch_matches_two_buffer_sequential_bitwise(storage, use_graph):
+    """Cases streamed in place share the batched launches: with uneven
+    budgets (odd ones leave the cases on different streaming phases for the
+    next chunk) every case ends exactly like a sequential two-buffer run of
+    its own length, with one population buffer per case."""
+    print(f"case: in-place batch of 5 reaches ({storage}, graph={use_graph}), budgets 61/35/100/47/81 "
+          "then 39/45/0/53/19", flush=True)
+    reaches = _cases()
+    rounds = [([61, 35, 100, 47, 81], 100), ([39, 45, 0, 53, 19], 60)]
+    seq = []
+    for r in reaches:
+        run = R.Run3D(r, storage=storage)
+        run.t_avg = 0.0
+        seq.append(run)
+    b = BatchRun(reaches, storage=storage, streaming="inplace")
+    assert all(run.lbm.inplace for run in b.runs)
+    for run in b.runs:
+        run.t_avg = 0.0
+    b.set_group(len(reaches))          # pushes the schedules again
+    for budgets, m in rounds:
+        for run, n in zip(seq, budgets):
+            if n:
+                run.advance(n)
+        b.set_budgets(budgets)
+        b.advance(m, use_graph=use_graph)
+        assert b.budgets_left() == [0] * len(reaches)
+        for k, (rs, rb) in enumerate(zip(seq, b.runs)):   # macroscopic fields are readable at any phase
+            fl = ~reaches[k].solid.ravel()
+            assert rb.lbm.nstep == rs.lbm.nstep, k
+            for a, c in zip(rs.lbm.macro(), rb.lbm.macro()):
+                assert np.array_equal(a[fl], c[fl]), k
+    b.set_budgets([None] * len(reaches))
+    for k, (rs, rb) in enumerate(zip(seq, b.runs)):
+        s_sums, s_navg, s_mac, s_pop = _state(rs)
+        b_sums, b_navg, b_mac, b_pop = _state(rb)      # an even number of steps: populations readable
+        assert s_navg == b_navg and rs.t == rb.t, k
+        fl = ~reaches[k].solid.ravel()
+        for name in s_sums:
+            assert np.array_equal(s_sums[name][fl], b_sums[name][fl]), (k, name)
+        kept = _kept(reaches[k].solid.shape) & np.tile(fl, 19)   # entries leaving through an x-end: never read
+        assert np.array_equal(s_pop[kept], b_pop[kept]), k
+    for run in seq:
+        run.close()
+    b.close()
+
+
+def test_batch_refuses_mixed_streaming_modes():
+    import ctypes
+    from water_flow_simulation_amd import _native as nat
+    print("case: batch of one two-buffer and one in-place reach", flush=True)
+    reaches, _ = ensemble(2, seed=3, scale=0.2)
+    runs = [R.Run3D(reaches[0]), R.Run3D(reaches[1], streaming="inplace")]
+    hs = (ctypes.c_void_p * 2)(*[run.lbm.handle.value for run in runs])
+    h = ctypes.c_void_p()
+    L = nat.lib()
+    assert L.wfs_batch_create(hs, 2, ctypes.byref(h)) != 0
+    assert b"streaming mode" in L.wfs_last_error()
+    for run in runs:
+        run.close()
+
+
+def test_inplace_batch_with_a_buffer_beyond_4_gib():
+    """A batch holding a case whose one population buffer exceeds 4 GiB
+    (FP64, 28.4 M cells) takes the 64-bit-address kernel for all its cases;
+    each case equals its own in-place solver stepped alone."""
+    import ctypes
+    from water_flow_simulation_amd import _native as nat
+    from water_flow_simulation_amd.lbm3d import LBM3D
+    nz, ny, nx = 300, 296, 320
+    print(f"case: batch of a {nx}x{ny}x{nz} channel (f64, buffer > 4 GiB) and the 128x32x16 channel, in place, "
+          "6 steps", flush=True)
+    assert 19 * nx * ny * nz * 8 >= 2 ** 32
+    big = np.zeros((nz, ny, nx), bool)
+    big[0] = True
+    big[:, :8, 100:120] = True
+    fb = np.zeros(big.shape, np.float32)
+    fb[150, 140:150, 60] = 1e-5
+    uin_b = np.where(big[:, :, 0], 0.0, 0.04).astype(np.float32)
+    from tests.helpers import solver_inputs
+    a = solver_inputs(channel128())
+
+    def make():
+        return [LBM3D(None, big, fb, fb, fb, uin_b, 0.51, storage="f64", streaming="inplace"),
+                LBM3D(None, a["solid"], a["Fx"], a["Fy"], a["Fz"], a["uin"], a["tau0"], a["cs"], a["cf_wall"],
+                      storage="f64", streaming="inplace")]
+
+    want = []
+    for g in make():
+        g.step(6)
+        want.append(g.macro(np.float64))
+        g.close()
+    gs = make()
+    hs = (ctypes.c_void_p * 2)(*[g.handle.value for g in gs])
+    h = ctypes.c_void_p()
+    L = nat.lib()
+    nat.check(L.wfs_batch_create(hs, 2, ctypes.byref(h)), "wfs_batch_create")
+    nat.check(L.wfs_batch_run(h, 6, 0), "wfs_batch_run")
+    nat.check(L.wfs_batch_synchronize(h), "wfs_batch_synchronize")
+    nat.check(L.wfs_batch_destroy(h), "wfs_batch_destroy")
+    for g, w, solid in zip(gs, want, (big, a["solid"])):
+        g.nstep = int(L.wfs_nstep(g.handle))
+        assert g.nstep == 6
+        fl = ~np.asarray(solid, bool).ravel()
+        for u, v in zip(w, g.macro(np.float64)):
+            assert np.array_equal(u[fl], v[fl])
+        g.close()

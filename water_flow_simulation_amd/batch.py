@@ -24,12 +24,15 @@ from .reach3d import Run3D
 
 
 class BatchRun:
-    def __init__(self, reaches, storage="f32", block_x=0, device=0, be=None, group=0, pop_stagger=0):
+    def __init__(self, reaches, storage="f32", block_x=0, device=0, be=None, group=0, pop_stagger=0,
+                 streaming="ab"):
         """block_x: work-group width (0: library default, "auto": autotuned on
         the largest case); group: cases per batched launch (0: all, "auto":
         autotuned); pop_stagger: bytes between each case's population
         direction arrays (a layout choice, results unchanged; "auto": tuned on
-        the largest case)."""
+        the largest case); streaming: "ab" (two population buffers per case)
+        or "inplace" (one; half the population memory, the same results bit
+        for bit)."""
         bx, ps = block_x, pop_stagger
         if block_x == "auto":
             from .autotune import block_x_for
@@ -37,7 +40,8 @@ class BatchRun:
         if pop_stagger == "auto":
             from .autotune import pop_stagger_for
             ps = pop_stagger_for(max(reaches, key=lambda r: r.cells), storage)
-        self.runs = [Run3D(r, be, storage=storage, block_x=bx, device=device, pop_stagger=ps) for r in reaches]
+        self.runs = [Run3D(r, be, storage=storage, block_x=bx, device=device, pop_stagger=ps, streaming=streaming)
+                     for r in reaches]
         self._hs = []
         self.group = 0
         self.steps_wall = 0.0

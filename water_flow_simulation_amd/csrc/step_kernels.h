@@ -1541,6 +1541,33 @@ k_step_batch(const CaseArgs<Codec>* __restrict__ cases, const unsigned short* __
     step_tile<Codec>(cs.f[par], cs.f[par ^ 1], a, row, tile);
 }
 
+// The batched step for cases streamed in place, each on its one population
+// buffer (f[0] == f[1]): the batch parity corrected by the case's flip is its
+// streaming phase, 0 for state S (the odd step) and 1 for the even step, so
+// cases stopped for an odd number of steps still take the right one. The
+// branch is uniform over the work-group. In-place cases have their outlet
+// row fused into the step. kPop32: every case's buffer is below 4 GiB.
+template <class Codec, bool kPop32>
+__global__ void __launch_bounds__(256)
+k_step_batch_aa(const CaseArgs<Codec>* __restrict__ cases, const unsigned short* __restrict__ group_case,
+                const int* __restrict__ af, int parity, int s) {
+    const unsigned g = blockIdx.x;
+    const int k = group_case[g];
+    const int act = af[2 * k];
+    if (s >= act) return;   // whole work-group: the case has stopped
+    const CaseArgs<Codec>& cs = cases[k];
+    const int phase = (parity ^ af[2 * k + 1]) & 1;
+    StepArgs<typename Codec::R> a = cs.a;
+    a.fscale = *cs.fscale;
+    a.write_all = s == act - 1 ? 1 : 0;
+    int row, tile;
+    if (!tile_of(g - cs.group_begin, a.tiles, a.rows, row, tile)) return;   // whole work-group
+    if (phase == 0)
+        step_tile_aa<Codec, true, kPop32, false>(cs.f[0], a, row, tile);
+    else
+        step_tile_aa<Codec, false, kPop32, false>(cs.f[0], a, row, tile);
+}
+
 // ---------------------------------------------------------------- inlet/outlet rows
 // Non-equilibrium extrapolation on the post-collision field, one work-item
 // per (y, z) row: inlet x = 0 takes rho from x = 1 and the prescribed

@@ -1502,6 +1502,8 @@ struct Batch final : BatchBase {
     hipGraphExec_t graph_exec = nullptr;
     bool dirty_sched = true;
     int bx = 128;                        // work-group width fixed when the batch is made
+    bool inplace = false;                // every case streams in place on one buffer
+    bool pop32 = true;                   // every case's population buffer is below 4 GiB
 
     ~Batch() override {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -1521,7 +1523,9 @@ struct Batch final : BatchBase {
             if (c->bx != bx0) return fail(WFS_EINVAL, "wfs_batch_create: cases must share block_x");
             if (c->tiled) return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the dense layout");
             if (c->colu_on) return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the run cadence's uref");
-            if (c->inplace) return fail(WFS_EINVAL, "wfs_batch_create: batched cases use two population buffers");
+            if (k == 0) inplace = c->inplace;
+            if (c->inplace != inplace)
+                return fail(WFS_EINVAL, "wfs_batch_create: cases must share the streaming mode (in place or two buffers)");
             if (c->closure != WFS_CLOSURE_SMAGORINSKY)
                 return fail(WFS_EINVAL, "wfs_batch_create: batched cases use the constant-Cs closure");
             int rs = c->ensure_sums();   // the batch accumulates into them
@@ -1533,7 +1537,8 @@ struct Batch final : BatchBase {
             int rc = c->fill_args(ca.a);
             if (rc) return rc;
             ca.f[0] = c->fp(c->cur);
-            ca.f[1] = c->fp(c->cur ^ 1);
+            ca.f[1] = c->inplace ? c->fp(c->cur) : c->fp(c->cur ^ 1);
+            pop32 = pop32 && ca.a.pop32;
             ca.fscale = nullptr;   // set below, once the schedule array exists
             ca.group_begin = (unsigned)gcase.size();
             const long long ng = (long long)((ca.a.rows + 7) / 8) * 8 * ca.a.tiles;
@@ -1552,7 +1557,7 @@ struct Batch final : BatchBase {
             sc.steps_left = -1;
             sc.steps_taken = 0;
             sc.active = 0;
-            sc.flip = 0;
+            sc.flip = c->inplace ? c->aa_phase : 0;   // in place: parity ^ flip is the streaming phase
             sched.push_back(sc);
         }
         groups = (unsigned)gcase.size();
@@ -1625,10 +1630,21 @@ struct Batch final : BatchBase {
         }
         const CaseArgs<Codec>* da = static_cast<const CaseArgs<Codec>*>(d_args.p);
         int par = parity0;
+        const unsigned short* gc = static_cast<const unsigned short*>(d_group_case.p);
         for (int s = 0; s < m; ++s) {
-            hipLaunchKernelGGL(k_step_batch<Codec>, dim3(groups), dim3(bx), 0, stream, da,
-                               static_cast<const unsigned short*>(d_group_case.p), static_cast<const int*>(af), par,
-                               s);
+            if (inplace) {   // outlet rows fused into the step (required for in-place streaming)
+                if (pop32)
+                    hipLaunchKernelGGL((k_step_batch_aa<Codec, true>), dim3(groups), dim3(bx), 0, stream, da, gc,
+                                       static_cast<const int*>(af), par, s);
+                else
+                    hipLaunchKernelGGL((k_step_batch_aa<Codec, false>), dim3(groups), dim3(bx), 0, stream, da, gc,
+                                       static_cast<const int*>(af), par, s);
+                WFS_LAUNCH_CHECK();
+                par ^= 1;
+                continue;
+            }
+            hipLaunchKernelGGL(k_step_batch<Codec>, dim3(groups), dim3(bx), 0, stream, da, gc,
+                               static_cast<const int*>(af), par, s);
             WFS_LAUNCH_CHECK();
             for (size_t k = 0; k < cases.size(); ++k) {
                 Solver<Codec>* c = cases[k];
@@ -1701,7 +1717,10 @@ struct Batch final : BatchBase {
             sched[k].steps_taken = 0;
             c->navg = sched[k].navg;
             c->fscale = sched[k].fscale;
-            c->cur = (args[k].f[(parity ^ sched[k].flip) & 1] == c->fp(0)) ? 0 : 1;
+            if (inplace)
+                c->aa_phase = (parity ^ sched[k].flip) & 1;
+            else
+                c->cur = (args[k].f[(parity ^ sched[k].flip) & 1] == c->fp(0)) ? 0 : 1;
         }
         dirty_sched = true;   // steps_taken cleared on the host copy
         return WFS_OK;

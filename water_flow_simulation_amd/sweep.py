@@ -68,11 +68,12 @@ def run_sequential(reaches, steps, storage="f32"):
     return runs, time.perf_counter() - t0, step_s
 
 
-def run_batched(reaches, steps, storage="f32", use_graph=True):
-    """All cases as one batch. Returns the batch, the wall time with the
-    set-up and the wall time of the stepping alone."""
+def run_batched(reaches, steps, storage="f32", use_graph=True, streaming="ab"):
+    """All cases as one batch (streaming: "ab" two population buffers per
+    case, "inplace" one). Returns the batch, the wall time with the set-up
+    and the wall time of the stepping alone."""
     t0 = time.perf_counter()
-    b = BatchRun(reaches, storage=storage)
+    b = BatchRun(reaches, storage=storage, streaming=streaming)
     t1 = time.perf_counter()
     b.advance(steps, use_graph=use_graph)
     return b, time.perf_counter() - t0, time.perf_counter() - t1
