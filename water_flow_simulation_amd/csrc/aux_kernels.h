@@ -199,6 +199,18 @@ __global__ void k_forced(uint8_t* __restrict__ flag, const int* __restrict__ fsl
     flag[c] = fl;
 }
 
+// cell_of for the column kernels, whose work-groups are whole 64-lane
+// x-tiles of one row (blockDim.x a multiple of 64, grid x-origin 0): every
+// lane of a wavefront reads the same tile-map entry, so the entry is read
+// once as a uniform value and a tile that is not stored is skipped by a
+// scalar branch.
+__device__ __forceinline__ long long column_cell(const CellMap& m, int x, long long j) {
+    if (!m.tmap) return x + (long long)m.px * j;
+    const int tx = __builtin_amdgcn_readfirstlane(x) >> 6;
+    const int t = __builtin_amdgcn_readfirstlane(m.tmap[tx + (long long)m.ntx * j]);
+    return t < 0 ? -1 : (long long)t * 64 + (x & 63);
+}
+
 // Depth-averaged column speed for the bed/bank wall model
 // (floodsim/reach3d.py:412-418): one work-item per (x, y) column, sums over
 // z = 0..nz-1 in order.
@@ -211,7 +223,7 @@ __device__ __forceinline__ void uref_column(const R* __restrict__ ux, const R* _
     if (x >= m.nx) return;
     R sx = R(0), sy = R(0), nfl = R(0);
     for (int z = 0; z < nz; ++z) {
-        const long long c = cell_of(m, x, y + (long long)m.ny * z);
+        const long long c = column_cell(m, x, y + (long long)m.ny * z);
         if (c < 0) continue;   // tile not stored: solid
         const R fl = (flag[c] & FLAG_SOLID) ? R(0) : R(1);
         sx += ux[c] * fl;
@@ -221,7 +233,7 @@ __device__ __forceinline__ void uref_column(const R* __restrict__ ux, const R* _
     const R ncol = nfl > R(1) ? nfl : R(1);
     const R U = sqrt(sx * sx + sy * sy) / ncol;
     for (int z = 0; z < nz; ++z) {
-        const long long c = cell_of(m, x, y + (long long)m.ny * z);
+        const long long c = column_cell(m, x, y + (long long)m.ny * z);
         if (c >= 0) uref[c] = (flag[c] & FLAG_BEDWALL) ? U : R(0);
     }
 }
